@@ -1,0 +1,139 @@
+"""Throughput of the fused RHS + SSP-RK3 step on one MI355X.
+
+Default workload: Lax-Liu Riemann configuration 3, 512x512, FP64, hybrid
+compact/WENO7 scheme, CFL 0.4, T 0.3 (fixed dt, 1343 steps for the full
+solve), sensor width 4, jump threshold 0.025, shear term off, compact-node
+hyperviscosity mn 0.001 every step, outflow guard 2 cells.  One "step" is a
+full RK3 step (3 fused stages: reductions, sensor, x/y sweeps with the compact
+solves, stage update and clean) plus the post-step hyperviscosity pass with
+a fresh sensor mask.  The state lives in HBM before timing starts.
+
+Prints one JSON line:  value = cell updates per second / 1e6 (whole job),
+roofline of the dominant kernel (the line sweep: algorithmic bytes per
+launch / average launch time from HIP events on its stream), and the
+NumPy oracle timed on a bounded sample on the host as cpu_baseline.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell-update of one full RK3 step, Riemann path with
+# hyperviscosity every step (dependency-minimal count: per stage one x-pass
+# reading q and writing dF/dx, one y-pass reading q, dF/dx, q0 and writing q,
+# the uint8 mask written and read once; plus the hyperviscosity pass).
+STEP_BYTES = {"fp64": 614.0, "fp32": 310.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--dtype", choices=("fp64", "fp32"), default="fp64")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the host oracle sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(nx, seconds):
+    """The NumPy oracle (the reference's algorithm, one core) on a bounded
+    sample of the same workload: Riemann config 3 at nx^2, steps from t=0."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import riemann as OR
+
+    run = OR.RiemannRun(OR.config3(nx=nx, ny=nx, shear_threshold=None, progress_every=10**9))
+    run.step()  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        run.step()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": round(nx * nx * n / el / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"NumPy oracle, Riemann C3 {nx}x{nx} FP64, {n} RK3 steps (+HV) after 1 warm-up step, "
+                      f"{el:.1f} s on 1 host core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is design-only "
+                                   "(instances are independent: shard the instance table)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannSolver
+
+    torch.cuda.set_device(0)
+    nx = args.nx
+    cfg = RiemannConfig3(nx=nx, ny=nx, cfl=0.4, tfinal=0.3, sensor_width=4, jump_threshold=0.025,
+                         shear_threshold=None, mn=0.001, hyperviscosity_interval=1, progress_every=10**9,
+                         dtype=args.dtype)
+    s = RiemannSolver(cfg)
+    for _ in range(args.warmup):
+        s.step(validate=False)
+    plan = s.rhs.plan
+    torch.cuda.synchronize()
+    plan.timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        s.step(validate=False)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    line_ms, line_n = plan.timing_read()
+    plan.timing(False)
+    ev_s = e0.elapsed_time(e1) / 1e3
+    T = max(wall, ev_s)
+    value = nx * nx * args.steps / T / 1e6
+    W = 8 if args.dtype == "fp64" else 4
+    # dominant kernel: one line sweep = read q (4W) + mask (1 B), write dF (4W) per cell
+    line_bytes = (8 * W + 1) * nx * nx
+    line_avg_s = line_ms / 1e3 / max(line_n, 1)
+    achieved = line_bytes / line_avg_s / 1e9
+    out = {
+        "metric": "Mcell-updates/s per RK3 step",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(T / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.dtype == "fp64" else "f32",
+        "data": "synthetic (Lax-Liu configuration 3 quadrant initial state)",
+        "config": {"workload": f"Riemann C3 {nx}x{nx} {args.dtype} hybrid compact/WENO7, CFL 0.4, "
+                               f"mn 0.001 HV every step, guard 2", "nx": nx, "ny": nx, "scheme": "hybrid",
+                   "steps_full_solve": s.n_steps, "dt": s.dt},
+        "roofline": {"bound": "hbm", "kernel": "k_line (x and y sweep)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "launches": line_n, "avg_launch_us": round(line_avg_s * 1e6, 2),
+                     "alg_bytes_per_launch": line_bytes},
+        "step_roofline": {"alg_bytes_per_cell_update": STEP_BYTES[args.dtype],
+                          "achieved_GBs": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9, 1),
+                          "frac": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(nx, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

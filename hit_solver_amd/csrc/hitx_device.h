@@ -1,0 +1,201 @@
+// Device-side building blocks of the hitx kernels (gfx950, wave64).
+//
+// Every floating-point expression below reproduces the operation order of the
+// reference NumPy code it cites, so that FP64 results agree bit-for-bit where
+// the algorithm is the same (pointwise algebra, stencils, WENO7) and to
+// rounding where it is not (line solves, reductions).  The library is built
+// with -ffp-contract=off: no multiply-add contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hitx {
+
+constexpr int kBlock = 256;  // threads per block (4 wave64s)
+constexpr int kWaves = kBlock / 64;
+
+// ---------------------------------------------------------------- parameters
+// Per-plan constants passed by value to every kernel.  T is the element type
+// of the state (double or float); thresholds compared against FP64
+// reductions stay double.
+template <class T>
+struct Params {
+  int nx, ny, periodic, weno_only;
+  T dx, dy, two_dx, two_dy;          // dx, dy and 2.0*dx, 2.0*dy
+  T gamma, gm1, rho_floor, p_floor, pf_gm1;  // gm1 = gamma-1.0, pf_gm1 = p_floor/(gamma-1.0)
+  // sensor
+  int width, bguard, shear_on;
+  double comp_thr, jump_thr, shear_thr;
+  // viscous terms
+  int visc_on;
+  T mu, two_mu, tt_mu, kcond, rgas;  // mu, 2.0*mu, (2.0/3.0)*mu, mu*cp/Pr, R
+  // hyperviscosity
+  T hv_coef[4];
+  T dx2, dy2;
+  int guard;
+  // sources
+  int drag_on, cool_on, force_on;
+  T neg_drag;                         // -drag_coef
+  double cool_pref, cool_tc, gamma_d; // cooling target pressure, time; gamma as double
+  int ndrag, nforce;
+  int kmx, kmy;                       // twiddle half-ranges (|k index| <= kmx, kmy)
+  const double* twx;                  // [(2*kmx+1)][nx][2]  cos, sin of 2*pi*k*i/nx
+  const double* twy;                  // [(2*kmy+1)][ny][2]
+  const int* drag_modes;              // [ndrag][2]
+  const int* force_modes;             // [nforce][2]
+};
+
+// Constants of the line solve along one axis (see solve_lines()).
+template <class T>
+struct LineK {
+  int n, L, nch, M;
+  T rho, c1p, inv1mrn, rnm1, one_m_rho2;  // rho, 1+rho^2, 1/(1-rho^n), rho^(n-1), 1-rho^2
+  T alpha_c;                              // compact alpha 3/8
+  T pw[66];                               // rho^k, k = 0..65 (0 beyond)
+};
+
+// Stage scalars written by the finalize kernel (double slots).
+enum {
+  S_EPS_R = 0, S_EPS_P, S_EPS_E, S_DIV_THR, S_COMP_ON, S_SHEAR_THR, S_SHEAR_ON,
+  S_ALPHA_X, S_ALPHA_Y, S_COOL, S_DRAG = 16,  // drag: [ndrag][2 comps][re, im] / N
+  S_COUNT = 16 + 4 * 24
+};
+
+// Stage reduction slots (block partials).
+enum { R_MAXR = 0, R_MAXP, R_MAXE, R_AX, R_AY, R_DIV2, R_VORT2, R_PSUM, R_DRAG };
+
+// ---------------------------------------------------------------- algebra
+// np.maximum: NaN-propagating maximum.
+template <class T>
+__device__ __forceinline__ T npmax(T a, T b) {
+  return (a != a) ? a : ((b != b) ? b : (a > b ? a : b));
+}
+
+template <class T>
+struct Prim {
+  T rho, u, v, p;
+};
+
+// OOP/equations.py:136-145 primitive_from_conservative
+template <class T>
+__device__ __forceinline__ Prim<T> prim(const Params<T>& P, T q0, T q1, T q2, T q3) {
+  Prim<T> r;
+  r.rho = npmax(q0, P.rho_floor);
+  r.u = q1 / r.rho;
+  r.v = q2 / r.rho;
+  T ke = (T(0.5) * r.rho) * (r.u * r.u + r.v * r.v);
+  T p = P.gm1 * (q3 - ke);
+  r.p = npmax(p, P.p_floor);
+  return r;
+}
+
+// OOP/equations.py:165-181 enforce_physical_state (in registers)
+template <class T>
+__device__ __forceinline__ void enforce(const Params<T>& P, T& q0, T& q1, T& q2, T& q3) {
+  T rho = npmax(q0, P.rho_floor);
+  T u = q1 / rho;
+  T v = q2 / rho;
+  T ke = (T(0.5) * rho) * (u * u + v * v);
+  T p = P.gm1 * (q3 - ke);
+  q0 = rho;
+  q1 = rho * u;
+  q2 = rho * v;
+  q3 = (p > P.p_floor) ? q3 : (P.pf_gm1 + ke);
+}
+
+template <class T>
+__device__ __forceinline__ void load4(const T* q, int N, int idx, T& a, T& b, T& c, T& d) {
+  a = q[idx];
+  b = q[N + idx];
+  c = q[2 * N + idx];
+  d = q[3 * N + idx];
+}
+
+// Primitives of q (enforce_first: of enforce(q)).
+template <class T>
+__device__ __forceinline__ Prim<T> prim_at(const Params<T>& P, const T* q, int N, int idx,
+                                           int enforce_first) {
+  T a, b, c, d;
+  load4(q, N, idx, a, b, c, d);
+  if (enforce_first) enforce(P, a, b, c, d);
+  return prim(P, a, b, c, d);
+}
+
+// ---------------------------------------------------------------- WENO7
+// OOP/spatial_operator.py:87-124 (IS0 carries the reference's 544).
+template <class T>
+__device__ __forceinline__ T weno7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
+  const T eps = T(1e-10);
+  T q0 = (((T(-(1.0 / 4.0)) * v1 + T(13.0 / 12.0) * v2) - T(23.0 / 12.0) * v3) + T(25.0 / 12.0) * v4);
+  T q1 = (((T(1.0 / 12.0) * v2 - T(5.0 / 12.0) * v3) + T(13.0 / 12.0) * v4) + T(1.0 / 4.0) * v5);
+  T q2 = (((T(-(1.0 / 12.0)) * v3 + T(7.0 / 12.0) * v4) + T(7.0 / 12.0) * v5) - T(1.0 / 12.0) * v6);
+  T q3 = (((T(1.0 / 4.0) * v4 + T(13.0 / 12.0) * v5) - T(5.0 / 12.0) * v6) + T(1.0 / 12.0) * v7);
+  T b0 = ((v1 * (((T(544) * v1 - T(3882) * v2) + T(4642) * v3) - T(1854) * v4)
+           + v2 * ((T(7043) * v2 - T(17246) * v3) + T(7042) * v4))
+          + v3 * (T(11003) * v3 - T(9402) * v4))
+         + T(2107) * (v4 * v4);
+  T b1 = ((v2 * (((T(267) * v2 - T(1642) * v3) + T(1602) * v4) - T(494) * v5)
+           + v3 * ((T(2843) * v3 - T(5966) * v4) + T(1922) * v5))
+          + v4 * (T(3443) * v4 - T(2522) * v5))
+         + T(547) * (v5 * v5);
+  T b2 = ((v3 * (((T(547) * v3 - T(2522) * v4) + T(1922) * v5) - T(494) * v6)
+           + v4 * ((T(3443) * v4 - T(5966) * v5) + T(1602) * v6))
+          + v5 * (T(2843) * v5 - T(1642) * v6))
+         + T(267) * (v6 * v6);
+  T b3 = ((v4 * (((T(2107) * v4 - T(9402) * v5) + T(7042) * v6) - T(1854) * v7)
+           + v5 * ((T(11003) * v5 - T(17246) * v6) + T(4642) * v7))
+          + v6 * (T(7043) * v6 - T(3882) * v7))
+         + T(547) * (v7 * v7);
+  T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2, e3 = eps + b3;
+  T a0 = T(1.0 / 35.0) / (e0 * e0);
+  T a1 = T(12.0 / 35.0) / (e1 * e1);
+  T a2 = T(18.0 / 35.0) / (e2 * e2);
+  T a3 = T(4.0 / 35.0) / (e3 * e3);
+  T asum = ((a0 + a1) + a2) + a3;
+  return (((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3) / asum;
+}
+
+// ---------------------------------------------------------------- reductions
+// Block-wide reduction of NR doubles; slot k is a max when bit k of MAXMASK
+// is set, else a sum.  Thread 0..NR-1 write the block's partials.
+template <int NR, uint64_t MAXMASK>
+__device__ __forceinline__ void block_reduce_write(double (&v)[NR], double* out) {
+  __shared__ double sh[kWaves][NR];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    double a = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double o = __shfl_xor(a, off, 64);
+      a = ((MAXMASK >> k) & 1) ? fmax(a, o) : a + o;
+    }
+    if (lane == 0) sh[w][k] = a;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NR; k += blockDim.x) {
+    double a = sh[0][k];
+    for (int j = 1; j < kWaves; ++j) a = ((MAXMASK >> k) & 1) ? fmax(a, sh[j][k]) : a + sh[j][k];
+    out[k] = a;
+  }
+}
+
+// Reduce nblk partial rows of NR values (row-major) with one block; result in
+// out[0..NR).  Fixed order: deterministic run to run.
+template <int NR, uint64_t MAXMASK>
+__device__ __forceinline__ void reduce_partials(const double* part, int nblk, double* out) {
+  double v[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      double a = part[(size_t)b * NR + k];
+      v[k] = ((MAXMASK >> k) & 1) ? fmax(v[k], a) : v[k] + a;
+    }
+  }
+  block_reduce_write<NR, MAXMASK>(v, out);
+}
+
+}  // namespace hitx

@@ -1,0 +1,872 @@
+// hitx kernels: stage reductions, shock sensor, line sweeps with the compact
+// solve, stage update with the HIT source closure, hyperviscosity, forcing.
+//
+// Layout in HBM: state q is [4][ny][nx] (x fastest), masks uint8 [ny][nx].
+// The y-direction sweep runs on a transposed copy qT [4][nx][ny] so both
+// sweeps stage whole lines through LDS with unit-stride global access.
+#pragma once
+
+#include "hitx_device.h"
+
+namespace hitx {
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+template <class T>
+__device__ __forceinline__ T absT(T a) { return fabs(a); }
+
+// ================================================================ reductions
+// Per-stage reductions that gate the sensor and the flux split
+// (OOP/spatial_operator.py:148 eps, :271 div_rms, :279 vort_rms, :332 alpha),
+// plus the HIT closure's cooling mean pressure (OOP/hit2d.py:423) and drag
+// projection onto the low-k modes (OOP/hit2d.py:391-394).  Sensor inputs use
+// enforce(q) when enforce_first (the operator rhs); cooling and drag use q as
+// given (the RK stage state), as the reference does.
+template <class T, bool DRAG>
+__global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* __restrict__ q,
+                                                         int enforce_first, double* __restrict__ part) {
+  constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
+  double v[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) v[k] = 0.0;
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+    const int y = idx / nx, x = idx - y * nx;
+    const Prim<T> c = prim_at(P, q, N, idx, enforce_first);
+    const T cs = sqrt((P.gamma * c.p) / c.rho);
+    v[R_MAXR] = fmax(v[R_MAXR], (double)absT(c.rho));
+    v[R_MAXP] = fmax(v[R_MAXP], (double)absT(c.p));
+    const T e = c.p / (c.rho * P.gm1);
+    v[R_MAXE] = fmax(v[R_MAXE], (double)absT(e));
+    v[R_AX] = fmax(v[R_AX], (double)(absT(c.u) + cs));
+    v[R_AY] = fmax(v[R_AY], (double)(absT(c.v) + cs));
+    T ux, uy, vx, vy;
+    if (P.periodic) {
+      const Prim<T> xr = prim_at(P, q, N, y * nx + wrap(x + 1, nx), enforce_first);
+      const Prim<T> xl = prim_at(P, q, N, y * nx + wrap(x - 1, nx), enforce_first);
+      const Prim<T> yr = prim_at(P, q, N, wrap(y + 1, ny) * nx + x, enforce_first);
+      const Prim<T> yl = prim_at(P, q, N, wrap(y - 1, ny) * nx + x, enforce_first);
+      ux = (xr.u - xl.u) / P.two_dx;
+      vx = (xr.v - xl.v) / P.two_dx;
+      uy = (yr.u - yl.u) / P.two_dy;
+      vy = (yr.v - yl.v) / P.two_dy;
+    } else {  // OOP/problems/riemann_config3.py:268-275
+      const int xa = x == 0 ? 0 : x - 1, xb = x == nx - 1 ? nx - 1 : x + 1;
+      const int ya = y == 0 ? 0 : y - 1, yb = y == ny - 1 ? ny - 1 : y + 1;
+      const Prim<T> xr = prim_at(P, q, N, y * nx + xb, enforce_first);
+      const Prim<T> xl = prim_at(P, q, N, y * nx + xa, enforce_first);
+      const Prim<T> yr = prim_at(P, q, N, yb * nx + x, enforce_first);
+      const Prim<T> yl = prim_at(P, q, N, ya * nx + x, enforce_first);
+      const bool xin = (x > 0 && x < nx - 1), yin = (y > 0 && y < ny - 1);
+      ux = xin ? (xr.u - xl.u) / P.two_dx : (xr.u - xl.u) / P.dx;
+      vx = xin ? (xr.v - xl.v) / P.two_dx : (xr.v - xl.v) / P.dx;
+      uy = yin ? (yr.u - yl.u) / P.two_dy : (yr.u - yl.u) / P.dy;
+      vy = yin ? (yr.v - yl.v) / P.two_dy : (yr.v - yl.v) / P.dy;
+    }
+    const T div = ux + vy;
+    v[R_DIV2] += (double)div * (double)div;
+    if (P.shear_on) {
+      const T w = vx - uy;
+      v[R_VORT2] += (double)w * (double)w;
+    }
+    if (P.cool_on || DRAG) {
+      T a, b, cc, d;
+      load4(q, N, idx, a, b, cc, d);
+      if (P.cool_on) v[R_PSUM] += (double)prim(P, a, b, cc, d).p;
+      if (DRAG) {
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+          const bool on = m < P.ndrag;
+          const int kx = on ? P.drag_modes[2 * m] : 0, ky = on ? P.drag_modes[2 * m + 1] : 0;
+          const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
+          const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + y) * 2;
+          const double cr = on ? tx[0] * ty[0] - tx[1] * ty[1] : 0.0;
+          const double si = on ? tx[1] * ty[0] + tx[0] * ty[1] : 0.0;
+          v[R_DRAG + 4 * m + 0] += (double)b * cr;
+          v[R_DRAG + 4 * m + 1] -= (double)b * si;
+          v[R_DRAG + 4 * m + 2] += (double)cc * cr;
+          v[R_DRAG + 4 * m + 3] -= (double)cc * si;
+        }
+      }
+    }
+  }
+  block_reduce_write<NR, 0x1Full>(v, part + (size_t)blockIdx.x * NR);
+}
+
+template <class T, bool DRAG>
+__global__ void __launch_bounds__(kBlock) k_stage_finalize(Params<T> P, const double* __restrict__ part,
+                                                           int nblk, double* __restrict__ scal) {
+  constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
+  __shared__ double r[NR];
+  reduce_partials<NR, 0x1Full>(part, nblk, r);
+  __syncthreads();
+  const double N = (double)P.nx * (double)P.ny;
+  if (threadIdx.x == 0) {
+    scal[S_EPS_R] = 1e-14 + 1e-12 * r[R_MAXR];
+    scal[S_EPS_P] = 1e-14 + 1e-12 * r[R_MAXP];
+    scal[S_EPS_E] = 1e-14 + 1e-12 * r[R_MAXE];
+    const double div_rms = sqrt(r[R_DIV2] / N);
+    scal[S_COMP_ON] = div_rms > 1e-12 ? 1.0 : 0.0;
+    scal[S_DIV_THR] = -P.comp_thr * div_rms;
+    const double w_rms = sqrt(r[R_VORT2] / N);
+    scal[S_SHEAR_ON] = (P.shear_on && w_rms > 1e-12) ? 1.0 : 0.0;
+    scal[S_SHEAR_THR] = P.shear_thr * w_rms;
+    scal[S_ALPHA_X] = r[R_AX];
+    scal[S_ALPHA_Y] = r[R_AY];
+    if (P.cool_on) {  // OOP/hit2d.py:422-425
+      const double err = (r[R_PSUM] / N - P.cool_pref) / (P.gamma_d - 1.0);
+      scal[S_COOL] = -err / P.cool_tc;
+    } else {
+      scal[S_COOL] = 0.0;
+    }
+  }
+  if (DRAG)
+    for (int k = threadIdx.x; k < 4 * P.ndrag; k += blockDim.x) scal[S_DRAG + k] = r[R_DRAG + k] / N;
+}
+
+// Generic column reduction: out[k] = sum over nblk rows of part[row*nr + k]
+// (one block per slot k; fixed order).
+__global__ void __launch_bounds__(kBlock) k_finalize_sum(const double* __restrict__ part, int nblk, int nr,
+                                                         double* __restrict__ out) {
+  const int k = blockIdx.x;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v[0] += part[(size_t)b * nr + k];
+  __shared__ double r[1];
+  block_reduce_write<1, 0>(v, r);
+  __syncthreads();
+  if (threadIdx.x == 0) out[k] = r[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize_max0(const double* __restrict__ part, int nblk, int nr,
+                                                          double* __restrict__ out) {
+  // slot 0 is a max, the others sums (step reductions)
+  const int k = blockIdx.x;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const double a = part[(size_t)b * nr + k];
+    v[0] = k == 0 ? fmax(v[0], a) : v[0] + a;
+  }
+  __shared__ double r[1];
+  if (k == 0)
+    block_reduce_write<1, 1>(v, r);
+  else
+    block_reduce_write<1, 0>(v, r);
+  __syncthreads();
+  if (threadIdx.x == 0) out[k] = r[0];
+}
+
+// Step-level reductions: max spectral radius (OOP/spatial_operator.py:369-377,
+// OOP/problems/riemann_config3.py:371-378), sum(u^2+v^2) and sum(c) for the
+// turbulent Mach number (OOP/hit2d.py:279-290).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_step_reduce(Params<T> P, const T* __restrict__ q,
+                                                        double* __restrict__ part) {
+  double v[3] = {0.0, 0.0, 0.0};
+  const int N = P.nx * P.ny;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+    const Prim<T> c = prim_at(P, q, N, idx, 0);
+    const T cs = sqrt((P.gamma * c.p) / c.rho);
+    T sr = (absT(c.u) + cs) / P.dx;
+    sr = sr + (absT(c.v) + cs) / P.dy;
+    v[0] = fmax(v[0], (double)sr);
+    v[1] += (double)(c.u * c.u + c.v * c.v);
+    v[2] += (double)cs;
+  }
+  block_reduce_write<3, 1>(v, part + (size_t)blockIdx.x * 3);
+}
+
+// ================================================================ sensor
+// Relative jump of f at a node along one axis: periodic form
+// OOP/spatial_operator.py:147-166, non-periodic form
+// OOP/problems/riemann_config3.py:128-146 (pair jumps credited to both nodes,
+// clamped curvature).  fl/fr are the left/right neighbours (clamped in the
+// non-periodic case), at_lo/at_hi flag the domain ends.
+template <class T>
+__device__ __forceinline__ T rel_jump(T fl, T f, T fr, T eps, bool periodic, bool at_lo, bool at_hi) {
+  const T cv = absT((fr - T(2.0) * f) + fl) / (((absT(fr) + T(2.0) * absT(f)) + absT(fl)) + eps);
+  if (periodic) {
+    const T jr = absT(fr - f) / ((absT(fr) + absT(f)) + eps);
+    const T jl = absT(f - fl) / ((absT(f) + absT(fl)) + eps);
+    return npmax(npmax(jl, jr), cv);
+  }
+  T j = T(0);
+  if (!at_hi) j = npmax(j, absT(fr - f) / ((absT(fr) + absT(f)) + eps));
+  if (!at_lo) j = npmax(j, absT(f - fl) / ((absT(f) + absT(fl)) + eps));
+  return npmax(j, cv);
+}
+
+// Undilated sensor mask (OOP/spatial_operator.py:263-303,
+// OOP/problems/riemann_config3.py:288-325 including the boundary band).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_rawmask(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                    const double* __restrict__ scal, uint8_t* __restrict__ raw) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= nx || y >= ny) return;
+  const bool per = P.periodic;
+  const int xl = per ? wrap(x - 1, nx) : (x == 0 ? 0 : x - 1);
+  const int xr = per ? wrap(x + 1, nx) : (x == nx - 1 ? nx - 1 : x + 1);
+  const int yl = per ? wrap(y - 1, ny) : (y == 0 ? 0 : y - 1);
+  const int yr = per ? wrap(y + 1, ny) : (y == ny - 1 ? ny - 1 : y + 1);
+  const Prim<T> c = prim_at(P, q, N, y * nx + x, enforce_first);
+  const Prim<T> pxl = prim_at(P, q, N, y * nx + xl, enforce_first);
+  const Prim<T> pxr = prim_at(P, q, N, y * nx + xr, enforce_first);
+  const Prim<T> pyl = prim_at(P, q, N, yl * nx + x, enforce_first);
+  const Prim<T> pyr = prim_at(P, q, N, yr * nx + x, enforce_first);
+  bool m = false;
+  // compression (and shear) from central / one-sided velocity gradients
+  T ux, uy, vx, vy;
+  const bool xin = per || (x > 0 && x < nx - 1), yin = per || (y > 0 && y < ny - 1);
+  ux = xin ? (pxr.u - pxl.u) / P.two_dx : (pxr.u - pxl.u) / P.dx;
+  vx = xin ? (pxr.v - pxl.v) / P.two_dx : (pxr.v - pxl.v) / P.dx;
+  uy = yin ? (pyr.u - pyl.u) / P.two_dy : (pyr.u - pyl.u) / P.dy;
+  vy = yin ? (pyr.v - pyl.v) / P.two_dy : (pyr.v - pyl.v) / P.dy;
+  const T div = ux + vy;
+  if (scal[S_COMP_ON] != 0.0 && (double)div < scal[S_DIV_THR]) m = true;
+  if (scal[S_SHEAR_ON] != 0.0) {
+    const T w = vx - uy;
+    if ((double)absT(w) > scal[S_SHEAR_THR]) m = true;
+  }
+  const bool xlo = !per && x == 0, xhi = !per && x == nx - 1, ylo = !per && y == 0, yhi = !per && y == ny - 1;
+  const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
+  const T jr = npmax(rel_jump(pxl.rho, c.rho, pxr.rho, er, per, xlo, xhi),
+                     rel_jump(pyl.rho, c.rho, pyr.rho, er, per, ylo, yhi));
+  const T jp = npmax(rel_jump(pxl.p, c.p, pxr.p, ep, per, xlo, xhi),
+                     rel_jump(pyl.p, c.p, pyr.p, ep, per, ylo, yhi));
+  const T e_c = c.p / (c.rho * P.gm1), e_xl = pxl.p / (pxl.rho * P.gm1), e_xr = pxr.p / (pxr.rho * P.gm1);
+  const T e_yl = pyl.p / (pyl.rho * P.gm1), e_yr = pyr.p / (pyr.rho * P.gm1);
+  const T je = npmax(rel_jump(e_xl, e_c, e_xr, ee, per, xlo, xhi), rel_jump(e_yl, e_c, e_yr, ee, per, ylo, yhi));
+  const double thr = P.jump_thr;
+  if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+  if (!per && P.bguard > 0) {
+    const int g = P.bguard;
+    if (y < g || y >= ny - g || x < g || x >= nx - g) m = true;
+  }
+  raw[y * nx + x] = m ? 1 : 0;
+}
+
+// (2w+1)^2 OR-dilation (periodic wrap, or truncated at the edges), written
+// both as mask [ny][nx] and transposed maskT [nx][ny].  Tile 32x32.
+__global__ void __launch_bounds__(kBlock) k_dilate(int nx, int ny, int w, int periodic,
+                                                   const uint8_t* __restrict__ raw, uint8_t* __restrict__ mask,
+                                                   uint8_t* __restrict__ maskT) {
+  constexpr int TS = 32, HMAX = 8, E = TS + 2 * HMAX;
+  __shared__ uint8_t in[E][E];
+  __shared__ uint8_t mid[E][TS];
+  __shared__ uint8_t outt[TS][TS + 1];
+  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS, ext = TS + 2 * w;
+  for (int t = threadIdx.x; t < ext * ext; t += kBlock) {
+    const int r = t / ext, c = t - r * ext;
+    int gy = y0 - w + r, gx = x0 - w + c;
+    uint8_t val = 0;
+    if (periodic) {
+      gy = ((gy % ny) + ny) % ny;
+      gx = ((gx % nx) + nx) % nx;
+      val = raw[gy * nx + gx];
+    } else if (gy >= 0 && gy < ny && gx >= 0 && gx < nx) {
+      val = raw[gy * nx + gx];
+    }
+    in[r][c] = val;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ext * TS; t += kBlock) {
+    const int r = t / TS, c = t - r * TS;
+    uint8_t o = 0;
+    for (int d = 0; d <= 2 * w; ++d) o |= in[r][c + d];
+    mid[r][c] = o;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < TS * TS; t += kBlock) {
+    const int r = t / TS, c = t - r * TS;
+    uint8_t o = 0;
+    for (int d = 0; d <= 2 * w; ++d) o |= mid[r + d][c];
+    outt[r][c] = o;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) mask[gy * nx + gx] = o;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < TS * TS; t += kBlock) {
+    const int c = t / TS, r = t - c * TS;  // consecutive threads -> consecutive y
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) maskT[(size_t)gx * ny + gy] = outt[r][c];
+  }
+}
+
+// [4][ny][nx] -> [4][nx][ny], 32x32 tiles through LDS.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_transpose4(int nx, int ny, const T* __restrict__ a, T* __restrict__ at) {
+  __shared__ T tile[32][33];
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const size_t N = (size_t)nx * ny;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int c = 0; c < 4; ++c) {
+    for (int r = ty; r < 32; r += 8) {
+      const int gy = y0 + r, gx = x0 + tx;
+      if (gy < ny && gx < nx) tile[r][tx] = a[c * N + (size_t)gy * nx + gx];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      const int gx = x0 + r, gy = y0 + tx;
+      if (gy < ny && gx < nx) at[c * N + (size_t)gx * ny + gy] = tile[tx][r];
+    }
+    __syncthreads();
+  }
+}
+
+// ================================================================ line sweep
+// Power of rho from the LDS table (zero beyond the table: |rho|^66 < 1e-22).
+template <class T>
+__device__ __forceinline__ T pwk(const T* PW, int k) { return k <= 65 ? PW[k] : T(0); }
+
+// Solve the four (alpha, 1, alpha) systems held in r[c*S + i], i < n, in place.
+// Periodic: cyclic matrix (OOP/spatial_operator.py:227-249 solves it by FFT).
+// Non-periodic: identity end rows (OOP/problems/riemann_config3.py:237-265,
+// Thomas).  Method: A = K (1 - rho S^-1)(1 - rho S), K = 1/(1+rho^2), so
+// A^-1 r is a causal then an anti-causal first-order recurrence with
+// coefficient rho = (-1 + sqrt(1 - 4 alpha^2)) / (2 alpha) = -0.4514.  Each
+// recurrence runs as a chunked scan: every thread sweeps one chunk of L
+// points with zero carry-in, then adds rho^k times the exact carry, which is a
+// short geometric sum over the preceding chunks' local results (|rho|^52 <
+// 1e-18 truncates it; the periodic wrap is summed in closed form).  The
+// non-periodic system is the zero-Dirichlet interior problem plus the
+// homogeneous solutions a rho^i + b rho^(n-1-i) that restore the end rows.
+template <class T, bool PER>
+__device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scratch) {
+  const int n = K.n, L = K.L, nch = K.nch, M = K.M;
+  T* e1 = scratch;             // [4][nch]
+  T* e2 = scratch + 4 * nch;   // [4][nch]
+  T* bnd = scratch + 8 * nch;  // [4][5]: r0, r_{n-1}, z_n, p0, p_{n-1}
+  const T rho = K.rho;
+  const int ntask = 4 * nch;
+  if (!PER) {
+    if (threadIdx.x < 4) {
+      bnd[threadIdx.x * 5 + 0] = r[threadIdx.x * S];
+      bnd[threadIdx.x * 5 + 1] = r[threadIdx.x * S + n - 1];
+    }
+    __syncthreads();
+  }
+  // causal pass, local
+  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
+    T* rl = r + line * S;
+    T acc = T(0);
+    for (int k = 0; k < L; ++k) {
+      const int i = s0 + k;
+      T xv = rl[i];
+      if (!PER) {
+        if (i == 0 || i == n - 1) {
+          xv = T(0);
+        } else {
+          if (i == 1) xv = xv - K.alpha_c * bnd[line * 5 + 0];
+          if (i == n - 2) xv = xv - K.alpha_c * bnd[line * 5 + 1];
+        }
+      }
+      acc = xv + rho * acc;
+      rl[i] = acc;
+    }
+    e1[line * nch + ch] = acc;
+  }
+  __syncthreads();
+  // causal pass, carries
+  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
+    T* rl = r + line * S;
+    const T* el = e1 + line * nch;
+    T carry = T(0);
+    if (PER) {
+      const int mm = nch < M ? nch : M;
+      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[(ch - m + nch) % nch];
+      carry = carry * K.inv1mrn;
+    } else {
+      const int mm = ch < M ? ch : M;
+      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch - m];
+    }
+    for (int k = 0; k < L; ++k) rl[s0 + k] = rl[s0 + k] + pwk(PW, k + 1) * carry;
+    if (!PER && ch == nch - 1) bnd[line * 5 + 2] = (rho * rl[n - 1]) / K.one_m_rho2;
+  }
+  __syncthreads();
+  // anti-causal pass, local
+  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
+    T* rl = r + line * S;
+    T acc = T(0);
+    for (int k = L - 1; k >= 0; --k) {
+      acc = rl[s0 + k] + rho * acc;
+      rl[s0 + k] = acc;
+    }
+    e2[line * nch + ch] = acc;
+  }
+  __syncthreads();
+  // anti-causal pass, carries, scale by 1 + rho^2
+  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
+    T* rl = r + line * S;
+    const T* el = e2 + line * nch;
+    T carry = T(0);
+    if (PER) {
+      const int mm = nch < M ? nch : M;
+      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[(ch + m) % nch];
+      carry = carry * K.inv1mrn;
+    } else {
+      const int rem = nch - 1 - ch;
+      const int mm = rem < M ? rem : M;
+      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch + m];
+      if (rem < M) carry = carry + pwk(PW, rem * L) * bnd[line * 5 + 2];
+    }
+    for (int k = 0; k < L; ++k) rl[s0 + k] = K.c1p * (rl[s0 + k] + pwk(PW, L - k) * carry);
+    if (!PER) {
+      if (ch == 0) bnd[line * 5 + 3] = rl[0];
+      if (ch == nch - 1) bnd[line * 5 + 4] = rl[n - 1];
+    }
+  }
+  __syncthreads();
+  if (!PER) {
+    for (int t = threadIdx.x; t < ntask; t += kBlock) {
+      const int line = t / nch, ch = t - line * nch, s0 = ch * L;
+      T* rl = r + line * S;
+      const T p0 = bnd[line * 5 + 3], pn = bnd[line * 5 + 4];
+      const T det = T(1) - K.rnm1 * K.rnm1;
+      const T a = (-p0 + pn * K.rnm1) / det, b = (-pn + p0 * K.rnm1) / det;
+      for (int k = 0; k < L; ++k) {
+        const int i = s0 + k;
+        if (i == 0)
+          rl[0] = bnd[line * 5 + 0];
+        else if (i == n - 1)
+          rl[n - 1] = bnd[line * 5 + 1];
+        else
+          rl[i] = (rl[i] + a * pwk(PW, i)) + b * pwk(PW, n - 1 - i);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One block per line: point fluxes, hybrid interface flux (smooth compact
+// flux, or compactified WENO7 of the Lax-Friedrichs split where the
+// interface is flagged), divided difference, compact solve.
+// axis 0: x-lines of q [4][ny][nx], flux_x, alpha_x, dx.
+// axis 1: y-lines of qT [4][nx][ny], flux_y, alpha_y, dy.
+// OOP/spatial_operator.py:179-249, :321-342; non-periodic
+// OOP/problems/riemann_config3.py:165-265, :344-357.
+template <class T, bool PER>
+__global__ void __launch_bounds__(kBlock) k_line(Params<T> P, LineK<T> K, int axis, const T* __restrict__ ql,
+                                                 const uint8_t* __restrict__ ml, const double* __restrict__ scal,
+                                                 T* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = K.n, S = n + 1, line = blockIdx.x;
+  T* A = reinterpret_cast<T*>(smem);
+  T* B = A + 4 * S;
+  T* C = B + 4 * S;
+  T* PW = C + 4 * S;                // 66
+  T* scratch = PW + 66;             // 8*nch + 20
+  uint8_t* Mk = reinterpret_cast<uint8_t*>(scratch + 8 * K.nch + 20);
+  const size_t NL = (size_t)gridDim.x * n;
+  const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
+  const T h = axis == 0 ? P.dx : P.dy;
+  for (int k = threadIdx.x; k < 66; k += kBlock) PW[k] = K.pw[k];
+  // 1. enforce, primitives, point flux
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    const size_t g = (size_t)line * n + i;
+    T a = ql[g], b = ql[NL + g], c = ql[2 * NL + g], d = ql[3 * NL + g];
+    enforce(P, a, b, c, d);
+    const Prim<T> pr = prim(P, a, b, c, d);
+    A[i] = a;
+    A[S + i] = b;
+    A[2 * S + i] = c;
+    A[3 * S + i] = d;
+    if (axis == 0) {  // OOP/equations.py:183-190
+      B[i] = b;
+      B[S + i] = pr.rho * (pr.u * pr.u) + pr.p;
+      B[2 * S + i] = (pr.rho * pr.u) * pr.v;
+      B[3 * S + i] = (d + pr.p) * pr.u;
+    } else {  // OOP/equations.py:192-199
+      B[i] = c;
+      B[S + i] = (pr.rho * pr.u) * pr.v;
+      B[2 * S + i] = pr.rho * (pr.v * pr.v) + pr.p;
+      B[3 * S + i] = (d + pr.p) * pr.v;
+    }
+    Mk[i] = ml[g];
+  }
+  __syncthreads();
+  const int nI = PER ? n : n + 1;
+  auto iflag = [&](int k) -> bool {
+    if (PER) {
+      k = wrap(k, n);
+      return (Mk[k] | Mk[wrap(k + 1, n)]) != 0;
+    }
+    k = clampi(k, 0, n);
+    return (Mk[clampi(k - 1, 0, n - 1)] | Mk[clampi(k, 0, n - 1)]) != 0;
+  };
+  auto nidx = [&](int j) -> int { return PER ? wrap(j, n) : clampi(j, 0, n - 1); };
+  // 2. WENO7 interface flux where a flagged interface's compactification uses it
+  for (int k = threadIdx.x; k < nI; k += kBlock) {
+    if (!(P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1))) continue;
+    const int il = PER ? k : k - 1;
+    for (int c = 0; c < 4; ++c) {
+      const T* qa = A + c * S;
+      const T* fa = B + c * S;
+      T fp[7], fm[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int ip = nidx(il - 3 + j);   // f+ stencil i-3 .. i+3
+        const int im = nidx(il + 4 - j);   // f- stencil i+4 .. i-2
+        fp[j] = T(0.5) * (fa[ip] + alpha * qa[ip]);
+        fm[j] = T(0.5) * (fa[im] - alpha * qa[im]);
+      }
+      C[c * S + k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
+                     weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
+    }
+  }
+  __syncthreads();
+  // 3. hybrid interface flux into A
+  const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
+          c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
+  for (int k = threadIdx.x; k < nI; k += kBlock) {
+    const bool fl = P.weno_only || iflag(k);
+    const int il = PER ? k : k - 1;
+    const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
+    const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
+    for (int c = 0; c < 4; ++c) {
+      T F;
+      if (fl) {
+        const T* w = C + c * S;
+        F = (K.alpha_c * w[kr] + w[k]) + K.alpha_c * w[kl];
+      } else {
+        const T* f = B + c * S;
+        F = (c3 * (f[nidx(il + 3)] + f[nidx(il - 2)]) + c2 * (f[nidx(il + 2)] + f[nidx(il - 1)])) +
+            c1 * (f[nidx(il + 1)] + f[nidx(il)]);
+      }
+      A[c * S + k] = F;
+    }
+  }
+  __syncthreads();
+  // 4. divided difference into B
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    for (int c = 0; c < 4; ++c) {
+      const T* F = A + c * S;
+      B[c * S + i] = PER ? (F[i] - F[wrap(i - 1, n)]) / h : (F[i + 1] - F[i]) / h;
+    }
+  }
+  __syncthreads();
+  // 5. compact solve, 6. store
+  solve_lines<T, PER>(B, S, K, PW, scratch);
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    const size_t g = (size_t)line * n + i;
+    for (int c = 0; c < 4; ++c) out[c * NL + g] = B[c * S + i];
+  }
+}
+
+// ================================================================ stage update
+// Periodic Navier-Stokes terms at one node (OOP/equations.py:232-267): nested
+// second-order central differences on the 13-point diamond.
+template <class T>
+__device__ void viscous_at(const Params<T>& P, const T* __restrict__ q, int y, int x, T out[4]) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  T U[5][5], V[5][5], Tt[5][5];
+#pragma unroll
+  for (int a = -2; a <= 2; ++a)
+#pragma unroll
+    for (int b = -2; b <= 2; ++b) {
+      if ((a < 0 ? -a : a) + (b < 0 ? -b : b) > 2) continue;
+      const Prim<T> pr = prim_at(P, q, N, wrap(y + a, ny) * nx + wrap(x + b, nx), 0);
+      U[a + 2][b + 2] = pr.u;
+      V[a + 2][b + 2] = pr.v;
+      Tt[a + 2][b + 2] = pr.p / (pr.rho * P.rgas);
+    }
+  // derivatives at node (a, b) relative to the centre
+  auto ux = [&](int a, int b) { return (U[a + 2][b + 3] - U[a + 2][b + 1]) / P.two_dx; };
+  auto uy = [&](int a, int b) { return (U[a + 3][b + 2] - U[a + 1][b + 2]) / P.two_dy; };
+  auto vx = [&](int a, int b) { return (V[a + 2][b + 3] - V[a + 2][b + 1]) / P.two_dx; };
+  auto vy = [&](int a, int b) { return (V[a + 3][b + 2] - V[a + 1][b + 2]) / P.two_dy; };
+  auto txx = [&](int a, int b) { return P.two_mu * ux(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
+  auto tyy = [&](int a, int b) { return P.two_mu * vy(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
+  auto txy = [&](int a, int b) { return P.mu * (uy(a, b) + vx(a, b)); };
+  auto ex = [&](int a, int b) {
+    const T tx = (Tt[a + 2][b + 3] - Tt[a + 2][b + 1]) / P.two_dx;
+    return (U[a + 2][b + 2] * txx(a, b) + V[a + 2][b + 2] * txy(a, b)) + P.kcond * tx;
+  };
+  auto ey = [&](int a, int b) {
+    const T ty = (Tt[a + 3][b + 2] - Tt[a + 1][b + 2]) / P.two_dy;
+    return (U[a + 2][b + 2] * txy(a, b) + V[a + 2][b + 2] * tyy(a, b)) + P.kcond * ty;
+  };
+  out[0] = T(0);
+  out[1] = (txx(0, 1) - txx(0, -1)) / P.two_dx + (txy(1, 0) - txy(-1, 0)) / P.two_dy;
+  out[2] = (txy(0, 1) - txy(0, -1)) / P.two_dx + (tyy(1, 0) - tyy(-1, 0)) / P.two_dy;
+  out[3] = (ex(0, 1) - ex(0, -1)) / P.two_dx + (ey(1, 0) - ey(-1, 0)) / P.two_dy;
+}
+
+enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
+
+// rhs = -dF/dx - dG/dy (OOP/spatial_operator.py:367); in MODE_STAGE also the
+// HIT closure (OOP/hit2d.py:847-867: + viscous + forcing + drag + cooling),
+// the Shu-Osher combination (OOP/time_operator.py:24-26) and clean =
+// enforce (+ outflow guard: every node takes the value of its clamped
+// interior source, OOP/problems/riemann_config3.py:419-429).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int stage, const T* __restrict__ q0,
+                                                   const T* __restrict__ qs, const T* __restrict__ dfx,
+                                                   const T* __restrict__ dfyT, const T* __restrict__ fx,
+                                                   const T* __restrict__ fy, const double* __restrict__ scal, T dt,
+                                                   T* __restrict__ out) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= nx || y >= ny) return;
+  int cx = x, cy = y;
+  if (P.guard > 0) {
+    cx = clampi(x, P.guard, nx - 1 - P.guard);
+    cy = clampi(y, P.guard, ny - 1 - P.guard);
+  }
+  const int idx = cy * nx + cx, oidx = y * nx + x;
+  const size_t tidx = (size_t)cx * ny + cy;
+  if (mode == MODE_VISC) {  // standalone viscous_rhs of qs
+    T vv[4];
+    viscous_at(P, qs, cy, cx, vv);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c * N + oidx] = vv[c];
+    return;
+  }
+  T Lv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) Lv[c] = (-dfx[c * N + idx]) - dfyT[c * N + tidx];
+  if (mode == MODE_RHS) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c * N + oidx] = Lv[c];
+    return;
+  }
+  T s0, s1, s2, s3;
+  load4(qs, N, idx, s0, s1, s2, s3);
+  if (P.visc_on) {
+    T vv[4];
+    viscous_at(P, qs, cy, cx, vv);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Lv[c] = Lv[c] + vv[c];
+  }
+  if (P.force_on || P.drag_on) {
+    const Prim<T> st = prim(P, s0, s1, s2, s3);
+    if (P.force_on) {  // OOP/hit2d.py:264-276
+      const T f1 = fx[idx], f2 = fy[idx];
+      Lv[1] = Lv[1] + st.rho * f1;
+      Lv[2] = Lv[2] + st.rho * f2;
+      Lv[3] = Lv[3] + st.rho * (st.u * f1 + st.v * f2);
+    }
+    if (P.drag_on) {  // OOP/hit2d.py:370-399
+      double lx = 0.0, ly = 0.0;
+      for (int m = 0; m < P.ndrag; ++m) {
+        const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
+        const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + cx) * 2;
+        const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + cy) * 2;
+        const double cr = tx[0] * ty[0] - tx[1] * ty[1];
+        const double si = tx[1] * ty[0] + tx[0] * ty[1];
+        const double* A = scal + S_DRAG + 4 * m;
+        lx += A[0] * cr - A[1] * si;
+        ly += A[2] * cr - A[3] * si;
+      }
+      const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
+      Lv[1] = Lv[1] + d1;
+      Lv[2] = Lv[2] + d2;
+      Lv[3] = Lv[3] + (st.u * d1 + st.v * d2);
+    }
+  }
+  if (P.cool_on) Lv[3] = Lv[3] + (T)scal[S_COOL];  // OOP/hit2d.py:402-426
+  T o[4];
+  const T qsv[4] = {s0, s1, s2, s3};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const T a0 = q0[c * N + idx];
+    if (stage == 0)
+      o[c] = a0 + dt * Lv[c];
+    else if (stage == 1)
+      o[c] = T(0.75) * a0 + T(0.25) * (qsv[c] + dt * Lv[c]);
+    else
+      o[c] = T(1.0 / 3.0) * a0 + T(2.0 / 3.0) * (qsv[c] + dt * Lv[c]);
+  }
+  enforce(P, o[0], o[1], o[2], o[3]);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c * N + oidx] = o[c];
+}
+
+// ================================================================ hyperviscosity
+// q - mn*w*h^4*active*lap(lap(q)), then enforce (+ guard).  Periodic
+// Laplacian with wrap (OOP/spatial_operator.py:394-435) or edge-padded
+// (OOP/problems/riemann_config3.py:390-416).
+template <class T>
+__device__ __forceinline__ T lap_at(const Params<T>& P, const T* __restrict__ a, int y, int x) {
+  const int nx = P.nx, ny = P.ny;
+  int xl, xr, yl, yr;
+  if (P.periodic) {
+    xl = wrap(x - 1, nx); xr = wrap(x + 1, nx); yl = wrap(y - 1, ny); yr = wrap(y + 1, ny);
+  } else {
+    xl = clampi(x - 1, 0, nx - 1); xr = clampi(x + 1, 0, nx - 1);
+    yl = clampi(y - 1, 0, ny - 1); yr = clampi(y + 1, 0, ny - 1);
+  }
+  const T c = a[y * nx + x];
+  const T lx = ((a[y * nx + xr] - T(2.0) * c) + a[y * nx + xl]) / P.dx2;
+  const T ly = ((a[yr * nx + x] - T(2.0) * c) + a[yl * nx + x]) / P.dy2;
+  return lx + ly;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict__ q, const uint8_t* __restrict__ active,
+                                               int guard, T* __restrict__ out) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= nx || y >= ny) return;
+  int cx = x, cy = y;
+  if (guard && P.guard > 0) {
+    cx = clampi(x, P.guard, nx - 1 - P.guard);
+    cy = clampi(y, P.guard, ny - 1 - P.guard);
+  }
+  int xl, xr, yl, yr;
+  if (P.periodic) {
+    xl = wrap(cx - 1, nx); xr = wrap(cx + 1, nx); yl = wrap(cy - 1, ny); yr = wrap(cy + 1, ny);
+  } else {
+    xl = clampi(cx - 1, 0, nx - 1); xr = clampi(cx + 1, 0, nx - 1);
+    yl = clampi(cy - 1, 0, ny - 1); yr = clampi(cy + 1, 0, ny - 1);
+  }
+  const bool act = active[cy * nx + cx] != 0;
+  T o[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const T* a = q + (size_t)c * N;
+    const T Lc = lap_at(P, a, cy, cx);
+    const T lx = ((lap_at(P, a, cy, xr) - T(2.0) * Lc) + lap_at(P, a, cy, xl)) / P.dx2;
+    const T ly = ((lap_at(P, a, yr, cx) - T(2.0) * Lc) + lap_at(P, a, yl, cx)) / P.dy2;
+    const T bih = lx + ly;
+    const T coef = act ? P.hv_coef[c] : T(0);
+    o[c] = a[cy * nx + cx] - (coef * bih);
+  }
+  enforce(P, o[0], o[1], o[2], o[3]);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[(size_t)c * N + y * nx + x] = o[c];
+}
+
+// clean(): enforce_physical_state, then the outflow guard if requested.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_enforce(Params<T> P, const T* __restrict__ q, int guard,
+                                                    T* __restrict__ out) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= nx || y >= ny) return;
+  int cx = x, cy = y;
+  if (guard && P.guard > 0) {
+    cx = clampi(x, P.guard, nx - 1 - P.guard);
+    cy = clampi(y, P.guard, ny - 1 - P.guard);
+  }
+  T a, b, c, d;
+  load4(q, N, cy * nx + cx, a, b, c, d);
+  enforce(P, a, b, c, d);
+  const int o = y * nx + x;
+  out[o] = a;
+  out[N + o] = b;
+  out[2 * N + o] = c;
+  out[3 * N + o] = d;
+}
+
+// ================================================================ forcing
+// Forward DFT of the noise at each shell mode (OOP/forcing.py:78-79 keeps only
+// these coefficients of fft2).  PPT points per thread held in registers; one
+// block reduction per mode.
+constexpr int kPPT = 8;
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fproject(Params<T> P, const double* __restrict__ noise,
+                                                     double* __restrict__ part) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  double val[kPPT];
+  int xs[kPPT], ys[kPPT];
+#pragma unroll
+  for (int j = 0; j < kPPT; ++j) {
+    const int idx = blockIdx.x * kBlock * kPPT + j * kBlock + threadIdx.x;
+    if (idx < N) {
+      val[j] = noise[idx];
+      ys[j] = idx / nx;
+      xs[j] = idx - ys[j] * nx;
+    } else {
+      val[j] = 0.0; ys[j] = 0; xs[j] = 0;
+    }
+  }
+  for (int m = 0; m < P.nforce; ++m) {
+    const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
+    double v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kPPT; ++j) {
+      const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + xs[j]) * 2;
+      const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + ys[j]) * 2;
+      const double cr = tx[0] * ty[0] - tx[1] * ty[1];
+      const double si = tx[1] * ty[0] + tx[0] * ty[1];
+      v[0] += val[j] * cr;
+      v[1] -= val[j] * si;
+    }
+    block_reduce_write<2, 0>(v, part + ((size_t)blockIdx.x * P.nforce + m) * 2);
+    __syncthreads();
+  }
+}
+
+// fx, fy = real inverse DFT of (fx_hat, fy_hat) on the shell modes
+// (OOP/forcing.py:82-85), plus sum(fx^2+fy^2) and sum(rho*(fx*u+fy*v)).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fsynth(Params<T> P, const double* __restrict__ fhat, T* __restrict__ fxo,
+                                                   T* __restrict__ fyo, const T* __restrict__ rho,
+                                                   const T* __restrict__ u, const T* __restrict__ v,
+                                                   double* __restrict__ part) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const double invN = 1.0 / ((double)nx * (double)ny);
+  double acc[2] = {0.0, 0.0};
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+    const int y = idx / nx, x = idx - y * nx;
+    double ax = 0.0, ay = 0.0;
+    for (int m = 0; m < P.nforce; ++m) {
+      const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
+      const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
+      const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + y) * 2;
+      const double cr = tx[0] * ty[0] - tx[1] * ty[1];
+      const double si = tx[1] * ty[0] + tx[0] * ty[1];
+      const double* fxm = fhat + 2 * m;
+      const double* fym = fhat + 2 * P.nforce + 2 * m;
+      ax += fxm[0] * cr - fxm[1] * si;
+      ay += fym[0] * cr - fym[1] * si;
+    }
+    const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
+    fxo[idx] = fxv;
+    fyo[idx] = fyv;
+    acc[0] += (double)(fxv * fxv + fyv * fyv);
+    acc[1] += (double)(rho[idx] * (fxv * u[idx] + fyv * v[idx]));
+  }
+  block_reduce_write<2, 0>(acc, part + (size_t)blockIdx.x * 2);
+}
+
+// fx = (fx*scale)*alpha, fy likewise (OOP/forcing.py:91-112), and the
+// statistics of OOP/forcing.py:113-118.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fscale(Params<T> P, T* __restrict__ fx, T* __restrict__ fy, T scale,
+                                                   T alpha, const T* __restrict__ rho, const T* __restrict__ u,
+                                                   const T* __restrict__ v, double* __restrict__ part) {
+  const int N = P.nx * P.ny;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+    const T a = (fx[idx] * scale) * alpha, b = (fy[idx] * scale) * alpha;
+    fx[idx] = a;
+    fy[idx] = b;
+    acc[0] += (double)(a * a);
+    acc[1] += (double)(b * b);
+    acc[2] += (double)(a * b);
+    acc[3] += (double)(rho[idx] * (a * u[idx] + b * v[idx]));
+  }
+  block_reduce_write<4, 0>(acc, part + (size_t)blockIdx.x * 4);
+}
+
+// OOP/equations.py:136-145 on the whole grid.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_primitives(Params<T> P, const T* __restrict__ q, T* __restrict__ rho,
+                                                       T* __restrict__ u, T* __restrict__ v, T* __restrict__ p) {
+  const int N = P.nx * P.ny;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+    const Prim<T> pr = prim_at(P, q, N, idx, 0);
+    rho[idx] = pr.rho;
+    u[idx] = pr.u;
+    v[idx] = pr.v;
+    p[idx] = pr.p;
+  }
+}
+
+}  // namespace hitx

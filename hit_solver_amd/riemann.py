@@ -1,0 +1,393 @@
+"""Lax-Liu 2-D Riemann problems on the hip backend.
+
+API of reference OOP/problems/riemann_config3.py (RiemannConfig3,
+NonPeriodicEulerShockSensor2D, NonPeriodicHybridEuler2DOperator,
+LocalHyperviscosity2D, apply_outflow_guard, validate_raw_physical_state,
+hyperviscosity_enabled_for_config, run_riemann), riemann_config3_08.py and
+riemann_config6.py.  The operator, sensor, hyperviscosity and guard run as
+HIP kernels; the quadrant initial state is built on the host (setup).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import asdict, dataclass
+from typing import ClassVar
+
+import numpy as np
+import torch
+
+from .domain import Domain2D
+from .equations import EulerEquation2D
+from .runtime import PlanSpec, get_plan
+from .time_operator import NativeRHS, SSPRK3
+
+
+@dataclass(frozen=True)
+class RiemannConfig3:
+    """Lax-Liu / Kurganov-Tadmor configuration 3 (riemann_config3.py:19-58)."""
+
+    configuration_number: ClassVar[int] = 3
+
+    nx: int = 512
+    ny: int = 512
+    tfinal: float = 0.3
+    cfl: float = 0.4
+    gamma: float = 1.4
+    x_min: float = 0.0
+    x_max: float = 1.0
+    y_min: float = 0.0
+    y_max: float = 1.0
+    x_split: float = 0.5
+    y_split: float = 0.5
+    backend: str = "hip"
+    scheme: str = "hybrid"
+    sensor_width: int = 4
+    jump_threshold: float = 0.025
+    compression_threshold: float = 2.5
+    shear_threshold: float | None = 0.0
+    boundary_guard: int = 4
+    mn: float = 0.001
+    hyperviscosity_interval: int = 1
+    hyperviscosity_density_weight: float = 1.0
+    hyperviscosity_momentum_weight: float = 1.0
+    hyperviscosity_energy_weight: float = 1.0
+    progress_every: int = 500
+    guard_cells: int = 2
+    save_final_npz: bool = True
+    dtype: str = "fp64"
+
+    # (rho, u, v, p): I top-right, II top-left, III bottom-left, IV bottom-right
+    quadrants: ClassVar[tuple] = ((1.5, 0.0, 0.0, 1.5), (0.5323, 1.206, 0.0, 0.3),
+                                  (0.138, 1.206, 1.206, 0.029), (0.5323, 0.0, 1.206, 0.3))
+
+    @property
+    def domain(self) -> Domain2D:
+        return Domain2D(self.x_min, self.x_max, self.y_min, self.y_max, self.nx, self.ny)
+
+    @property
+    def equation(self) -> EulerEquation2D:
+        return EulerEquation2D(gamma=self.gamma)
+
+    @property
+    def torch_dtype(self):
+        return torch.float32 if self.dtype == "fp32" else torch.float64
+
+    def initial_primitives(self):
+        """Quadrant states on the cell centres (riemann_config3.py:60-99)."""
+        d = self.domain
+        X, Y = np.meshgrid(d.x, d.y, indexing="xy")
+        top, right = Y >= self.y_split, X >= self.x_split
+        q1, q2, q3, q4 = self.quadrants
+        prim = [np.full((self.ny, self.nx), q3[k]) for k in range(4)]
+        for sel, st in ((top & right, q1), (top & ~right, q2), (~top & right, q4)):
+            for k in range(4):
+                prim[k][sel] = st[k]
+        return prim
+
+    def initial_state(self):
+        rho, u, v, p = self.initial_primitives()
+        rho = np.maximum(rho, 1e-10)
+        p = np.maximum(p, 1e-10)
+        q = np.zeros((4, self.ny, self.nx))
+        q[0] = rho
+        q[1] = rho * u
+        q[2] = rho * v
+        q[3] = p / (self.gamma - 1.0) + 0.5 * rho * (u**2 + v**2)
+        return torch.from_numpy(q).to(device="cuda", dtype=self.torch_dtype)
+
+
+@dataclass(frozen=True)
+class RiemannConfig3Offset08(RiemannConfig3):
+    """riemann_config3_08.py:9-19."""
+
+    scenario_name: ClassVar[str] = "riemann_config3_08"
+    configuration_label: ClassVar[str] = "Configuration 3 (x0=y0=0.8)"
+    x_split: float = 0.8
+    y_split: float = 0.8
+    hyperviscosity_interval: int = 5
+
+
+@dataclass(frozen=True)
+class RiemannConfig6(RiemannConfig3):
+    """riemann_config6.py:40-87: four interacting contact discontinuities."""
+
+    configuration_number: ClassVar[int] = 6
+    tfinal: float = 0.25
+    quadrants: ClassVar[tuple] = ((1.0, 0.75, -0.50, 1.0), (2.0, 0.75, 0.50, 1.0),
+                                  (1.0, -0.75, 0.50, 1.0), (3.0, -0.75, -0.50, 1.0))
+
+
+@dataclass(frozen=True)
+class RiemannDiagnostics:
+    backend: str
+    scheme: str
+    hyperviscosity_enabled: bool
+    hyperviscosity_applications: int
+    final_sensor_fraction: float
+    density_min: float
+    density_max: float
+    pressure_min: float
+    pressure_max: float
+    max_abs_vorticity: float
+    has_nonfinite: bool
+    density_positive: bool
+    pressure_positive: bool
+    time: float
+    steps: int
+
+
+@dataclass
+class NonPeriodicLineCompactDerivative:
+    """Compact-scheme constants (riemann_config3.py:225-265); the Thomas-type
+    solve with identity end rows runs inside the sweep kernels."""
+
+    n: int
+    dx: float
+    alpha: float = 3.0 / 8.0
+
+
+@dataclass(frozen=True)
+class NonPeriodicEulerShockSensor2D:
+    """riemann_config3.py:278-326."""
+
+    domain: Domain2D
+    equation: EulerEquation2D
+    width: int = 4
+    compression_threshold: float = 2.5
+    jump_threshold: float = 0.04
+    shear_threshold: float | None = 2.0
+    boundary_guard: int = 4
+
+    def plan_spec(self, dtype, **extra) -> PlanSpec:
+        eq = self.equation
+        return PlanSpec(
+            nx=self.domain.nx, ny=self.domain.ny, dx=self.domain.dx, dy=self.domain.dy, periodic=False,
+            dtype=dtype, gamma=eq.gamma, rho_floor=eq.rho_floor, p_floor=eq.pressure_floor,
+            sensor_width=self.width, boundary_guard=self.boundary_guard,
+            compression_threshold=self.compression_threshold, jump_threshold=self.jump_threshold,
+            shear_threshold=self.shear_threshold, **extra,
+        )
+
+    def detect_u8(self, q):
+        return get_plan(self.plan_spec(q.dtype)).sensor(q, enforce_first=False)
+
+    def detect(self, q):
+        return self.detect_u8(q).bool()
+
+
+@dataclass
+class NonPeriodicHybridEuler2DOperator:
+    """riemann_config3.py:329-378."""
+
+    domain: Domain2D
+    equation: EulerEquation2D
+    sensor: NonPeriodicEulerShockSensor2D
+    scheme: str = "hybrid"
+    compact_x: NonPeriodicLineCompactDerivative = None
+    compact_y: NonPeriodicLineCompactDerivative = None
+
+    def __post_init__(self):
+        if self.scheme not in {"hybrid", "weno"}:
+            raise ValueError("scheme must be 'hybrid' or 'weno'")
+        self.compact_x = NonPeriodicLineCompactDerivative(self.domain.nx, self.domain.dx)
+        self.compact_y = NonPeriodicLineCompactDerivative(self.domain.ny, self.domain.dy)
+
+    def plan(self, dtype, **extra):
+        return get_plan(self.sensor.plan_spec(dtype, scheme=self.scheme, **extra))
+
+    def rhs(self, q):
+        return self.plan(q.dtype).rhs(q)
+
+    def fixed_time_step(self, q, cfl: float, t_end: float):
+        max_sr, _, _ = self.plan(q.dtype).step_reduce(q)
+        dt = cfl / max_sr
+        n = int(math.ceil(t_end / dt))
+        return t_end / n, n
+
+    def native_rhs(self, dtype, guard_cells: int):
+        """Stage-fused RHS whose clean is enforce + outflow guard."""
+        return NativeRHS(self.plan(dtype, guard_cells=guard_cells))
+
+
+@dataclass(frozen=True)
+class LocalHyperviscosity2D:
+    """riemann_config3.py:381-416: edge-padded biharmonic filter on compact nodes."""
+
+    domain: Domain2D
+    mn: float = 0.001
+    density_weight: float = 1.0
+    momentum_weight: float = 1.0
+    energy_weight: float = 1.0
+
+    def coefficients(self):
+        h = min(self.domain.dx, self.domain.dy)
+        w = (self.density_weight, self.momentum_weight, self.momentum_weight, self.energy_weight)
+        return tuple((self.mn * wk) * h**4 for wk in w)
+
+    def spec(self, dtype, equation, guard_cells=0):
+        d = self.domain
+        return PlanSpec(nx=d.nx, ny=d.ny, dx=d.dx, dy=d.dy, periodic=False, dtype=dtype, gamma=equation.gamma,
+                        rho_floor=equation.rho_floor, p_floor=equation.pressure_floor,
+                        hv_coef=self.coefficients(), guard_cells=guard_cells)
+
+    def apply(self, q, compact_mask, equation, guard_cells: int = 0):
+        if tuple(compact_mask.shape) != tuple(q.shape[-2:]):
+            raise ValueError(f"compact_mask shape {tuple(compact_mask.shape)} does not match grid {tuple(q.shape[-2:])}")
+        act = compact_mask if compact_mask.dtype == torch.uint8 else compact_mask.to(torch.uint8)
+        plan = get_plan(self.spec(q.dtype, equation, guard_cells))
+        return plan.hyperviscosity(q, act.contiguous(), apply_guard=guard_cells > 0)
+
+
+def apply_outflow_guard(q, guard_cells: int = 2, equation: EulerEquation2D | None = None):
+    """riemann_config3.py:419-429.  Runs fused with enforce on the device, so
+    it is exact only on states that enforce leaves unchanged (every state the
+    solver produces has been enforced)."""
+    if guard_cells <= 0:
+        return q
+    eq = equation or EulerEquation2D()
+    ny, nx = q.shape[1:]
+    plan = get_plan(PlanSpec(nx=int(nx), ny=int(ny), dx=1.0, dy=1.0, periodic=False, dtype=q.dtype,
+                             gamma=eq.gamma, rho_floor=eq.rho_floor, p_floor=eq.pressure_floor,
+                             guard_cells=guard_cells))
+    return plan.enforce(q, apply_guard=True)
+
+
+def raw_density_pressure(q, equation):
+    rho = q[0]
+    u = q[1] / rho
+    v = q[2] / rho
+    ke = 0.5 * rho * (u**2 + v**2)
+    return rho, (equation.gamma - 1.0) * (q[3] - ke)
+
+
+def validate_raw_physical_state(q, equation, label: str = "state") -> None:
+    """riemann_config3.py:432-452 (raises FloatingPointError)."""
+    if bool(torch.any(~torch.isfinite(q))):
+        raise FloatingPointError(f"{label} contains NaN or Inf values")
+    rho, p = raw_density_pressure(q, equation)
+    if bool(torch.any(~torch.isfinite(rho))) or bool(torch.any(~torch.isfinite(p))):
+        raise FloatingPointError(f"{label} has non-finite density or pressure")
+    if bool(torch.any(rho <= 0.0)):
+        raise FloatingPointError(f"{label} contains non-positive density")
+    if bool(torch.any(p <= 0.0)):
+        raise FloatingPointError(f"{label} contains non-positive pressure")
+
+
+def hyperviscosity_enabled_for_config(config: RiemannConfig3) -> bool:
+    """riemann_config3.py:667-678."""
+    return config.scheme == "hybrid" and config.mn > 0.0 and config.hyperviscosity_interval > 0
+
+
+def vorticity_z(q, domain: Domain2D, equation):
+    """riemann_config3.py:459-463 with one-sided ends (diagnostics only)."""
+    rho, u, v, _p = equation.primitive_from_conservative(q)
+
+    def fd(a, h, dim):
+        d = torch.empty_like(a)
+        if dim == -1:
+            d[:, 1:-1] = (a[:, 2:] - a[:, :-2]) / (2.0 * h)
+            d[:, 0] = (a[:, 1] - a[:, 0]) / h
+            d[:, -1] = (a[:, -1] - a[:, -2]) / h
+        else:
+            d[1:-1] = (a[2:] - a[:-2]) / (2.0 * h)
+            d[0] = (a[1] - a[0]) / h
+            d[-1] = (a[-1] - a[-2]) / h
+        return d
+
+    return fd(v, domain.dx, -1) - fd(u, domain.dy, -2)
+
+
+class RiemannSolver:
+    """The reference step loop (riemann_config3.py:681-749) as an object:
+    ``step()`` runs one fused SSP-RK3 step, then the scheduled hyperviscosity
+    pass with a fresh mask and the guard."""
+
+    def __init__(self, config: RiemannConfig3, q0=None):
+        if config.backend not in ("hip", "auto"):
+            raise ValueError("this package implements the 'hip' backend")
+        self.config = config
+        eq = config.equation
+        self.equation = eq
+        dt_ = config.torch_dtype
+        q = config.initial_state() if q0 is None else q0
+        self.sensor = NonPeriodicEulerShockSensor2D(
+            config.domain, eq, width=config.sensor_width, compression_threshold=config.compression_threshold,
+            jump_threshold=config.jump_threshold, shear_threshold=config.shear_threshold,
+            boundary_guard=config.boundary_guard)
+        self.operator = NonPeriodicHybridEuler2DOperator(config.domain, eq, self.sensor, scheme=config.scheme)
+        self.rhs = self.operator.native_rhs(dt_, config.guard_cells)
+        q = self.rhs.clean(q)  # outflow guard of enforce(q), riemann_config3.py:685-686
+        validate_raw_physical_state(q, eq, "initial state")
+        self.dt, self.n_steps = self.operator.fixed_time_step(q, config.cfl, config.tfinal)
+        self.hv_enabled = hyperviscosity_enabled_for_config(config)
+        self.hv = LocalHyperviscosity2D(
+            config.domain, mn=config.mn, density_weight=config.hyperviscosity_density_weight,
+            momentum_weight=config.hyperviscosity_momentum_weight,
+            energy_weight=config.hyperviscosity_energy_weight) if self.hv_enabled else None
+        self.integrator = SSPRK3()
+        self.q = q
+        self.steps = 0
+        self.hv_applications = 0
+        self.time = 0.0
+        self._spare = torch.empty_like(q)
+        if self.hv_enabled:
+            self._hv_plan = get_plan(self.hv.spec(dt_, eq, config.guard_cells))
+            self._sensor_plan = get_plan(self.sensor.plan_spec(dt_))
+            self._mask = torch.empty(q.shape[1:], dtype=torch.uint8, device=q.device)
+
+    def step(self, validate: bool = True):
+        c = self.config
+        q = self.integrator.step(self.q, self.rhs, self.dt, clean=self.rhs.clean, out=self._spare)
+        self._spare = self.q
+        if self.hv_enabled and (self.steps + 1) % c.hyperviscosity_interval == 0:
+            weno = self._sensor_plan.sensor(q, enforce_first=False, out=self._mask)
+            active = 1 - weno
+            q_new = self._hv_plan.hyperviscosity(q, active, apply_guard=c.guard_cells > 0, out=self._spare)
+            self._spare = q
+            q = q_new
+            self.hv_applications += 1
+        self.steps += 1
+        self.time = self.steps * self.dt
+        if validate and (self.steps % c.progress_every == 0 or self.steps == self.n_steps):
+            validate_raw_physical_state(q, self.equation, f"state at step {self.steps}")
+        self.q = q
+        return q
+
+    def run(self, max_steps: int | None = None):
+        n = self.n_steps if max_steps is None else min(max_steps, self.n_steps)
+        while self.steps < n:
+            self.step()
+        return self.q
+
+    def diagnostics(self) -> RiemannDiagnostics:
+        q, c, eq = self.q, self.config, self.equation
+        rho, p = raw_density_pressure(q, eq)
+        w = vorticity_z(q, c.domain, eq)
+        nonfinite = bool(torch.any(~torch.isfinite(q)) | torch.any(~torch.isfinite(rho)) | torch.any(~torch.isfinite(p)))
+        return RiemannDiagnostics(
+            backend=c.backend, scheme=c.scheme, hyperviscosity_enabled=self.hv_enabled,
+            hyperviscosity_applications=self.hv_applications,
+            final_sensor_fraction=float(self.sensor.detect(q).double().mean()),
+            density_min=float(rho.min()), density_max=float(rho.max()), pressure_min=float(p.min()),
+            pressure_max=float(p.max()), max_abs_vorticity=float(w.abs().max()), has_nonfinite=nonfinite,
+            density_positive=bool(torch.all(rho > 0.0)), pressure_positive=bool(torch.all(p > 0.0)),
+            time=self.time, steps=self.steps)
+
+
+def run_riemann(config: RiemannConfig3):
+    """riemann_config3.py:681-749: returns (q, RiemannDiagnostics)."""
+    s = RiemannSolver(config)
+    q = s.run()
+    d = s.diagnostics()
+    validate_raw_physical_state(q, config.equation, "final state")
+    return q, d
+
+
+def riemann_config_dict(config: RiemannConfig3) -> dict:
+    values = {"configuration_number": config.configuration_number, **asdict(config)}
+    for k in ("scenario_name", "configuration_label"):
+        v = getattr(config, k, None)
+        if v is not None:
+            values[k] = v
+    return values

@@ -1,0 +1,174 @@
+/* hitx — MI355X (gfx950) RHS evaluation and SSP-RK3 step of the 2-D
+ * compressible hybrid compact/WENO7 solver of HIT_Solver.
+ *
+ * C ABI: plain pointers and sizes, no exceptions, no torch types.  Every
+ * array pointer is a DEVICE pointer (e.g. torch.Tensor.data_ptr() of a
+ * contiguous tensor on the current HIP device) unless named *_host.  Arrays
+ * are C-order: state q is [4][ny][nx] (rho, rho*u, rho*v, E; x fastest),
+ * masks are uint8 [ny][nx] (1 = WENO node).  The element type of q is
+ * double when desc.dtype == HITX_F64 and float when HITX_F32.  Reductions
+ * are always accumulated in double.
+ *
+ * Calls are asynchronous on `stream` (a hipStream_t, 0 = default stream).
+ * Return value 0 = success, negative = error; hitx_last_error() returns a
+ * thread-local message for the last failure.  A plan is used by one host
+ * thread at a time.  The caller owns all memory; a plan owns only constant
+ * tables (twiddles, mode lists).
+ *
+ * Reference interfaces replaced (paths in the HIT_Solver tree):
+ *   OOP/parallel/backend.py:17-32        select_array_module -> new backend "hip"
+ *   OOP/spatial_operator.py:263-304      PeriodicEulerShockSensor2D.detect   -> hitx_sensor
+ *   OOP/problems/riemann_config3.py:288  NonPeriodicEulerShockSensor2D.detect -> hitx_sensor
+ *   OOP/spatial_operator.py:344-367      PeriodicHybridEuler2DOperator.rhs    -> hitx_rhs
+ *   OOP/problems/riemann_config3.py:359  NonPeriodicHybridEuler2DOperator.rhs -> hitx_rhs
+ *   OOP/time_operator.py:21-26           SSPRK3.step (one stage each call)   -> hitx_rk3_stage
+ *   OOP/hit2d.py:847-867                 rhs_with_forcing sources (in stage)  -> hitx_rk3_stage
+ *   OOP/spatial_operator.py:407-435      PeriodicHyperviscosity2D.apply       -> hitx_hyperviscosity
+ *   OOP/problems/riemann_config3.py:398  LocalHyperviscosity2D.apply          -> hitx_hyperviscosity
+ *   OOP/equations.py:165-181             enforce_physical_state               -> hitx_enforce
+ *   OOP/problems/riemann_config3.py:419  apply_outflow_guard                  -> hitx_enforce (guard flag)
+ *   OOP/spatial_operator.py:369-377      stable_time_step (and hit2d.py:279-290 Mt) -> hitx_step_reduce
+ *   OOP/problems/riemann_config3.py:371  fixed_time_step                      -> hitx_step_reduce
+ *   OOP/forcing.py:77-79                 fft2 + shell mask of the noise       -> hitx_forcing_project
+ *   OOP/forcing.py:82-87                 curl + ifft2 + rms                   -> hitx_forcing_synthesize
+ *   OOP/forcing.py:91-118                rescale and statistics               -> hitx_forcing_scale
+ *   OOP/hit2d.py:132-136                 conservative_to_primitive            -> hitx_primitives
+ *   OOP/equations.py:232-267             CompressibleNavierStokes2D.viscous_rhs -> hitx_viscous_rhs
+ */
+#ifndef HITX_H
+#define HITX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HITX_ABI_VERSION 1
+
+enum { HITX_F64 = 0, HITX_F32 = 1 };
+enum { HITX_HYBRID = 0, HITX_WENO = 1 };
+
+enum {
+  HITX_OK = 0,
+  HITX_EINVAL = -1,   /* bad argument or descriptor (Python: ValueError) */
+  HITX_EDEVICE = -2,  /* HIP runtime failure (Python: RuntimeError)     */
+  HITX_ENOMEM = -3,   /* workspace too small                             */
+};
+
+#define HITX_MAX_MODES 128
+
+typedef struct hitx_desc {
+  int32_t abi_version;    /* = HITX_ABI_VERSION */
+  int32_t nx, ny;         /* grid; both >= 8 */
+  int32_t periodic;       /* 1: periodic HIT operator; 0: non-periodic Riemann operator */
+  int32_t dtype;          /* HITX_F64 or HITX_F32 */
+  int32_t scheme;         /* HITX_HYBRID or HITX_WENO (WENO: non-periodic only) */
+  double dx, dy;          /* Domain2D.dx/dy */
+  double dx2, dy2;        /* dx**2, dy**2 as the host language rounds them */
+  double gamma, rho_floor, p_floor;
+  /* shock sensor */
+  int32_t sensor_width;   /* dilation half-width, <= 8 */
+  int32_t boundary_guard; /* non-periodic: forced band before dilation */
+  int32_t shear_enabled;  /* 0: no shear term */
+  int32_t pad0;
+  double compression_threshold, jump_threshold, shear_threshold;
+  /* Navier-Stokes terms (periodic only); viscosity 0 disables them */
+  double viscosity, prandtl, gas_constant;
+  /* hyperviscosity: per-component coefficient mn*w_k*h**4 */
+  double hv_coef[4];
+  /* non-periodic outflow guard cells (0 = none) */
+  int32_t guard_cells;
+  /* HIT sources */
+  int32_t drag_enabled, cooling_enabled, forcing_enabled;
+  double drag_coef, cooling_time, cooling_p_ref;
+  int32_t n_drag_modes, n_force_modes;
+  /* (kx index, ky index) pairs in FFT index space, |index| <= 16; host pointers,
+   * copied at plan creation */
+  const int32_t* drag_modes_host;   /* [n_drag_modes][2] */
+  const int32_t* force_modes_host;  /* [n_force_modes][2] */
+} hitx_desc;
+
+typedef struct hitx_plan hitx_plan;
+
+/* Plan lifetime.  The plan binds to the HIP device current at creation. */
+int hitx_plan_create(const hitx_desc* desc, hitx_plan** out);
+int hitx_plan_destroy(hitx_plan* plan);
+/* Bytes of device workspace that every compute entry point below needs. */
+size_t hitx_workspace_bytes(const hitx_plan* plan);
+const char* hitx_last_error(void);
+int hitx_abi_version(void);
+
+/* Shock sensor: mask_out[ny][nx] = 1 at WENO nodes.  enforce_first = 1
+ * applies enforce_physical_state to q before detection, as the operator's
+ * rhs does; 0 detects on q as given (the post-step detect calls). */
+int hitx_sensor(hitx_plan* plan, const void* q, int32_t enforce_first, uint8_t* mask_out,
+                void* ws, size_t ws_bytes, void* stream);
+
+/* Operator RHS only (no viscous or forcing sources): dq = -dF/dx - dG/dy. */
+int hitx_rhs(hitx_plan* plan, const void* q, void* dq, void* ws, size_t ws_bytes, void* stream);
+
+/* One Shu-Osher stage (0, 1, 2) of SSP-RK3 with the full HIT source closure
+ * (viscous, forcing fx/fy, drag, cooling as enabled in the descriptor):
+ *   stage 0: out = clean(q0 + dt L(q0))                     (qs == q0)
+ *   stage 1: out = clean(3/4 q0 + 1/4 (qs + dt L(qs)))
+ *   stage 2: out = clean(1/3 q0 + 2/3 (qs + dt L(qs)))
+ * clean = enforce_physical_state, then the outflow guard when guard_cells > 0.
+ * fx, fy: [ny][nx] forcing accelerations (ignored unless forcing_enabled).
+ * out must not alias q0 or qs. */
+int hitx_rk3_stage(hitx_plan* plan, int32_t stage, const void* q0, const void* qs, void* out,
+                   const void* fx, const void* fy, double dt, void* ws, size_t ws_bytes,
+                   void* stream);
+
+/* Biharmonic filter on the nodes where active[ny][nx] != 0, then
+ * enforce_physical_state, then the outflow guard if apply_guard.  out must
+ * not alias q. */
+int hitx_hyperviscosity(hitx_plan* plan, const void* q, const uint8_t* active, void* out,
+                        int32_t apply_guard, void* ws, size_t ws_bytes, void* stream);
+
+/* out = enforce_physical_state(q), then the outflow guard if apply_guard. */
+int hitx_enforce(hitx_plan* plan, const void* q, void* out, int32_t apply_guard, void* stream);
+
+/* Step-level reductions of q into out_dev (device, double[3]):
+ *   [0] max((|u|+c)/dx + (|v|+c)/dy)   [1] sum(u^2+v^2)   [2] sum(c)   */
+int hitx_step_reduce(hitx_plan* plan, const void* q, double* out_dev, void* ws, size_t ws_bytes,
+                     void* stream);
+
+/* Forcing.  noise: device double[ny][nx].  modes_out: device double[n_force_modes][2]
+ * = sum over the grid of noise * exp(-i k.x) (unnormalised forward DFT at each
+ * shell mode). */
+int hitx_forcing_project(hitx_plan* plan, const double* noise, double* modes_out, void* ws,
+                         size_t ws_bytes, void* stream);
+/* fhat: device double[2][n_force_modes][2] (fx_hat, fy_hat, complex), spectral
+ * coefficients of the inverse DFT (1/(nx*ny) normalisation applied here).
+ * Writes fx, fy (element type of q) and stats_out (device double[2]):
+ *   [0] sum(fx^2 + fy^2)   [1] sum(rho*(fx*u + fy*v)).
+ * rho, u, v: [ny][nx] arrays of the element type of q. */
+int hitx_forcing_synthesize(hitx_plan* plan, const double* fhat, void* fx, void* fy, const void* rho,
+                            const void* u, const void* v, double* stats_out, void* ws, size_t ws_bytes,
+                            void* stream);
+/* In place fx = (fx*scale)*alpha (same for fy), then stats_out (double[4]):
+ *   [0] sum(fx^2) [1] sum(fy^2) [2] sum(fx*fy) [3] sum(rho*(fx*u + fy*v)). */
+int hitx_forcing_scale(hitx_plan* plan, void* fx, void* fy, double scale, double alpha, const void* rho,
+                       const void* u, const void* v, double* stats_out, void* ws, size_t ws_bytes,
+                       void* stream);
+
+/* rho, u, v, p = primitive_from_conservative(q), each [ny][nx]. */
+int hitx_primitives(hitx_plan* plan, const void* q, void* rho, void* u, void* v, void* p, void* stream);
+
+/* Periodic viscous and heat-conduction terms of q (the same terms the stage
+ * adds when desc.viscosity > 0): out [4][ny][nx], out[0] = 0. */
+int hitx_viscous_rhs(hitx_plan* plan, const void* q, void* out, void* stream);
+
+/* Measurement support (no reference counterpart): when enabled, every
+ * line-sweep kernel launch of this plan is bracketed by HIP events on its
+ * stream.  hitx_timing_read synchronises those events, returns the summed
+ * kernel time and the number of launches since the last read, and resets. */
+int hitx_timing_enable(hitx_plan* plan, int32_t on);
+int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HITX_H */
