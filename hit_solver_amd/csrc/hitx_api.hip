@@ -44,7 +44,13 @@ LineK<T> make_linek(int n) {
   const double a = kAlphaC;
   const double rho = (-1.0 + std::sqrt(1.0 - 4.0 * a * a)) / (2.0 * a);
   k.n = n;
-  k.L = (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : 1;
+  // largest chunk length that still gives every thread of the line block a task
+  k.L = 1;
+  for (int L : {8, 4, 2, 1})
+    if (n % L == 0 && 4 * (n / L) >= kLineBlock) {
+      k.L = L;
+      break;
+    }
   k.nch = n / k.L;
   k.M = (52 + k.L - 1) / k.L;
   k.rho = (T)rho;
@@ -76,7 +82,8 @@ struct hitx_plan {
   LineK<float> lx_f{}, ly_f{};
   // workspace layout
   size_t off_raw = 0, off_mask = 0, off_maskT = 0, off_qT = 0, off_dfx = 0, off_dfyT = 0, off_part = 0,
-         off_scal = 0, off_misc = 0, ws_bytes = 0;
+         off_scal = 0, off_misc = 0, off_ticket = 0, ws_bytes = 0;
+  size_t sensor_lds = 0;
   int nblk_red = 0, nblk_proj = 0;
   size_t line_lds_x = 0, line_lds_y = 0;
   // optional per-launch timing of the line-sweep kernel (HIP events on the launch stream)
@@ -141,7 +148,13 @@ Params<T> make_params(const hitx_plan* pl) {
 }
 
 size_t line_lds_bytes(int n, int nch, int esize) {
-  return (size_t)(12 * (n + 1) + 66 + 8 * nch + 20) * esize + n + 16;
+  return (size_t)(12 * (n + 1) + 66 + 8 * nch + 20) * esize + 4 * (size_t)(n + 1) + n + 64;
+}
+
+size_t sensor_lds_bytes(int w, int esize) {
+  const int H = w + 1, EX = kSTX + 2 * H, EY = kSTY + 2 * H;
+  return (size_t)4 * EX * EY * esize + (size_t)(kSTX + 2 * w) * (kSTY + 2 * w) + (size_t)(kSTY + 2 * w) * kSTX +
+         (size_t)kSTY * (kSTX + 1) + 64;
 }
 
 template <class T>
@@ -155,6 +168,7 @@ struct WS {
   uint8_t *raw, *mask, *maskT;
   void *qT, *dfx, *dfyT;
   double *part, *scal, *misc;
+  unsigned int* ticket;
 };
 
 WS carve(const hitx_plan* pl, void* ws) {
@@ -169,6 +183,7 @@ WS carve(const hitx_plan* pl, void* ws) {
   w.part = reinterpret_cast<double*>(b + pl->off_part);
   w.scal = reinterpret_cast<double*>(b + pl->off_scal);
   w.misc = reinterpret_cast<double*>(b + pl->off_misc);
+  w.ticket = reinterpret_cast<unsigned int*>(b + pl->off_ticket);
   return w;
 }
 
@@ -181,22 +196,17 @@ int check_ws(const hitx_plan* pl, void* ws, size_t ws_bytes) {
 
 dim3 grid16(const hitx_plan* pl) { return dim3((pl->d.nx + 15) / 16, (pl->d.ny + 15) / 16); }
 
-// reductions + raw mask + dilation: mask (and maskT) of q
+// stage reductions (one kernel, last block finalizes) + fused sensor tile:
+// mask and maskT of q, and the enforced transposed state when qT != nullptr
 template <class T>
-int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, const WS& w, hipStream_t s) {
+int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, T* qT, const WS& w, hipStream_t s) {
   const Params<T> P = make_params<T>(pl);
-  const bool drag = P.drag_on;
-  if (drag) {
-    k_stage_reduce<T, true><<<pl->nblk_red, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_red, w.scal);
-  } else {
-    k_stage_reduce<T, false><<<pl->nblk_red, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_red, w.scal);
-  }
-  LAUNCHCHK();
-  k_rawmask<T><<<grid16(pl), kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
-  dim3 g32((pl->d.nx + 31) / 32, (pl->d.ny + 31) / 32);
-  k_dilate<<<g32, kBlock, 0, s>>>(pl->d.nx, pl->d.ny, pl->d.sensor_width, pl->d.periodic, w.raw, mask_out, w.maskT);
+  if (P.drag_on)
+    k_stage_reduce<T, true><<<pl->nblk_red, kBlock, 0, s>>>(P, q, enforce_first, w.part, w.ticket, w.scal);
+  else
+    k_stage_reduce<T, false><<<pl->nblk_red, kBlock, 0, s>>>(P, q, enforce_first, w.part, w.ticket, w.scal);
+  dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
+  k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
   LAUNCHCHK();
   return HITX_OK;
 }
@@ -216,28 +226,27 @@ int tmark(hitx_plan* pl, hipStream_t s) {
 // both sweeps of the operator on state qs: dfx and dfyT in the workspace
 template <class T>
 int run_sweeps(hitx_plan* pl, const T* qs, const WS& w, hipStream_t s) {
-  int rc = run_sensor<T>(pl, qs, 1, w.mask, w, s);
+  int rc = run_sensor<T>(pl, qs, 1, w.mask, static_cast<T*>(w.qT), w, s);
   if (rc) return rc;
   const Params<T> P = make_params<T>(pl);
-  dim3 g32((pl->d.nx + 31) / 32, (pl->d.ny + 31) / 32);
-  k_transpose4<T><<<g32, kBlock, 0, s>>>(pl->d.nx, pl->d.ny, qs, static_cast<T*>(w.qT));
-  int rc2 = tmark(pl, s);
-  if (rc2) return rc2;
+  if ((rc = tmark(pl, s))) return rc;
   if (P.periodic)
-    k_line<T, true><<<pl->d.ny, kBlock, pl->line_lds_x, s>>>(P, linek<T>(pl, 0), 0, qs, w.mask, w.scal,
-                                                             static_cast<T*>(w.dfx));
+    k_line<T, true><<<pl->d.ny, kLineBlock, pl->line_lds_x, s>>>(P, linek<T>(pl, 0), 0, 0, qs, w.mask, w.scal,
+                                                                 static_cast<T*>(w.dfx));
   else
-    k_line<T, false><<<pl->d.ny, kBlock, pl->line_lds_x, s>>>(P, linek<T>(pl, 0), 0, qs, w.mask, w.scal,
-                                                              static_cast<T*>(w.dfx));
-  if ((rc2 = tmark(pl, s))) return rc2;
-  if ((rc2 = tmark(pl, s))) return rc2;
+    k_line<T, false><<<pl->d.ny, kLineBlock, pl->line_lds_x, s>>>(P, linek<T>(pl, 0), 0, 0, qs, w.mask, w.scal,
+                                                                  static_cast<T*>(w.dfx));
+  if ((rc = tmark(pl, s))) return rc;
+  if ((rc = tmark(pl, s))) return rc;
   if (P.periodic)
-    k_line<T, true><<<pl->d.nx, kBlock, pl->line_lds_y, s>>>(P, linek<T>(pl, 1), 1, static_cast<const T*>(w.qT),
-                                                             w.maskT, w.scal, static_cast<T*>(w.dfyT));
+    k_line<T, true><<<pl->d.nx, kLineBlock, pl->line_lds_y, s>>>(P, linek<T>(pl, 1), 1, 1,
+                                                                 static_cast<const T*>(w.qT), w.maskT, w.scal,
+                                                                 static_cast<T*>(w.dfyT));
   else
-    k_line<T, false><<<pl->d.nx, kBlock, pl->line_lds_y, s>>>(P, linek<T>(pl, 1), 1, static_cast<const T*>(w.qT),
-                                                              w.maskT, w.scal, static_cast<T*>(w.dfyT));
-  if ((rc2 = tmark(pl, s))) return rc2;
+    k_line<T, false><<<pl->d.nx, kLineBlock, pl->line_lds_y, s>>>(P, linek<T>(pl, 1), 1, 1,
+                                                                  static_cast<const T*>(w.qT), w.maskT, w.scal,
+                                                                  static_cast<T*>(w.dfyT));
+  if ((rc = tmark(pl, s))) return rc;
   LAUNCHCHK();
   return HITX_OK;
 }
@@ -369,6 +378,7 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->ly_f = make_linek<float>(desc->ny);
   pl->line_lds_x = line_lds_bytes(desc->nx, pl->lx_d.nch, pl->esize);
   pl->line_lds_y = line_lds_bytes(desc->ny, pl->ly_d.nch, pl->esize);
+  pl->sensor_lds = sensor_lds_bytes(desc->sensor_width, pl->esize);
   rc = pl->esize == 8 ? set_line_lds<double>(pl) : set_line_lds<float>(pl);
   if (rc) {
     std::string keep = g_err;
@@ -392,6 +402,7 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->off_part = o; o = align256(o + part * sizeof(double));
   pl->off_scal = o; o = align256(o + S_COUNT * sizeof(double));
   pl->off_misc = o; o = align256(o + 64 * sizeof(double));
+  pl->off_ticket = o; o = align256(o + sizeof(unsigned int));
   pl->ws_bytes = o;
   *out = pl;
   return HITX_OK;
@@ -415,8 +426,9 @@ int hitx_sensor(hitx_plan* pl, const void* q, int32_t enforce_first, uint8_t* ma
   if (!q || !mask_out) return fail(HITX_EINVAL, "null array");
   const WS w = carve(pl, ws);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return pl->esize == 8 ? run_sensor<double>(pl, static_cast<const double*>(q), enforce_first, mask_out, w, s)
-                        : run_sensor<float>(pl, static_cast<const float*>(q), enforce_first, mask_out, w, s);
+  return pl->esize == 8
+             ? run_sensor<double>(pl, static_cast<const double*>(q), enforce_first, mask_out, nullptr, w, s)
+             : run_sensor<float>(pl, static_cast<const float*>(q), enforce_first, mask_out, nullptr, w, s);
 }
 
 int hitx_rhs(hitx_plan* pl, const void* q, void* dq, void* ws, size_t ws_bytes, void* stream) {
@@ -441,8 +453,8 @@ int hitx_rk3_stage(hitx_plan* pl, int32_t stage, const void* q0, const void* qs,
                         : rhs_or_stage<float>(pl, MODE_STAGE, stage, q0, qs, out, fx, fy, dt, ws, s);
 }
 
-int hitx_hyperviscosity(hitx_plan* pl, const void* q, const uint8_t* active, void* out, int32_t apply_guard, void* ws,
-                        size_t ws_bytes, void* stream) {
+int hitx_hyperviscosity(hitx_plan* pl, const void* q, const uint8_t* active, int32_t invert, void* out,
+                        int32_t apply_guard, void* ws, size_t ws_bytes, void* stream) {
   int rc = check_ws(pl, ws, ws_bytes);
   if (rc) return rc;
   if (!q || !active || !out) return fail(HITX_EINVAL, "null array");
@@ -450,10 +462,10 @@ int hitx_hyperviscosity(hitx_plan* pl, const void* q, const uint8_t* active, voi
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (pl->esize == 8)
     k_hv<double><<<grid16(pl), kBlock, 0, s>>>(make_params<double>(pl), static_cast<const double*>(q), active,
-                                              apply_guard, static_cast<double*>(out));
+                                              apply_guard, invert, static_cast<double*>(out));
   else
     k_hv<float><<<grid16(pl), kBlock, 0, s>>>(make_params<float>(pl), static_cast<const float*>(q), active,
-                                             apply_guard, static_cast<float*>(out));
+                                             apply_guard, invert, static_cast<float*>(out));
   LAUNCHCHK();
   return HITX_OK;
 }

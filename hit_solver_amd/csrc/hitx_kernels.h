@@ -24,8 +24,17 @@ __device__ __forceinline__ T absT(T a) { return fabs(a); }
 // enforce(q) when enforce_first (the operator rhs); cooling and drag use q as
 // given (the RK stage state), as the reference does.
 template <class T, bool DRAG>
+__device__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
+                                    double* __restrict__ scal);
+
+// The last block to finish (atomic ticket, no waiting) reduces all partials
+// in a fixed order and writes the stage scalars, then re-arms the ticket.
+// The ticket must be zero before the first launch on a workspace.
+template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* __restrict__ q,
-                                                         int enforce_first, double* __restrict__ part) {
+                                                         int enforce_first, double* __restrict__ part,
+                                                         unsigned int* __restrict__ ticket,
+                                                         double* __restrict__ scal) {
   constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
   double v[NR];
 #pragma unroll
@@ -92,11 +101,20 @@ __global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* _
     }
   }
   block_reduce_write<NR, 0x1Full>(v, part + (size_t)blockIdx.x * NR);
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // acquire: every block's partials are visible
+  stage_finalize_body<T, DRAG>(P, part, gridDim.x, scal);
+  if (threadIdx.x == 0) *ticket = 0u;
 }
 
 template <class T, bool DRAG>
-__global__ void __launch_bounds__(kBlock) k_stage_finalize(Params<T> P, const double* __restrict__ part,
-                                                           int nblk, double* __restrict__ scal) {
+__device__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
+                                    double* __restrict__ scal) {
   constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
   __shared__ double r[NR];
   reduce_partials<NR, 0x1Full>(part, nblk, r);
@@ -314,6 +332,134 @@ __global__ void __launch_bounds__(kBlock) k_transpose4(int nx, int ny, const T* 
   }
 }
 
+// Fused sensor on a 32x16 output tile: primitives of (enforced) q on the tile
+// plus a (w+1) halo in LDS, undilated mask on the tile plus a w halo, the
+// separable (2w+1)^2 OR dilation, mask and maskT stores, and (when qT is not
+// null) the enforced state written transposed for the y sweep.
+// Same per-node arithmetic as k_rawmask; out-of-domain nodes of the
+// non-periodic grid are clamped copies for the neighbour gathers and false
+// in the dilation, as in OOP/problems/riemann_config3.py:128-162.
+constexpr int kSTX = 32, kSTY = 16;
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                   const double* __restrict__ scal, uint8_t* __restrict__ mask,
+                                                   uint8_t* __restrict__ maskT, T* __restrict__ qT) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nx = P.nx, ny = P.ny, N = nx * ny, w = P.width, H = w + 1;
+  const int EX = kSTX + 2 * H, EY = kSTY + 2 * H, RX = kSTX + 2 * w, RY = kSTY + 2 * w;
+  const int x0 = blockIdx.x * kSTX, y0 = blockIdx.y * kSTY;
+  const bool per = P.periodic;
+  T* Lr = reinterpret_cast<T*>(smem);
+  T* Lu = Lr + EX * EY;
+  T* Lv = Lu + EX * EY;
+  T* Lp = Lv + EX * EY;
+  uint8_t* raw = reinterpret_cast<uint8_t*>(Lp + EX * EY);
+  uint8_t* mid = raw + RX * RY;
+  uint8_t* outt = mid + RY * kSTX;  // [kSTY][kSTX + 1]
+  for (int t = threadIdx.x; t < EX * EY; t += kBlock) {
+    const int r = t / EX, c = t - r * EX;
+    int gy = y0 - H + r, gx = x0 - H + c;
+    if (per) {
+      gy = ((gy % ny) + ny) % ny;
+      gx = ((gx % nx) + nx) % nx;
+    } else {
+      gy = clampi(gy, 0, ny - 1);
+      gx = clampi(gx, 0, nx - 1);
+    }
+    const Prim<T> pr = prim_at(P, q, N, gy * nx + gx, enforce_first);
+    Lr[t] = pr.rho;
+    Lu[t] = pr.u;
+    Lv[t] = pr.v;
+    Lp[t] = pr.p;
+  }
+  __syncthreads();
+  const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
+  const bool comp_on = scal[S_COMP_ON] != 0.0, shear_on = scal[S_SHEAR_ON] != 0.0;
+  const double div_thr = scal[S_DIV_THR], shear_thr = scal[S_SHEAR_THR], thr = P.jump_thr;
+  for (int t = threadIdx.x; t < RX * RY; t += kBlock) {
+    const int r = t / RX, c = t - r * RX;
+    const int gy = y0 - w + r, gx = x0 - w + c;
+    bool m = false;
+    if (per || (gx >= 0 && gx < nx && gy >= 0 && gy < ny)) {
+      const int ic = (r + 1) * EX + (c + 1);
+      const int ixl = ic - 1, ixr = ic + 1, iyl = ic - EX, iyr = ic + EX;
+      const bool xin = per || (gx > 0 && gx < nx - 1), yin = per || (gy > 0 && gy < ny - 1);
+      const T ux = xin ? (Lu[ixr] - Lu[ixl]) / P.two_dx : (Lu[ixr] - Lu[ixl]) / P.dx;
+      const T vx = xin ? (Lv[ixr] - Lv[ixl]) / P.two_dx : (Lv[ixr] - Lv[ixl]) / P.dx;
+      const T uy = yin ? (Lu[iyr] - Lu[iyl]) / P.two_dy : (Lu[iyr] - Lu[iyl]) / P.dy;
+      const T vy = yin ? (Lv[iyr] - Lv[iyl]) / P.two_dy : (Lv[iyr] - Lv[iyl]) / P.dy;
+      const T div = ux + vy;
+      if (comp_on && (double)div < div_thr) m = true;
+      if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
+      const bool xlo = !per && gx == 0, xhi = !per && gx == nx - 1;
+      const bool ylo = !per && gy == 0, yhi = !per && gy == ny - 1;
+      const T jr = npmax(rel_jump(Lr[ixl], Lr[ic], Lr[ixr], er, per, xlo, xhi),
+                         rel_jump(Lr[iyl], Lr[ic], Lr[iyr], er, per, ylo, yhi));
+      const T jp = npmax(rel_jump(Lp[ixl], Lp[ic], Lp[ixr], ep, per, xlo, xhi),
+                         rel_jump(Lp[iyl], Lp[ic], Lp[iyr], ep, per, ylo, yhi));
+      auto e = [&](int i) { return Lp[i] / (Lr[i] * P.gm1); };
+      const T je = npmax(rel_jump(e(ixl), e(ic), e(ixr), ee, per, xlo, xhi),
+                         rel_jump(e(iyl), e(ic), e(iyr), ee, per, ylo, yhi));
+      if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+      if (!per && P.bguard > 0) {
+        const int g = P.bguard;
+        if (gy < g || gy >= ny - g || gx < g || gx >= nx - g) m = true;
+      }
+    }
+    raw[t] = m ? 1 : 0;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < RY * kSTX; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    uint8_t o = 0;
+    for (int d = 0; d <= 2 * w; ++d) o |= raw[r * RX + c + d];
+    mid[t] = o;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSTY * kSTX; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    uint8_t o = 0;
+    for (int d = 0; d <= 2 * w; ++d) o |= mid[(r + d) * kSTX + c];
+    outt[r * (kSTX + 1) + c] = o;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) mask[gy * nx + gx] = o;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSTY * kSTX; t += kBlock) {
+    const int c = t / kSTY, r = t - c * kSTY;  // consecutive threads -> consecutive y
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) maskT[(size_t)gx * ny + gy] = outt[r * (kSTX + 1) + c];
+  }
+  if (qT == nullptr) return;
+  // enforced state, transposed, staged through the (now free) primitive area
+  T* st = Lr;  // [4][kSTY][kSTX + 1]
+  for (int t = threadIdx.x; t < kSTY * kSTX; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) {
+      T a, b, cc, d;
+      load4(q, N, gy * nx + gx, a, b, cc, d);
+      if (enforce_first) enforce(P, a, b, cc, d);
+      const int o = r * (kSTX + 1) + c;
+      st[o] = a;
+      st[kSTY * (kSTX + 1) + o] = b;
+      st[2 * kSTY * (kSTX + 1) + o] = cc;
+      st[3 * kSTY * (kSTX + 1) + o] = d;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSTY * kSTX; t += kBlock) {
+    const int c = t / kSTY, r = t - c * kSTY;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) {
+      const int o = r * (kSTX + 1) + c;
+      const size_t g = (size_t)gx * ny + gy;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) qT[k * (size_t)N + g] = st[k * kSTY * (kSTX + 1) + o];
+    }
+  }
+}
+
 // ================================================================ line sweep
 // Power of rho from the LDS table (zero beyond the table: |rho|^66 < 1e-22).
 template <class T>
@@ -338,7 +484,7 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
   T* e2 = scratch + 4 * nch;   // [4][nch]
   T* bnd = scratch + 8 * nch;  // [4][5]: r0, r_{n-1}, z_n, p0, p_{n-1}
   const T rho = K.rho;
-  const int ntask = 4 * nch;
+  const int ntask = 4 * nch, B = blockDim.x;
   if (!PER) {
     if (threadIdx.x < 4) {
       bnd[threadIdx.x * 5 + 0] = r[threadIdx.x * S];
@@ -347,7 +493,7 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
     __syncthreads();
   }
   // causal pass, local
-  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+  for (int t = threadIdx.x; t < ntask; t += B) {
     const int line = t / nch, ch = t - line * nch, s0 = ch * L;
     T* rl = r + line * S;
     T acc = T(0);
@@ -369,7 +515,7 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
   }
   __syncthreads();
   // causal pass, carries
-  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+  for (int t = threadIdx.x; t < ntask; t += B) {
     const int line = t / nch, ch = t - line * nch, s0 = ch * L;
     T* rl = r + line * S;
     const T* el = e1 + line * nch;
@@ -387,7 +533,7 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
   }
   __syncthreads();
   // anti-causal pass, local
-  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+  for (int t = threadIdx.x; t < ntask; t += B) {
     const int line = t / nch, ch = t - line * nch, s0 = ch * L;
     T* rl = r + line * S;
     T acc = T(0);
@@ -399,7 +545,7 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
   }
   __syncthreads();
   // anti-causal pass, carries, scale by 1 + rho^2
-  for (int t = threadIdx.x; t < ntask; t += kBlock) {
+  for (int t = threadIdx.x; t < ntask; t += B) {
     const int line = t / nch, ch = t - line * nch, s0 = ch * L;
     T* rl = r + line * S;
     const T* el = e2 + line * nch;
@@ -422,7 +568,7 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
   }
   __syncthreads();
   if (!PER) {
-    for (int t = threadIdx.x; t < ntask; t += kBlock) {
+    for (int t = threadIdx.x; t < ntask; t += B) {
       const int line = t / nch, ch = t - line * nch, s0 = ch * L;
       T* rl = r + line * S;
       const T p0 = bnd[line * 5 + 3], pn = bnd[line * 5 + 4];
@@ -449,27 +595,32 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
 // axis 1: y-lines of qT [4][nx][ny], flux_y, alpha_y, dy.
 // OOP/spatial_operator.py:179-249, :321-342; non-periodic
 // OOP/problems/riemann_config3.py:165-265, :344-357.
+constexpr int kLineBlock = 512;
 template <class T, bool PER>
-__global__ void __launch_bounds__(kBlock) k_line(Params<T> P, LineK<T> K, int axis, const T* __restrict__ ql,
-                                                 const uint8_t* __restrict__ ml, const double* __restrict__ scal,
-                                                 T* __restrict__ out) {
+__global__ void __launch_bounds__(kLineBlock) k_line(Params<T> P, LineK<T> K, int axis, int enforced_input,
+                                                     const T* __restrict__ ql, const uint8_t* __restrict__ ml,
+                                                     const double* __restrict__ scal, T* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int BL = kLineBlock;
   const int n = K.n, S = n + 1, line = blockIdx.x;
   T* A = reinterpret_cast<T*>(smem);
   T* B = A + 4 * S;
   T* C = B + 4 * S;
   T* PW = C + 4 * S;                // 66
   T* scratch = PW + 66;             // 8*nch + 20
-  uint8_t* Mk = reinterpret_cast<uint8_t*>(scratch + 8 * K.nch + 20);
+  int* wl = reinterpret_cast<int*>(scratch + 8 * K.nch + 20);  // WENO work list, n+1
+  uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + S);
+  __shared__ int wcnt;
   const size_t NL = (size_t)gridDim.x * n;
   const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
   const T h = axis == 0 ? P.dx : P.dy;
-  for (int k = threadIdx.x; k < 66; k += kBlock) PW[k] = K.pw[k];
-  // 1. enforce, primitives, point flux
-  for (int i = threadIdx.x; i < n; i += kBlock) {
+  for (int k = threadIdx.x; k < 66; k += BL) PW[k] = K.pw[k];
+  if (threadIdx.x == 0) wcnt = 0;
+  // 1. (enforce), primitives, point flux
+  for (int i = threadIdx.x; i < n; i += BL) {
     const size_t g = (size_t)line * n + i;
     T a = ql[g], b = ql[NL + g], c = ql[2 * NL + g], d = ql[3 * NL + g];
-    enforce(P, a, b, c, d);
+    if (!enforced_input) enforce(P, a, b, c, d);
     const Prim<T> pr = prim(P, a, b, c, d);
     A[i] = a;
     A[S + i] = b;
@@ -499,59 +650,61 @@ __global__ void __launch_bounds__(kBlock) k_line(Params<T> P, LineK<T> K, int ax
     return (Mk[clampi(k - 1, 0, n - 1)] | Mk[clampi(k, 0, n - 1)]) != 0;
   };
   auto nidx = [&](int j) -> int { return PER ? wrap(j, n) : clampi(j, 0, n - 1); };
-  // 2. WENO7 interface flux where a flagged interface's compactification uses it
-  for (int k = threadIdx.x; k < nI; k += kBlock) {
-    if (!(P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1))) continue;
+  // 2a. compact the interfaces whose WENO7 flux a flagged interface's
+  //     compactification reads (i-1, i, i+1 of every flagged i)
+  for (int k = threadIdx.x; k < nI; k += BL)
+    if (P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1)) wl[atomicAdd(&wcnt, 1)] = k;
+  __syncthreads();
+  // 2b. WENO7 of the Lax-Friedrichs split, one (interface, component) per item
+  const int nw = wcnt;
+  for (int t = threadIdx.x; t < 4 * nw; t += BL) {
+    const int c = t / nw, k = wl[t - c * nw];
     const int il = PER ? k : k - 1;
-    for (int c = 0; c < 4; ++c) {
-      const T* qa = A + c * S;
-      const T* fa = B + c * S;
-      T fp[7], fm[7];
+    const T* qa = A + c * S;
+    const T* fa = B + c * S;
+    T fp[7], fm[7];
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
-        const int ip = nidx(il - 3 + j);   // f+ stencil i-3 .. i+3
-        const int im = nidx(il + 4 - j);   // f- stencil i+4 .. i-2
-        fp[j] = T(0.5) * (fa[ip] + alpha * qa[ip]);
-        fm[j] = T(0.5) * (fa[im] - alpha * qa[im]);
-      }
-      C[c * S + k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
-                     weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
+    for (int j = 0; j < 7; ++j) {
+      const int ip = nidx(il - 3 + j);   // f+ stencil i-3 .. i+3
+      const int im = nidx(il + 4 - j);   // f- stencil i+4 .. i-2
+      fp[j] = T(0.5) * (fa[ip] + alpha * qa[ip]);
+      fm[j] = T(0.5) * (fa[im] - alpha * qa[im]);
     }
+    C[c * S + k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
+                   weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
   }
   __syncthreads();
   // 3. hybrid interface flux into A
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
-  for (int k = threadIdx.x; k < nI; k += kBlock) {
+  for (int t = threadIdx.x; t < 4 * nI; t += BL) {
+    const int c = t / nI, k = t - c * nI;
     const bool fl = P.weno_only || iflag(k);
     const int il = PER ? k : k - 1;
-    const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
-    const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
-    for (int c = 0; c < 4; ++c) {
-      T F;
-      if (fl) {
-        const T* w = C + c * S;
-        F = (K.alpha_c * w[kr] + w[k]) + K.alpha_c * w[kl];
-      } else {
-        const T* f = B + c * S;
-        F = (c3 * (f[nidx(il + 3)] + f[nidx(il - 2)]) + c2 * (f[nidx(il + 2)] + f[nidx(il - 1)])) +
-            c1 * (f[nidx(il + 1)] + f[nidx(il)]);
-      }
-      A[c * S + k] = F;
+    T F;
+    if (fl) {
+      const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
+      const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
+      const T* wv = C + c * S;
+      F = (K.alpha_c * wv[kr] + wv[k]) + K.alpha_c * wv[kl];
+    } else {
+      const T* f = B + c * S;
+      F = (c3 * (f[nidx(il + 3)] + f[nidx(il - 2)]) + c2 * (f[nidx(il + 2)] + f[nidx(il - 1)])) +
+          c1 * (f[nidx(il + 1)] + f[nidx(il)]);
     }
+    A[c * S + k] = F;
   }
   __syncthreads();
   // 4. divided difference into B
-  for (int i = threadIdx.x; i < n; i += kBlock) {
-    for (int c = 0; c < 4; ++c) {
-      const T* F = A + c * S;
-      B[c * S + i] = PER ? (F[i] - F[wrap(i - 1, n)]) / h : (F[i + 1] - F[i]) / h;
-    }
+  for (int t = threadIdx.x; t < 4 * n; t += BL) {
+    const int c = t / n, i = t - c * n;
+    const T* F = A + c * S;
+    B[c * S + i] = PER ? (F[i] - F[wrap(i - 1, n)]) / h : (F[i + 1] - F[i]) / h;
   }
   __syncthreads();
   // 5. compact solve, 6. store
   solve_lines<T, PER>(B, S, K, PW, scratch);
-  for (int i = threadIdx.x; i < n; i += kBlock) {
+  for (int i = threadIdx.x; i < n; i += BL) {
     const size_t g = (size_t)line * n + i;
     for (int c = 0; c < 4; ++c) out[c * NL + g] = B[c * S + i];
   }
@@ -708,7 +861,7 @@ __device__ __forceinline__ T lap_at(const Params<T>& P, const T* __restrict__ a,
 
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict__ q, const uint8_t* __restrict__ active,
-                                               int guard, T* __restrict__ out) {
+                                               int guard, int invert, T* __restrict__ out) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x >= nx || y >= ny) return;
@@ -724,7 +877,7 @@ __global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict_
     xl = clampi(cx - 1, 0, nx - 1); xr = clampi(cx + 1, 0, nx - 1);
     yl = clampi(cy - 1, 0, ny - 1); yr = clampi(cy + 1, 0, ny - 1);
   }
-  const bool act = active[cy * nx + cx] != 0;
+  const bool act = (active[cy * nx + cx] != 0) != (invert != 0);
   T o[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {

@@ -265,8 +265,7 @@ class HITSolver:
         self.last_dt = dt
         if c.hyperviscosity_mn > 0.0 and self.steps % c.hyperviscosity_interval == 0:
             weno = self.step_plan.sensor(q_new, enforce_first=False, out=self._mask)
-            active = 1 - weno
-            q_hv = self.hv_plan.hyperviscosity(q_new, active, apply_guard=False, out=self._spare)
+            q_hv = self.hv_plan.hyperviscosity(q_new, weno, apply_guard=False, out=self._spare, invert=True)
             self._spare = q_new
             q_new = q_hv
         self.q = q_new

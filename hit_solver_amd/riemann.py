@@ -342,8 +342,8 @@ class RiemannSolver:
         self._spare = self.q
         if self.hv_enabled and (self.steps + 1) % c.hyperviscosity_interval == 0:
             weno = self._sensor_plan.sensor(q, enforce_first=False, out=self._mask)
-            active = 1 - weno
-            q_new = self._hv_plan.hyperviscosity(q, active, apply_guard=c.guard_cells > 0, out=self._spare)
+            q_new = self._hv_plan.hyperviscosity(q, weno, apply_guard=c.guard_cells > 0, out=self._spare,
+                                                 invert=True)
             self._spare = q
             q = q_new
             self.hv_applications += 1
@@ -353,6 +353,44 @@ class RiemannSolver:
             validate_raw_physical_state(q, self.equation, f"state at step {self.steps}")
         self.q = q
         return q
+
+    def capture(self, before_capture=None):
+        """Record one step (3 fused stages + the hyperviscosity pass) as a HIP
+        graph.  The fixed dt and the buffer rotation make every step launch the
+        same kernels on the same buffers when hyperviscosity runs every step."""
+        c = self.config
+        if self.hv_enabled and c.hyperviscosity_interval != 1:
+            raise ValueError("graph capture needs hyperviscosity every step (or none)")
+        if not self.hv_enabled:
+            raise ValueError("graph capture is implemented for the hyperviscosity-every-step loop")
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._step_kernels()  # warm-up on the capture stream (plans, buffers)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        if before_capture is not None:
+            before_capture()
+        self._graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self._graph):
+            self._step_kernels()
+        torch.cuda.synchronize()
+        return self._graph
+
+    def _step_kernels(self):
+        """Step body without host-side bookkeeping (used for capture); the
+        buffers come back to the same places after the hyperviscosity pass."""
+        q = self.integrator.step(self.q, self.rhs, self.dt, clean=self.rhs.clean, out=self._spare)
+        weno = self._sensor_plan.sensor(q, enforce_first=False, out=self._mask)
+        self._hv_plan.hyperviscosity(q, weno, apply_guard=self.config.guard_cells > 0, out=self.q, invert=True)
+
+    def replay(self, n: int = 1):
+        """Advance n steps by replaying the captured graph."""
+        for _ in range(n):
+            self._graph.replay()
+            self.steps += 1
+            self.hv_applications += 1
+        self.time = self.steps * self.dt
 
     def run(self, max_steps: int | None = None):
         n = self.n_steps if max_steps is None else min(max_steps, self.n_steps)

@@ -114,7 +114,7 @@ class Plan:
         check(lib().hitx_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
         self.ws_bytes = int(lib().hitx_workspace_bytes(h))
-        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device="cuda")
+        self.ws = torch.zeros(self.ws_bytes, dtype=torch.uint8, device="cuda")  # ticket starts at 0
         self.red = torch.zeros(64, dtype=torch.float64, device="cuda")
         self.device = torch.cuda.current_device()
 
@@ -163,11 +163,13 @@ class Plan:
                                    float(dt), _p(self.ws), self.ws_bytes, _stream()))
         return out
 
-    def hyperviscosity(self, q, active_u8, apply_guard: bool, out=None):
+    def hyperviscosity(self, q, mask_u8, apply_guard: bool, out=None, invert: bool = False):
+        """Filter where mask != 0 (invert=False: mask marks active nodes) or
+        where mask == 0 (invert=True: mask is the WENO-node mask)."""
         self._q(q)
-        require_device(active_u8, "active mask", torch.uint8, (self.spec.ny, self.spec.nx))
+        require_device(mask_u8, "mask", torch.uint8, (self.spec.ny, self.spec.nx))
         out = torch.empty_like(q) if out is None else out
-        check(lib().hitx_hyperviscosity(self.handle, _p(q), _p(active_u8), _p(out), int(apply_guard),
+        check(lib().hitx_hyperviscosity(self.handle, _p(q), _p(mask_u8), int(invert), _p(out), int(apply_guard),
                                         _p(self.ws), self.ws_bytes, _stream()))
         return out
 

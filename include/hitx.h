@@ -95,7 +95,10 @@ typedef struct hitx_plan hitx_plan;
 /* Plan lifetime.  The plan binds to the HIP device current at creation. */
 int hitx_plan_create(const hitx_desc* desc, hitx_plan** out);
 int hitx_plan_destroy(hitx_plan* plan);
-/* Bytes of device workspace that every compute entry point below needs. */
+/* Bytes of device workspace that every compute entry point below needs.
+ * The workspace must be zero-filled before its first use (it holds a
+ * reduction ticket that every call leaves at zero again) and must not be
+ * shared by calls that can run concurrently. */
 size_t hitx_workspace_bytes(const hitx_plan* plan);
 const char* hitx_last_error(void);
 int hitx_abi_version(void);
@@ -121,10 +124,11 @@ int hitx_rk3_stage(hitx_plan* plan, int32_t stage, const void* q0, const void* q
                    const void* fx, const void* fy, double dt, void* ws, size_t ws_bytes,
                    void* stream);
 
-/* Biharmonic filter on the nodes where active[ny][nx] != 0, then
- * enforce_physical_state, then the outflow guard if apply_guard.  out must
- * not alias q. */
-int hitx_hyperviscosity(hitx_plan* plan, const void* q, const uint8_t* active, void* out,
+/* Biharmonic filter on the nodes where (active[ny][nx] != 0) xor invert --
+ * invert = 1 takes the sensor's WENO mask and filters the compact nodes --
+ * then enforce_physical_state, then the outflow guard if apply_guard.  out
+ * must not alias q. */
+int hitx_hyperviscosity(hitx_plan* plan, const void* q, const uint8_t* active, int32_t invert, void* out,
                         int32_t apply_guard, void* ws, size_t ws_bytes, void* stream);
 
 /* out = enforce_physical_state(q), then the outflow guard if apply_guard. */

@@ -7,8 +7,11 @@ Tolerances (FP64):
   periodic compact system by FFT and the non-periodic one by sequential
   Thomas; the kernels use a chunked recurrence -- equal to rounding);
 * one SSP-RK3 step: relative L2 <= 1e-12 per conserved field;
-* multi-step runs: relative L2 <= 1e-10 per field (rounding-level
-  differences grow slowly through the nonlinear steps).
+* multi-step runs: relative L2 <= 10x the oracle's own divergence under a
+  1-ulp perturbation of the initial state, measured in the same test (floor
+  1e-12).  WENO7's smoothness indicators amplify rounding noise on nearly
+  flat data (the reference's CPU and GPU runs diverge the same way): on
+  Riemann C3 at 48x40 the oracle moves 1.7e-10 from itself in 8 steps.
 """
 
 import numpy as np
@@ -43,6 +46,29 @@ def assert_fields(a, b, tol, what=""):
     for k in range(4):
         e = rel_l2(a[k], b[k])
         assert e <= tol, f"{what} field {k}: rel L2 {e:.3e} > {tol:.1e}"
+
+
+def ulp_sensitivity(make_run, steps, seed=0):
+    """Per-field rel-L2 between two oracle runs whose initial states differ
+    by one ulp per entry."""
+    a, b = make_run(), make_run()
+    rng = np.random.default_rng(seed)
+    up = rng.random(b.q.shape) < 0.5
+    b.q = np.where(up, np.nextafter(b.q, np.inf), np.nextafter(b.q, -np.inf))
+    for r in (a, b):
+        if hasattr(r, "n_steps"):
+            r.run(max_steps=steps)
+        else:
+            while not r.done():
+                r.step()
+    return [rel_l2(b.q[k], a.q[k]) for k in range(4)], a
+
+
+def assert_within_sensitivity(got, sens, ref_q, what, factor=10.0):
+    for k in range(4):
+        e = rel_l2(got[k], ref_q[k])
+        lim = max(factor * sens[k], 1e-12)
+        assert e <= lim, f"{what} field {k}: rel L2 {e:.3e} > {factor:g} x 1-ulp sensitivity {sens[k]:.3e}"
 
 
 def assert_rhs(a, b, tol=1e-12, what=""):
@@ -186,7 +212,7 @@ def test_hit_run_plain_golden(golden):
     q, t, steps = run_simulation(HIT2DConfig(tfinal=0.02, viscosity=0.0, **_hit_common()))
     assert steps == int(golden["F4_plain_steps"])
     assert t == pytest.approx(float(golden["F4_plain_t"]), rel=1e-13)
-    assert_fields(host(q), golden["F4_plain_q"], 1e-10, "HIT plain run")
+    assert_fields(host(q), golden["F4_plain_q"], 1e-10, "HIT plain run")  # smooth flow: no amplification
 
 
 def test_hit_run_production_golden(golden):
@@ -211,8 +237,12 @@ def test_riemann_runs_golden(golden, tag, cls, kw):
     q, diag = R.run_riemann(cfg)
     assert diag.steps == int(golden[f"F5_{tag}_steps"])
     assert diag.hyperviscosity_applications == int(golden[f"F5_{tag}_hvapps"])
-    assert_fields(host(q), golden[f"F5_{tag}_q"], 1e-10, f"Riemann {tag}")
-    assert diag.final_sensor_fraction == pytest.approx(float(golden[f"F5_{tag}_sensor_fraction"]), abs=1e-12)
+    factory = {"c3": OR.config3, "c308": OR.config3_08, "c6": OR.config6}[tag]
+    p = factory(tfinal=0.02, shear_threshold=None, progress_every=10_000, **kw)
+    sens, ref = ulp_sensitivity(lambda: OR.RiemannRun(p), diag.steps)
+    np.testing.assert_array_equal(ref.q, golden[f"F5_{tag}_q"])  # the oracle run is the reference's
+    assert_within_sensitivity(host(q), sens, golden[f"F5_{tag}_q"], f"Riemann {tag}")
+    assert abs(diag.final_sensor_fraction - float(golden[f"F5_{tag}_sensor_fraction"])) <= 2.0 / (cfg.nx * cfg.ny)
 
 
 # ---------------------------------------------------------------- vs oracle, other sizes
@@ -247,12 +277,27 @@ def test_riemann_oracle_multistep_128():
     from hit_solver_amd.riemann import RiemannConfig3, RiemannSolver
 
     p = OR.config3(nx=128, ny=128, shear_threshold=None, progress_every=10_000)
-    ref = OR.RiemannRun(p)
-    ref.run(max_steps=10)
+    sens, ref = ulp_sensitivity(lambda: OR.RiemannRun(p), 10)
     s = RiemannSolver(RiemannConfig3(nx=128, ny=128, shear_threshold=None, progress_every=10_000))
     assert s.n_steps == ref.n_steps and s.dt == pytest.approx(ref.dt, rel=1e-15)
     s.run(max_steps=10)
-    assert_fields(host(s.q), ref.q, 1e-11, "Riemann 128^2 x10")
+    assert_within_sensitivity(host(s.q), sens, ref.q, "Riemann 128^2 x10")
+
+
+def test_riemann_graph_replay_matches_eager():
+    """A captured step replayed from the graph gives the eager steps' result."""
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannSolver
+
+    cfg = RiemannConfig3(nx=96, ny=80, shear_threshold=None, progress_every=10_000)
+    a, b = RiemannSolver(cfg), RiemannSolver(cfg)
+    a.run(max_steps=6)
+    b.run(max_steps=1)
+    b.capture()  # capture runs one step while recording
+    b.steps += 1
+    for _ in range(4):
+        b.replay(1)
+        torch.cuda.synchronize()
+    np.testing.assert_array_equal(host(b.q), host(a.q))
 
 
 def test_fp32_step_error_vs_oracle():
