@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--kernel-steps", type=int, default=10, help="eager steps timed per sweep launch")
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--dtype", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the host oracle sample")
@@ -86,16 +87,15 @@ def main():
         s.step(validate=False)
     plan = s.rhs.plan
     torch.cuda.synchronize()
-    # Capture one step as a HIP graph with the line-sweep events inside it; if
-    # event capture is refused, capture without them and time the sweeps on
-    # eager steps right after the timed region.
-    events_in_graph = True
-    try:
-        s.capture(before_capture=lambda: plan.timing(True))
-    except RuntimeError:
-        events_in_graph = False
-        plan.timing(False)
-        s.capture()
+    # Line-sweep kernel time: HIP events recorded around each sweep launch on
+    # its stream, over eager steps that run just before the timed replays
+    # (events recorded inside a captured graph cannot be read back).
+    plan.timing(True)
+    for _ in range(args.kernel_steps):
+        s.step(validate=False)
+    line_ms, line_n = plan.timing_read()
+    plan.timing(False)
+    s.capture()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -104,14 +104,6 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    if events_in_graph:
-        line_ms, line_n = plan.timing_read()  # the launches of the last replayed step
-    else:
-        plan.timing(True)
-        for _ in range(5):
-            s.step(validate=False)
-        line_ms, line_n = plan.timing_read()
-    plan.timing(False)
     ev_s = e0.elapsed_time(e1) / 1e3
     T = max(wall, ev_s)
     value = nx * nx * args.steps / T / 1e6
@@ -140,8 +132,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "launches": line_n, "avg_launch_us": round(line_avg_s * 1e6, 2),
                      "alg_bytes_per_launch": line_bytes,
-                     "timing": "HIP events in the replayed step graph" if events_in_graph else
-                               "HIP events on eager steps after the timed region"},
+                     "timing": f"HIP events around each sweep launch on its stream, {args.kernel_steps} eager "
+                               "steps before the timed graph replays"},
         "graph": "one step (3 fused stages + hyperviscosity) captured as a HIP graph, replayed",
         "step_roofline": {"alg_bytes_per_cell_update": STEP_BYTES[args.dtype],
                           "achieved_GBs": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9, 1),

@@ -1,0 +1,205 @@
+"""Batched runs: many HIT or Riemann instances of different grid sizes advance
+together through ``hitx_batch_step`` (one launch per kernel kind per stage for
+all instances, per-instance scalar control on the device).
+
+Each instance follows the reference's step loop for its own configuration
+(OOP/hit2d.py:764-918 or OOP/problems/riemann_config3.py:681-749).  The
+forcing noise of each HIT instance is its own NumPy PCG64 stream
+(default_rng(forcing_seed + 1)), drawn on the host and copied into the
+instance's noise buffer before each step.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .forcing import shell_modes
+from .hit2d import HIT2DConfig, _make_equation, create_grid, drag_modes, initialize_hit_2d
+from .riemann import RiemannConfig3, hyperviscosity_enabled_for_config
+from .runtime import PlanSpec, get_plan, make_desc
+from .spatial_operator import PeriodicHyperviscosity2D
+
+NAN = float("nan")
+# control-scalar slots (include/hitx.h)
+C_ACTIVE, C_TIME, C_TFINAL, C_DT, C_STEP, C_MAXSTEPS = 0, 1, 2, 3, 4, 5
+C_HVAPPS, C_POWER, C_ALPHA, C_NSTEPS = 7, 11, 12, 26
+C_FXX, C_FYY, C_FXY, C_PIN, C_PBEFORE, C_ATGT, C_MACH, C_PDES = 18, 19, 20, 21, 22, 23, 24, 25
+BUF_STATE, BUF_NOISE, BUF_CTL = 0, 1, 2
+
+
+def _none(v):
+    return NAN if v is None else float(v)
+
+
+def hit_spec_kpar(c: HIT2DConfig):
+    """PlanSpec and run parameters of one forced-HIT instance."""
+    domain, _, _ = create_grid(c)
+    eq = _make_equation(c)
+    hv = PeriodicHyperviscosity2D(domain, mn=c.hyperviscosity_mn)
+    dm = drag_modes(domain, c.large_scale_drag_kmax) if c.large_scale_drag > 0.0 and c.large_scale_drag_kmax > 0.0 \
+        else np.zeros((0, 2), int)
+    fmodes, _, _ = shell_modes(domain, 0.0 if c.forcing_kmin is None else c.forcing_kmin, c.forcing_kmax)
+    spec = PlanSpec(
+        nx=c.nx, ny=c.ny, dx=domain.dx, dy=domain.dy, periodic=True, dtype=c.torch_dtype, gamma=c.gamma,
+        sensor_width=c.sensor_width, compression_threshold=c.compression_threshold,
+        jump_threshold=c.jump_threshold, shear_threshold=c.shear_threshold, viscosity=c.viscosity,
+        prandtl=c.prandtl, hv_coef=hv.coefficients(), forcing=True,
+        drag_coef=c.large_scale_drag if len(dm) else 0.0, drag_modes=tuple(int(v) for v in dm.reshape(-1)),
+        cooling_time=c.cooling_time if (c.cooling_time is not None and c.cooling_time > 0.0) else None,
+        cooling_p_ref=c.cooling_target_pressure, force_modes=tuple(int(v) for v in fmodes.reshape(-1)),
+    )
+    h = min(domain.dx, domain.dy)
+    tgt = c.target_mach if c.mach_control_target is None else c.mach_control_target
+    k = [0.0] * _lib_kpar()
+    k[0] = c.cfl
+    k[1] = 0.25 * h**2 / c.viscosity if c.viscosity > 0.0 else NAN
+    k[2] = 0.0
+    k[3] = float(c.hyperviscosity_interval)
+    k[4] = 1.0 if c.hyperviscosity_mn > 0.0 else 0.0
+    k[5] = 1.0 if c.mach_control else 0.0
+    k[6], k[7], k[8] = float(tgt), c.mach_control_memory, c.mach_control_exponent
+    k[9], k[10] = _none(c.mach_control_min_power), _none(c.mach_control_max_power)
+    k[11], k[12], k[13] = _none(c.target_energy_injection), c.min_forcing_power, _none(c.max_forcing_rescale)
+    k[14], k[15], k[16], k[17] = c.forcing_alpha_memory, c.forcing_rms, c.forcing_correlation_time, 1.0
+    k[18] = c.tfinal
+    k[19] = 1e300 if c.max_steps is None else float(c.max_steps)
+    return spec, k, domain, eq
+
+
+def riemann_spec_kpar(c: RiemannConfig3, max_steps=None):
+    from .riemann import LocalHyperviscosity2D
+
+    d = c.domain
+    hv = LocalHyperviscosity2D(d, mn=c.mn, density_weight=c.hyperviscosity_density_weight,
+                               momentum_weight=c.hyperviscosity_momentum_weight,
+                               energy_weight=c.hyperviscosity_energy_weight)
+    spec = PlanSpec(
+        nx=c.nx, ny=c.ny, dx=d.dx, dy=d.dy, periodic=False, dtype=c.torch_dtype, scheme=c.scheme, gamma=c.gamma,
+        sensor_width=c.sensor_width, boundary_guard=c.boundary_guard,
+        compression_threshold=c.compression_threshold, jump_threshold=c.jump_threshold,
+        shear_threshold=c.shear_threshold, hv_coef=hv.coefficients(), guard_cells=c.guard_cells,
+    )
+    k = [0.0] * _lib_kpar()
+    k[0] = c.cfl
+    k[1] = NAN
+    k[2] = 1.0
+    k[3] = float(c.hyperviscosity_interval)
+    k[4] = 1.0 if hyperviscosity_enabled_for_config(c) else 0.0
+    k[18] = c.tfinal
+    k[19] = 1e300 if max_steps is None else float(max_steps)
+    return spec, k
+
+
+def _lib_kpar():
+    return 24  # HITX_KPAR
+
+
+class BatchRunner:
+    """All instances share periodicity, dtype, scheme, sensor width and the drag
+    on/off setting (one kernel instantiation per batch)."""
+
+    def __init__(self, configs, max_steps=None):
+        if not configs:
+            raise ValueError("empty batch")
+        self.configs = list(configs)
+        self.hit = isinstance(self.configs[0], HIT2DConfig)
+        specs, kpars, raw = [], [], []
+        self.rngs = []
+        for c in self.configs:
+            if self.hit != isinstance(c, HIT2DConfig):
+                raise ValueError("a batch holds either HIT or Riemann instances")
+            if self.hit:
+                spec, k, domain, eq = hit_spec_kpar(c)
+                rng = np.random.default_rng(c.forcing_seed)
+                raw.append(initialize_hit_2d(c, domain, eq, rng))
+                self.rngs.append(np.random.default_rng(c.forcing_seed + 1))
+            else:
+                spec, k = riemann_spec_kpar(c, max_steps)
+                raw.append(None)
+            specs.append(spec)
+            kpars.append(k)
+        self.specs = specs
+        n = len(specs)
+        descs = (_lib.HitxDesc * n)()
+        keep = []
+        for i, sp in enumerate(specs):
+            d, kp = make_desc(sp)
+            descs[i] = d
+            keep.append(kp)
+        kp = np.ascontiguousarray(np.array(kpars, dtype=np.float64))
+        h = ctypes.c_void_p()
+        check(lib().hitx_batch_create(descs, kp.ctypes.data_as(ctypes.c_void_p), n, ctypes.byref(h)))
+        self.handle = h
+        self.n = n
+        nbytes = int(lib().hitx_batch_workspace_bytes(h))
+        self.ws = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+        check(lib().hitx_batch_bind(h, ctypes.c_void_p(self.ws.data_ptr()), nbytes))
+        self.q = [self._view(i, BUF_STATE, sp.dtype, (4, sp.ny, sp.nx)) for i, sp in enumerate(specs)]
+        self.ctl_views = [self._view(i, BUF_CTL, torch.float64, (32,)) for i in range(n)]
+        self.noise = [self._view(i, BUF_NOISE, torch.float64, (sp.ny, sp.nx)) for i, sp in enumerate(specs)] \
+            if self.hit else None
+        # initial states: enforce (+ outflow guard), as the single-instance drivers
+        for i, (c, sp) in enumerate(zip(self.configs, specs)):
+            q0 = torch.from_numpy(raw[i]).to("cuda", sp.dtype) if self.hit else c.initial_state()
+            get_plan(sp).enforce(q0, apply_guard=not self.hit, out=self.q[i])
+            ctl = torch.zeros(32, dtype=torch.float64)
+            ctl[C_ACTIVE] = 1.0
+            ctl[C_TFINAL] = c.tfinal
+            ctl[C_MAXSTEPS] = kpars[i][19]
+            ctl[C_POWER] = kpars[i][11]
+            ctl[C_ALPHA] = 1.0
+            self.ctl_views[i].copy_(ctl)
+        if not self.hit:
+            check(lib().hitx_batch_fixed_dt(h, self._stream()))
+        self._pinned = None
+        torch.cuda.synchronize()
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().hitx_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _view(self, i, which, dtype, shape):
+        p = ctypes.c_void_p()
+        check(lib().hitx_batch_buffer(self.handle, i, which, ctypes.byref(p)))
+        off = p.value - self.ws.data_ptr()
+        nb = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
+        return self.ws[off:off + nb].view(dtype).view(*shape)
+
+    def fill_noise(self):
+        """Draw each HIT instance's next nx*ny normals on the host."""
+        for i, sp in enumerate(self.specs):
+            a = self.rngs[i].standard_normal((sp.ny, sp.nx))
+            self.noise[i].copy_(torch.from_numpy(a), non_blocking=False)
+
+    def step(self):
+        if self.hit:
+            self.fill_noise()
+        check(lib().hitx_batch_step(self.handle, self._stream()))
+
+    def control(self):
+        """Control scalars of all instances (host numpy, [n, 32]); synchronises."""
+        return torch.stack(self.ctl_views).cpu().numpy()
+
+    def active(self):
+        return self.control()[:, C_ACTIVE] != 0.0
+
+    def run(self, max_steps=None):
+        k = 0
+        while self.active().any() and (max_steps is None or k < max_steps):
+            self.step()
+            k += 1
+        return k

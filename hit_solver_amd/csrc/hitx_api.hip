@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "../../include/hitx.h"
-#include "hitx_kernels.h"
+#include "hitx_batch.h"
 
 using namespace hitx;
 
@@ -82,9 +82,9 @@ struct hitx_plan {
   LineK<float> lx_f{}, ly_f{};
   // workspace layout
   size_t off_raw = 0, off_mask = 0, off_maskT = 0, off_qT = 0, off_dfx = 0, off_dfyT = 0, off_part = 0,
-         off_scal = 0, off_misc = 0, off_ticket = 0, ws_bytes = 0;
+         off_scal = 0, off_misc = 0, ws_bytes = 0;
   size_t sensor_lds = 0;
-  int nblk_red = 0, nblk_proj = 0;
+  int nblk_red = 0, nblk_proj = 0, nblk_stage = 0;
   size_t line_lds_x = 0, line_lds_y = 0;
   // optional per-launch timing of the line-sweep kernel (HIP events on the launch stream)
   int timing = 0;
@@ -168,7 +168,6 @@ struct WS {
   uint8_t *raw, *mask, *maskT;
   void *qT, *dfx, *dfyT;
   double *part, *scal, *misc;
-  unsigned int* ticket;
 };
 
 WS carve(const hitx_plan* pl, void* ws) {
@@ -183,7 +182,6 @@ WS carve(const hitx_plan* pl, void* ws) {
   w.part = reinterpret_cast<double*>(b + pl->off_part);
   w.scal = reinterpret_cast<double*>(b + pl->off_scal);
   w.misc = reinterpret_cast<double*>(b + pl->off_misc);
-  w.ticket = reinterpret_cast<unsigned int*>(b + pl->off_ticket);
   return w;
 }
 
@@ -201,10 +199,13 @@ dim3 grid16(const hitx_plan* pl) { return dim3((pl->d.nx + 15) / 16, (pl->d.ny +
 template <class T>
 int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, T* qT, const WS& w, hipStream_t s) {
   const Params<T> P = make_params<T>(pl);
-  if (P.drag_on)
-    k_stage_reduce<T, true><<<pl->nblk_red, kBlock, 0, s>>>(P, q, enforce_first, w.part, w.ticket, w.scal);
-  else
-    k_stage_reduce<T, false><<<pl->nblk_red, kBlock, 0, s>>>(P, q, enforce_first, w.part, w.ticket, w.scal);
+  if (P.drag_on) {
+    k_stage_reduce<T, true><<<pl->nblk_stage, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_stage, w.scal);
+  } else {
+    k_stage_reduce<T, false><<<pl->nblk_stage, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_stage, w.scal);
+  }
   dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
   k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
   LAUNCHCHK();
@@ -389,6 +390,7 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   // workspace layout
   const size_t N = pl->N, E = pl->esize;
   pl->nblk_red = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 1024);
+  pl->nblk_stage = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 512);  // stage reductions: fewer partials
   pl->nblk_proj = (int)((N + (size_t)kBlock * kPPT - 1) / ((size_t)kBlock * kPPT));
   size_t o = 0;
   pl->off_raw = o; o = align256(o + N);
@@ -402,7 +404,6 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->off_part = o; o = align256(o + part * sizeof(double));
   pl->off_scal = o; o = align256(o + S_COUNT * sizeof(double));
   pl->off_misc = o; o = align256(o + 64 * sizeof(double));
-  pl->off_ticket = o; o = align256(o + sizeof(unsigned int));
   pl->ws_bytes = o;
   *out = pl;
   return HITX_OK;
@@ -621,6 +622,295 @@ int hitx_timing_read(hitx_plan* pl, double* total_ms, int32_t* launches) {
   *launches = (int32_t)(pl->ev_used / 2);
   pl->ev_used = 0;
   return HITX_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== batches
+struct hitx_batch {
+  int n = 0, esize = 8, periodic = 1, drag = 0, device = 0;
+  std::vector<hitx_plan*> plans;
+  std::vector<double> kpar;  // n * K_COUNT
+  struct Buf {
+    size_t Q, Q1, Q2, S, qT, dfx, dfyT, fx, fy, mask, maskT, part, scal, ctl, kpar, phi, fhat, noise, fpart, kxy, rng;
+  };
+  std::vector<Buf> buf;
+  size_t off_tab = 0, off_offs = 0, ws_bytes = 0;
+  int total[OFF_KINDS] = {0};
+  std::vector<int> offs;  // OFF_KINDS * (n + 1)
+  size_t line_lds = 0, sensor_lds = 0;
+  void* ws = nullptr;
+};
+
+namespace {
+
+int batch_blocks(const hitx_plan* pl, int kind) {
+  const int nx = pl->d.nx, ny = pl->d.ny;
+  switch (kind) {
+    case OFF_RED: return pl->nblk_stage;
+    case OFF_SENS: return ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
+    case OFF_LX: return ny;
+    case OFF_LY: return nx;
+    case OFF_T16: return ((nx + 15) / 16) * ((ny + 15) / 16);
+    case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
+    default: return 0;
+  }
+}
+
+template <class T>
+void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
+  const hitx_plan* pl = b->plans[i];
+  const hitx_batch::Buf& o = b->buf[i];
+  I.P = make_params<T>(pl);
+  I.Kx = linek<T>(pl, 0);
+  I.Ky = linek<T>(pl, 1);
+  auto t = [&](size_t off) { return reinterpret_cast<T*>(base + off); };
+  auto d = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
+  I.Q = t(o.Q); I.Q1 = t(o.Q1); I.Q2 = t(o.Q2); I.S = t(o.S);
+  I.qT = t(o.qT); I.dfx = t(o.dfx); I.dfyT = t(o.dfyT); I.fx = t(o.fx); I.fy = t(o.fy);
+  I.mask = base + o.mask; I.maskT = base + o.maskT;
+  I.part = d(o.part); I.scal = d(o.scal); I.ctl = d(o.ctl); I.kpar = d(o.kpar); I.phi = d(o.phi);
+  I.fhat = d(o.fhat); I.noise = d(o.noise); I.fpart = d(o.fpart); I.kxy = d(o.kxy);
+  I.rng = reinterpret_cast<uint64_t*>(base + o.rng);
+  I.nblk_stage = pl->nblk_stage;
+  I.nblk_proj = pl->nblk_proj;
+}
+
+template <class T>
+int batch_bind_t(hitx_batch* b, void* ws) {
+  uint8_t* base = static_cast<uint8_t*>(ws);
+  std::vector<Inst<T>> tab(b->n);
+  for (int i = 0; i < b->n; ++i) fill_inst<T>(b, i, base, tab[i]);
+  HIPCHK(hipMemcpy(base + b->off_tab, tab.data(), tab.size() * sizeof(Inst<T>), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(base + b->off_offs, b->offs.data(), b->offs.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (int i = 0; i < b->n; ++i) {
+    const hitx_plan* pl = b->plans[i];
+    HIPCHK(hipMemcpy(base + b->buf[i].kpar, &b->kpar[(size_t)i * K_COUNT], K_COUNT * sizeof(double),
+                     hipMemcpyHostToDevice));
+    // physical wavenumbers of the forcing modes: 2*pi * fftfreq(n, d) as NumPy forms it
+    const int nf = pl->d.n_force_modes;
+    if (nf > 0) {
+      std::vector<double> kxy(2 * nf);
+      const double vx = 1.0 / ((double)pl->d.nx * pl->d.dx), vy = 1.0 / ((double)pl->d.ny * pl->d.dy);
+      for (int m = 0; m < nf; ++m) {
+        kxy[2 * m] = (2.0 * M_PI) * ((double)pl->force_modes[2 * m] * vx);
+        kxy[2 * m + 1] = (2.0 * M_PI) * ((double)pl->force_modes[2 * m + 1] * vy);
+      }
+      HIPCHK(hipMemcpy(base + b->buf[i].kxy, kxy.data(), kxy.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  return HITX_OK;
+}
+
+template <class T>
+const Inst<T>* btab(const hitx_batch* b) {
+  return reinterpret_cast<const Inst<T>*>(static_cast<uint8_t*>(b->ws) + b->off_tab);
+}
+const int* boffs(const hitx_batch* b) {
+  return reinterpret_cast<const int*>(static_cast<uint8_t*>(b->ws) + b->off_offs);
+}
+
+template <class T, bool DRAG>
+int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
+  const Inst<T>* tab = btab<T>(b);
+  const int* offs = boffs(b);
+  const int n = b->n;
+  b_stage_reduce<T, DRAG><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n, st);
+  b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st);
+  b_sensor<T><<<b->total[OFF_SENS], kBlock, b->sensor_lds, s>>>(tab, offs, n, st);
+  if (st == 3) {
+    LAUNCHCHK();
+    return HITX_OK;
+  }
+  if (b->periodic) {
+    b_line<T, true><<<b->total[OFF_LX], kLineBlock, b->line_lds, s>>>(tab, offs, n, 0, st);
+    b_line<T, true><<<b->total[OFF_LY], kLineBlock, b->line_lds, s>>>(tab, offs, n, 1, st);
+  } else {
+    b_line<T, false><<<b->total[OFF_LX], kLineBlock, b->line_lds, s>>>(tab, offs, n, 0, st);
+    b_line<T, false><<<b->total[OFF_LY], kLineBlock, b->line_lds, s>>>(tab, offs, n, 1, st);
+  }
+  b_update<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n, st);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+template <class T>
+int batch_step_t(hitx_batch* b, hipStream_t s) {
+  const Inst<T>* tab = btab<T>(b);
+  const int* offs = boffs(b);
+  const int n = b->n;
+  b_step_reduce<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
+  b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab);
+  if (b->periodic && b->total[OFF_PROJ] > 0) {
+    b_fproject<T><<<b->total[OFF_PROJ], kBlock, 0, s>>>(tab, offs, n);
+    b_fou<T><<<n, kBlock, 0, s>>>(tab);
+    b_fsynth<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
+    b_falpha<T><<<n, kBlock, 0, s>>>(tab);
+    b_fscale<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
+    b_fstats<T><<<n, kBlock, 0, s>>>(tab);
+  }
+  LAUNCHCHK();
+  for (int st = 0; st < 4; ++st) {
+    int rc = b->drag ? batch_stage_sequence<T, true>(b, st, s) : batch_stage_sequence<T, false>(b, st, s);
+    if (rc) return rc;
+  }
+  b_finish<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n);
+  b_ctl_end<T><<<(n + 63) / 64, 64, 0, s>>>(tab, n);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+template <class T>
+int batch_fixed_dt_t(hitx_batch* b, hipStream_t s) {
+  const Inst<T>* tab = btab<T>(b);
+  b_step_reduce<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, boffs(b), b->n);
+  b_fixed_dt<T><<<b->n, kBlock, 0, s>>>(tab);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+template <class T>
+int batch_set_lds(hitx_batch* b) {
+  int maxsh = 0;
+  HIPCHK(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
+  if (b->line_lds > (size_t)maxsh) return fail(HITX_EINVAL, "grid line too long for LDS");
+  if (b->line_lds > 65536) {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&b_line<T, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->line_lds));
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&b_line<T, false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->line_lds));
+  }
+  return HITX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hitx_batch** out) {
+  if (!out || !descs || !kpar || n <= 0) return fail(HITX_EINVAL, "bad batch arguments");
+  *out = nullptr;
+  for (int i = 1; i < n; ++i) {
+    const hitx_desc &a = descs[0], &c = descs[i];
+    if (a.periodic != c.periodic || a.dtype != c.dtype || a.scheme != c.scheme || a.sensor_width != c.sensor_width ||
+        (a.drag_enabled != 0) != (c.drag_enabled != 0))
+      return fail(HITX_EINVAL, "batch instances must share periodic, dtype, scheme, sensor_width and drag on/off");
+  }
+  hitx_batch* b = new hitx_batch();
+  b->n = n;
+  b->periodic = descs[0].periodic;
+  b->esize = descs[0].dtype == HITX_F64 ? 8 : 4;
+  b->kpar.assign(kpar, kpar + (size_t)n * K_COUNT);
+  (void)hipGetDevice(&b->device);
+  for (int i = 0; i < n; ++i) {
+    hitx_plan* pl = nullptr;
+    int rc = hitx_plan_create(&descs[i], &pl);
+    if (rc) {
+      std::string keep = g_err;
+      hitx_batch_destroy(b);
+      g_err = "instance " + std::to_string(i) + ": " + keep;
+      return rc;
+    }
+    b->plans.push_back(pl);
+    b->line_lds = std::max({b->line_lds, pl->line_lds_x, pl->line_lds_y});
+    b->sensor_lds = std::max(b->sensor_lds, pl->sensor_lds);
+  }
+  b->drag = make_params<double>(b->plans[0]).drag_on;
+  int rc = b->esize == 8 ? batch_set_lds<double>(b) : batch_set_lds<float>(b);
+  if (rc) {
+    std::string keep = g_err;
+    hitx_batch_destroy(b);
+    g_err = keep;
+    return rc;
+  }
+  // layout
+  size_t o = 0;
+  const size_t E = b->esize;
+  for (int i = 0; i < n; ++i) {
+    const hitx_plan* pl = b->plans[i];
+    const size_t N = pl->N;
+    const int nf = std::max(1, pl->d.n_force_modes);
+    hitx_batch::Buf u;
+    auto take = [&](size_t bytes) { size_t r = o; o = align256(o + bytes); return r; };
+    u.Q = take(4 * N * E); u.Q1 = take(4 * N * E); u.Q2 = take(4 * N * E); u.S = take(4 * N * E);
+    u.qT = take(4 * N * E); u.dfx = take(4 * N * E); u.dfyT = take(4 * N * E);
+    u.fx = take(N * E); u.fy = take(N * E);
+    u.mask = take(N); u.maskT = take(N);
+    u.part = take((size_t)pl->nblk_stage * 64 * sizeof(double));
+    u.scal = take(S_COUNT * sizeof(double));
+    u.ctl = take(C_COUNT * sizeof(double));
+    u.kpar = take(K_COUNT * sizeof(double));
+    u.phi = take(2 * nf * sizeof(double));
+    u.fhat = take(4 * nf * sizeof(double));
+    u.noise = take(pl->d.n_force_modes > 0 ? N * sizeof(double) : 8);
+    u.fpart = take((size_t)pl->nblk_proj * 2 * nf * sizeof(double));
+    u.kxy = take(2 * nf * sizeof(double));
+    u.rng = take(4 * sizeof(uint64_t));
+    b->buf.push_back(u);
+  }
+  b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);
+  for (int k = 0; k < OFF_KINDS; ++k) {
+    int acc = 0;
+    for (int i = 0; i < n; ++i) {
+      b->offs[(size_t)k * (n + 1) + i] = acc;
+      acc += batch_blocks(b->plans[i], k);
+    }
+    b->offs[(size_t)k * (n + 1) + n] = acc;
+    b->total[k] = acc;
+  }
+  const size_t inst_bytes = b->esize == 8 ? sizeof(Inst<double>) : sizeof(Inst<float>);
+  b->off_tab = o; o = align256(o + inst_bytes * n);
+  b->off_offs = o; o = align256(o + b->offs.size() * sizeof(int));
+  b->ws_bytes = o;
+  *out = b;
+  return HITX_OK;
+}
+
+int hitx_batch_destroy(hitx_batch* b) {
+  if (!b) return HITX_OK;
+  for (hitx_plan* p : b->plans) hitx_plan_destroy(p);
+  delete b;
+  return HITX_OK;
+}
+
+size_t hitx_batch_workspace_bytes(const hitx_batch* b) { return b ? b->ws_bytes : 0; }
+
+int hitx_batch_bind(hitx_batch* b, void* ws, size_t ws_bytes) {
+  if (!b || !ws) return fail(HITX_EINVAL, "null argument");
+  if (ws_bytes < b->ws_bytes) return fail(HITX_ENOMEM, "batch workspace too small");
+  b->ws = ws;
+  return b->esize == 8 ? batch_bind_t<double>(b, ws) : batch_bind_t<float>(b, ws);
+}
+
+int hitx_batch_buffer(const hitx_batch* b, int32_t i, int32_t which, void** ptr) {
+  if (!b || !ptr || i < 0 || i >= b->n) return fail(HITX_EINVAL, "bad instance index");
+  if (!b->ws) return fail(HITX_EINVAL, "batch not bound to a workspace");
+  const hitx_batch::Buf& u = b->buf[i];
+  size_t off;
+  switch (which) {
+    case HITX_BUF_STATE: off = u.Q; break;
+    case HITX_BUF_NOISE: off = u.noise; break;
+    case HITX_BUF_CTL: off = u.ctl; break;
+    case HITX_BUF_PHI: off = u.phi; break;
+    case HITX_BUF_RNG: off = u.rng; break;
+    case HITX_BUF_FX: off = u.fx; break;
+    case HITX_BUF_FY: off = u.fy; break;
+    default: return fail(HITX_EINVAL, "unknown buffer");
+  }
+  *ptr = static_cast<uint8_t*>(b->ws) + off;
+  return HITX_OK;
+}
+
+int hitx_batch_fixed_dt(hitx_batch* b, void* stream) {
+  if (!b || !b->ws) return fail(HITX_EINVAL, "batch not bound");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->esize == 8 ? batch_fixed_dt_t<double>(b, s) : batch_fixed_dt_t<float>(b, s);
+}
+
+int hitx_batch_step(hitx_batch* b, void* stream) {
+  if (!b || !b->ws) return fail(HITX_EINVAL, "batch not bound");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->esize == 8 ? batch_step_t<double>(b, s) : batch_step_t<float>(b, s);
 }
 
 }  // extern "C"

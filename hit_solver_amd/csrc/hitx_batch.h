@@ -1,0 +1,376 @@
+// Batched launches: many instances of possibly different grid sizes advance
+// together, one launch per kernel kind per stage for all of them.  Each
+// instance has a descriptor (Inst) in device memory; a kernel finds its
+// instance by binary search of the block index in a per-kind table of block
+// offsets and runs the same per-block body as the single-instance kernels.
+// Per-instance scalar control (dt, Mach control, the forcing's OU recurrence
+// and power rescale, step counting) runs in small device kernels, so a whole
+// step needs no host synchronisation and can be captured in a HIP graph.
+#pragma once
+
+#include "hitx_kernels.h"
+
+namespace hitx {
+
+// per-instance control scalars (double)
+enum {
+  C_ACTIVE = 0, C_TIME, C_TFINAL, C_DT, C_STEP, C_MAXSTEPS, C_HVNOW, C_HVAPPS,
+  C_MAXSR, C_SUV, C_SC, C_POWER, C_ALPHA, C_DECAY, C_INCR, C_SCALE,
+  C_SFF, C_SP, C_FXX, C_FYY, C_FXY, C_PIN, C_PBEFORE, C_ATGT, C_MACH, C_PDES, C_NSTEPS,
+  C_COUNT = 32
+};
+// per-instance run parameters (double, constant); NaN encodes "None"
+enum {
+  K_CFL = 0, K_VCLIP, K_FIXED_DT, K_HVINT, K_HVON, K_MACHON, K_MTGT, K_MMEM, K_MEXP, K_MMIN, K_MMAX,
+  K_PTGT, K_MINP, K_MAXR, K_AMEM, K_FRMS, K_TCORR, K_FORCE, K_TFINAL, K_MAXSTEPS, K_NSTEPS,
+  K_COUNT = 24
+};
+
+template <class T>
+struct Inst {
+  Params<T> P;
+  LineK<T> Kx, Ky;
+  T *Q, *Q1, *Q2, *S;  // state, stage 1 and 2 results, stage 3 result (before hyperviscosity)
+  T *qT, *dfx, *dfyT, *fx, *fy;
+  uint8_t *mask, *maskT;
+  double *part, *scal, *ctl, *kpar, *phi, *fhat, *noise, *fpart, *kxy;
+  uint64_t* rng;  // PCG64 state: state hi, state lo, increment hi, increment lo
+  int nblk_stage, nblk_proj;
+};
+
+// offsets kinds (rows of the offsets table, each ninst+1 ints)
+enum { OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_KINDS };
+
+__device__ __forceinline__ int find_inst(const int* __restrict__ o, int n, int b) {
+  int lo = 0, hi = n;  // o[0] = 0 <= b < o[n]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (o[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class T>
+__device__ __forceinline__ const T* stage_in(const Inst<T>& I, int st) {
+  return st == 0 ? I.Q : st == 1 ? I.Q1 : st == 2 ? I.Q2 : I.S;
+}
+template <class T>
+__device__ __forceinline__ T* stage_out(const Inst<T>& I, int st) {
+  return st == 0 ? I.Q1 : st == 1 ? I.Q2 : I.S;
+}
+
+#define HITX_BATCH_INST(KIND)                               \
+  const int* __restrict__ offs_ = offs + (KIND) * (ninst + 1); \
+  const int ii_ = find_inst(offs_, ninst, blockIdx.x);      \
+  const Inst<T>& I = tab[ii_];                              \
+  if (I.ctl[C_ACTIVE] == 0.0) return;                       \
+  const int lb = blockIdx.x - offs_[ii_];
+
+// ---------------------------------------------------------------- stage path
+// st = 0, 1, 2: RK stages on Q, Q1, Q2 (enforced input, transposed copy);
+// st = 3: the post-step sensor on S for hyperviscosity (as given, no copy).
+template <class T, bool DRAG>
+__global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                         int ninst, int st) {
+  HITX_BATCH_INST(OFF_RED)
+  if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb, I.nblk_stage);
+}
+
+template <class T, bool DRAG>
+__global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __restrict__ tab, int st) {
+  const Inst<T>& I = tab[blockIdx.x];
+  if (I.ctl[C_ACTIVE] == 0.0 || (st == 3 && I.ctl[C_HVNOW] == 0.0)) return;
+  stage_finalize_body<T, DRAG>(I.P, I.part, I.nblk_stage, I.scal);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_sensor(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                   int ninst, int st) {
+  HITX_BATCH_INST(OFF_SENS)
+  if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  const int ntx = (I.P.nx + kSTX - 1) / kSTX;
+  const int by = lb / ntx, bx = lb - by * ntx;
+  sensor_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.mask, I.maskT, st < 3 ? I.qT : nullptr, bx, by);
+}
+
+template <class T, bool PER>
+__global__ void __launch_bounds__(kLineBlock, 4) b_line(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                     int ninst, int axis, int st) {
+  HITX_BATCH_INST(axis == 0 ? OFF_LX : OFF_LY)
+  if (axis == 0)
+    line_body<T, PER>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, lb, I.P.ny);
+  else
+    line_body<T, PER>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, lb, I.P.nx);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                   int ninst, int st) {
+  HITX_BATCH_INST(OFF_T16)
+  const int ntx = (I.P.nx + 15) / 16;
+  const int by = lb / ntx, bx = lb - by * ntx;
+  update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, (T)I.ctl[C_DT],
+                 stage_out(I, st), bx, by);
+}
+
+// After the stages: hyperviscosity S -> Q with the fresh mask, or S -> Q copy.
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_finish(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                   int ninst) {
+  HITX_BATCH_INST(OFF_T16)
+  const int ntx = (I.P.nx + 15) / 16;
+  const int by = lb / ntx, bx = lb - by * ntx;
+  if (I.ctl[C_HVNOW] != 0.0) {
+    hv_body<T>(I.P, I.S, I.mask, 1, 1, I.Q, bx, by);
+    return;
+  }
+  const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
+  if (x >= I.P.nx || y >= I.P.ny) return;
+  const int N = I.P.nx * I.P.ny, o = y * I.P.nx + x;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) I.Q[c * N + o] = I.S[c * N + o];
+}
+
+// ---------------------------------------------------------------- step control
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_step_reduce(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                        int ninst) {
+  HITX_BATCH_INST(OFF_RED)
+  step_reduce_body<T>(I.P, I.Q, I.part, lb, I.nblk_stage);
+}
+
+__device__ __forceinline__ bool is_none(double v) { return v != v; }
+
+// One block per instance: finish the step reductions, then (thread 0) the
+// reference's scalar control: dt with the clip to the remaining time and the
+// viscous limit (OOP/hit2d.py:829-832; fixed dt for the Riemann loop), the
+// Mach-controlled target power (OOP/hit2d.py:293-349), the OU decay and
+// increment (OOP/forcing.py:75-76), and whether hyperviscosity runs this step.
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_ctl_begin(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  double* c = I.ctl;
+  if (c[C_ACTIVE] == 0.0) return;
+  __shared__ double r[3];
+  reduce_partials<3, 1>(I.part, I.nblk_stage, r);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* k = I.kpar;
+  c[C_MAXSR] = r[0];
+  c[C_SUV] = r[1];
+  c[C_SC] = r[2];
+  const double remaining = c[C_TFINAL] - c[C_TIME];
+  if (k[K_FIXED_DT] == 0.0) {
+    double dt = k[K_CFL] / r[0];
+    dt = fmin(dt, remaining);
+    if (!is_none(k[K_VCLIP])) dt = fmin(fmin(dt, k[K_VCLIP]), remaining);
+    c[C_DT] = dt;
+  }
+  if (k[K_FORCE] != 0.0) {
+    const double N = (double)I.P.nx * (double)I.P.ny;
+    const double mach = sqrt(r[1] / N) / (r[2] / N);
+    c[C_MACH] = mach;
+    const double power = c[C_POWER];
+    c[C_PDES] = power;
+    if (k[K_MACHON] != 0.0 && !is_none(power)) {
+      const double ratio = pow(k[K_MTGT] / fmax(mach, 1.0e-8), k[K_MEXP]);
+      double desired = power * ratio;
+      if (!is_none(k[K_MMIN])) desired = fmax(desired, k[K_MMIN]);
+      if (!is_none(k[K_MMAX])) desired = fmin(desired, k[K_MMAX]);
+      c[C_PDES] = desired;
+      c[C_POWER] = k[K_MMEM] * power + (1.0 - k[K_MMEM]) * desired;
+    }
+    const double decay = exp(-c[C_DT] / k[K_TCORR]);
+    c[C_DECAY] = decay;
+    c[C_INCR] = sqrt(fmax(1.0 - decay * decay, 0.0));
+  }
+  const int hv_int = (int)k[K_HVINT];
+  c[C_HVNOW] = (k[K_HVON] != 0.0 && hv_int > 0 && (((long long)c[C_STEP] + 1) % hv_int) == 0) ? 1.0 : 0.0;
+}
+
+// Fixed dt of the Riemann loop from the initial state: n = ceil(T/dt0),
+// dt = T/n (OOP/problems/riemann_config3.py:371-378).  One block per instance.
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_fixed_dt(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  __shared__ double r[3];
+  reduce_partials<3, 1>(I.part, I.nblk_stage, r);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double* c = I.ctl;
+  const double dt0 = I.kpar[K_CFL] / r[0];
+  const double nsteps = ceil(c[C_TFINAL] / dt0);
+  c[C_NSTEPS] = nsteps;
+  c[C_DT] = c[C_TFINAL] / nsteps;
+  c[C_ACTIVE] = nsteps > 0.0 ? 1.0 : 0.0;
+}
+
+// time, step and activity after the step (OOP/hit2d.py:870-871; the Riemann
+// loop's time = (step+1)*dt, OOP/problems/riemann_config3.py:741)
+template <class T>
+__global__ void b_ctl_end(const Inst<T>* __restrict__ tab, int ninst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  double* c = tab[i].ctl;
+  const double* k = tab[i].kpar;
+  if (c[C_ACTIVE] == 0.0) return;
+  c[C_STEP] = c[C_STEP] + 1.0;
+  c[C_HVAPPS] = c[C_HVAPPS] + c[C_HVNOW];
+  if (k[K_FIXED_DT] != 0.0) {
+    c[C_TIME] = c[C_STEP] * c[C_DT];
+    c[C_ACTIVE] = (c[C_STEP] < c[C_NSTEPS] && c[C_STEP] < k[K_MAXSTEPS]) ? 1.0 : 0.0;
+  } else {
+    c[C_TIME] = c[C_TIME] + c[C_DT];
+    c[C_ACTIVE] = (c[C_TIME] < c[C_TFINAL] && c[C_STEP] < k[K_MAXSTEPS]) ? 1.0 : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------- forcing
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_fproject(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                     int ninst) {
+  HITX_BATCH_INST(OFF_PROJ)
+  if (I.kpar[K_FORCE] == 0.0) return;
+  fproject_body<T>(I.P, I.noise, I.fpart, lb, I.nblk_proj);
+}
+
+// One block per instance: sum the projection partials, OU recurrence of the
+// shell coefficients (OOP/forcing.py:80) and the curl (:82-83).
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_fou(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
+  const int nm = I.P.nforce;
+  const double decay = I.ctl[C_DECAY], incr = I.ctl[C_INCR];
+  for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < I.nblk_proj; ++b) {
+      re += I.fpart[((size_t)b * nm + m) * 2];
+      im += I.fpart[((size_t)b * nm + m) * 2 + 1];
+    }
+    const double pr = decay * I.phi[2 * m] + incr * re;
+    const double pi = decay * I.phi[2 * m + 1] + incr * im;
+    I.phi[2 * m] = pr;
+    I.phi[2 * m + 1] = pi;
+    const double kx = I.kxy[2 * m], ky = I.kxy[2 * m + 1];
+    I.fhat[2 * m] = 0.0 * pr - ky * pi;  // (1j*ky) * phi
+    I.fhat[2 * m + 1] = 0.0 * pi + ky * pr;
+    I.fhat[2 * nm + 2 * m] = 0.0 * pr - (-kx) * pi;  // (-1j*kx) * phi
+    I.fhat[2 * nm + 2 * m + 1] = 0.0 * pi + (-kx) * pr;
+  }
+}
+
+// fx, fy from the shell coefficients, with the primitives of Q for the power.
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                   int ninst) {
+  HITX_BATCH_INST(OFF_RED)
+  if (I.kpar[K_FORCE] == 0.0) return;
+  const Params<T>& P = I.P;
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const double invN = 1.0 / ((double)nx * (double)ny);
+  double acc[2] = {0.0, 0.0};
+  for (int idx = lb * kBlock + threadIdx.x; idx < N; idx += I.nblk_stage * kBlock) {
+    const int y = idx / nx, x = idx - y * nx;
+    double ax = 0.0, ay = 0.0;
+    for (int m = 0; m < P.nforce; ++m) {
+      const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
+      const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
+      const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + y) * 2;
+      const double cr = tx[0] * ty[0] - tx[1] * ty[1];
+      const double si = tx[1] * ty[0] + tx[0] * ty[1];
+      ax += I.fhat[2 * m] * cr - I.fhat[2 * m + 1] * si;
+      ay += I.fhat[2 * P.nforce + 2 * m] * cr - I.fhat[2 * P.nforce + 2 * m + 1] * si;
+    }
+    const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
+    I.fx[idx] = fxv;
+    I.fy[idx] = fyv;
+    const Prim<T> pr = prim_at(P, I.Q, N, idx, 0);
+    acc[0] += (double)(fxv * fxv + fyv * fyv);
+    acc[1] += (double)(pr.rho * (fxv * pr.u + fyv * pr.v));
+  }
+  block_reduce_write<2, 0>(acc, I.part + (size_t)lb * 2);
+}
+
+// One block per instance: rms normalisation and the power rescale with the
+// alpha memory (OOP/forcing.py:87-112).
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_falpha(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
+  __shared__ double r[2];
+  reduce_partials<2, 0>(I.part, I.nblk_stage, r);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double* c = I.ctl;
+  const double* k = I.kpar;
+  const double N = (double)I.P.nx * (double)I.P.ny;
+  c[C_SFF] = r[0];
+  c[C_SP] = r[1];
+  const double vrms = sqrt(r[0] / N);
+  double scale;
+  if (k[K_FRMS] <= 0.0)
+    scale = 0.0;
+  else if (vrms > 2.220446049250313e-16)
+    scale = k[K_FRMS] / vrms;
+  else
+    scale = 1.0;
+  c[C_SCALE] = scale;
+  const double pb = scale * r[1] / N;
+  c[C_PBEFORE] = pb;
+  const double target = c[C_POWER];
+  double a = c[C_ALPHA];
+  double atgt = a;
+  if (!is_none(target) && fabs(pb) > k[K_MINP]) {
+    atgt = target / pb;
+    if (!is_none(k[K_MAXR])) atgt = fmin(fmax(atgt, -k[K_MAXR]), k[K_MAXR]);
+    a = k[K_AMEM] * a + (1.0 - k[K_AMEM]) * atgt;
+  } else if (is_none(target)) {
+    a = 1.0;
+  }
+  c[C_ALPHA] = a;
+  c[C_ATGT] = atgt;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_fscale(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                   int ninst) {
+  HITX_BATCH_INST(OFF_RED)
+  if (I.kpar[K_FORCE] == 0.0) return;
+  const Params<T>& P = I.P;
+  const int N = P.nx * P.ny;
+  const T scale = (T)I.ctl[C_SCALE], alpha = (T)I.ctl[C_ALPHA];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int idx = lb * kBlock + threadIdx.x; idx < N; idx += I.nblk_stage * kBlock) {
+    const T a = (I.fx[idx] * scale) * alpha, b = (I.fy[idx] * scale) * alpha;
+    I.fx[idx] = a;
+    I.fy[idx] = b;
+    const Prim<T> pr = prim_at(P, I.Q, N, idx, 0);
+    acc[0] += (double)(a * a);
+    acc[1] += (double)(b * b);
+    acc[2] += (double)(a * b);
+    acc[3] += (double)(pr.rho * (a * pr.u + b * pr.v));
+  }
+  block_reduce_write<4, 0>(acc, I.part + (size_t)lb * 4);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_fstats(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
+  __shared__ double r[4];
+  reduce_partials<4, 0>(I.part, I.nblk_stage, r);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double N = (double)I.P.nx * (double)I.P.ny;
+  I.ctl[C_FXX] = r[0] / N;
+  I.ctl[C_FYY] = r[1] / N;
+  I.ctl[C_FXY] = r[2] / N;
+  I.ctl[C_PIN] = r[3] / N;
+}
+
+}  // namespace hitx

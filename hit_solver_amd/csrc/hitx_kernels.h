@@ -24,23 +24,14 @@ __device__ __forceinline__ T absT(T a) { return fabs(a); }
 // enforce(q) when enforce_first (the operator rhs); cooling and drag use q as
 // given (the RK stage state), as the reference does.
 template <class T, bool DRAG>
-__device__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
-                                    double* __restrict__ scal);
-
-// The last block to finish (atomic ticket, no waiting) reduces all partials
-// in a fixed order and writes the stage scalars, then re-arms the ticket.
-// The ticket must be zero before the first launch on a workspace.
-template <class T, bool DRAG>
-__global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* __restrict__ q,
-                                                         int enforce_first, double* __restrict__ part,
-                                                         unsigned int* __restrict__ ticket,
-                                                         double* __restrict__ scal) {
+__device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* __restrict__ q,
+                                                         int enforce_first, double* __restrict__ part, int bid, int nblk) {
   constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
   double v[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) v[k] = 0.0;
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+  for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
     const int y = idx / nx, x = idx - y * nx;
     const Prim<T> c = prim_at(P, q, N, idx, enforce_first);
     const T cs = sqrt((P.gamma * c.p) / c.rho);
@@ -100,21 +91,20 @@ __global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* _
       }
     }
   }
-  block_reduce_write<NR, 0x1Full>(v, part + (size_t)blockIdx.x * NR);
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();  // acquire: every block's partials are visible
-  stage_finalize_body<T, DRAG>(P, part, gridDim.x, scal);
-  if (threadIdx.x == 0) *ticket = 0u;
+  block_reduce_write<NR, 0x1Full>(v, part + (size_t)bid * NR);
 }
 
 template <class T, bool DRAG>
-__device__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
-                                    double* __restrict__ scal) {
+__global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* __restrict__ q,
+                                                         int enforce_first, double* __restrict__ part) {
+  stage_reduce_body<T, DRAG>(P, q, enforce_first, part, blockIdx.x, gridDim.x);
+}
+
+// One block: reduce the partial rows in a fixed order and write the stage
+// scalars (a separate launch: kernel boundaries order the partial writes).
+template <class T, bool DRAG>
+__device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ scal) {
   constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
   __shared__ double r[NR];
   reduce_partials<NR, 0x1Full>(part, nblk, r);
@@ -141,6 +131,12 @@ __device__ void stage_finalize_body(const Params<T>& P, const double* __restrict
   }
   if (DRAG)
     for (int k = threadIdx.x; k < 4 * P.ndrag; k += blockDim.x) scal[S_DRAG + k] = r[R_DRAG + k] / N;
+}
+
+template <class T, bool DRAG>
+__global__ void __launch_bounds__(kBlock) k_stage_finalize(Params<T> P, const double* __restrict__ part, int nblk,
+                                                           double* __restrict__ scal) {
+  stage_finalize_body<T, DRAG>(P, part, nblk, scal);
 }
 
 // Generic column reduction: out[k] = sum over nblk rows of part[row*nr + k]
@@ -178,11 +174,11 @@ __global__ void __launch_bounds__(kBlock) k_finalize_max0(const double* __restri
 // OOP/problems/riemann_config3.py:371-378), sum(u^2+v^2) and sum(c) for the
 // turbulent Mach number (OOP/hit2d.py:279-290).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_step_reduce(Params<T> P, const T* __restrict__ q,
-                                                        double* __restrict__ part) {
+__device__ __forceinline__ void step_reduce_body(const Params<T>& P, const T* __restrict__ q,
+                                                        double* __restrict__ part, int bid, int nblk) {
   double v[3] = {0.0, 0.0, 0.0};
   const int N = P.nx * P.ny;
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+  for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
     const Prim<T> c = prim_at(P, q, N, idx, 0);
     const T cs = sqrt((P.gamma * c.p) / c.rho);
     T sr = (absT(c.u) + cs) / P.dx;
@@ -191,7 +187,13 @@ __global__ void __launch_bounds__(kBlock) k_step_reduce(Params<T> P, const T* __
     v[1] += (double)(c.u * c.u + c.v * c.v);
     v[2] += (double)cs;
   }
-  block_reduce_write<3, 1>(v, part + (size_t)blockIdx.x * 3);
+  block_reduce_write<3, 1>(v, part + (size_t)bid * 3);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_step_reduce(Params<T> P, const T* __restrict__ q,
+                                                        double* __restrict__ part) {
+  step_reduce_body<T>(P, q, part, blockIdx.x, gridDim.x);
 }
 
 // ================================================================ sensor
@@ -214,124 +216,6 @@ __device__ __forceinline__ T rel_jump(T fl, T f, T fr, T eps, bool periodic, boo
   return npmax(j, cv);
 }
 
-// Undilated sensor mask (OOP/spatial_operator.py:263-303,
-// OOP/problems/riemann_config3.py:288-325 including the boundary band).
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_rawmask(Params<T> P, const T* __restrict__ q, int enforce_first,
-                                                    const double* __restrict__ scal, uint8_t* __restrict__ raw) {
-  const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= nx || y >= ny) return;
-  const bool per = P.periodic;
-  const int xl = per ? wrap(x - 1, nx) : (x == 0 ? 0 : x - 1);
-  const int xr = per ? wrap(x + 1, nx) : (x == nx - 1 ? nx - 1 : x + 1);
-  const int yl = per ? wrap(y - 1, ny) : (y == 0 ? 0 : y - 1);
-  const int yr = per ? wrap(y + 1, ny) : (y == ny - 1 ? ny - 1 : y + 1);
-  const Prim<T> c = prim_at(P, q, N, y * nx + x, enforce_first);
-  const Prim<T> pxl = prim_at(P, q, N, y * nx + xl, enforce_first);
-  const Prim<T> pxr = prim_at(P, q, N, y * nx + xr, enforce_first);
-  const Prim<T> pyl = prim_at(P, q, N, yl * nx + x, enforce_first);
-  const Prim<T> pyr = prim_at(P, q, N, yr * nx + x, enforce_first);
-  bool m = false;
-  // compression (and shear) from central / one-sided velocity gradients
-  T ux, uy, vx, vy;
-  const bool xin = per || (x > 0 && x < nx - 1), yin = per || (y > 0 && y < ny - 1);
-  ux = xin ? (pxr.u - pxl.u) / P.two_dx : (pxr.u - pxl.u) / P.dx;
-  vx = xin ? (pxr.v - pxl.v) / P.two_dx : (pxr.v - pxl.v) / P.dx;
-  uy = yin ? (pyr.u - pyl.u) / P.two_dy : (pyr.u - pyl.u) / P.dy;
-  vy = yin ? (pyr.v - pyl.v) / P.two_dy : (pyr.v - pyl.v) / P.dy;
-  const T div = ux + vy;
-  if (scal[S_COMP_ON] != 0.0 && (double)div < scal[S_DIV_THR]) m = true;
-  if (scal[S_SHEAR_ON] != 0.0) {
-    const T w = vx - uy;
-    if ((double)absT(w) > scal[S_SHEAR_THR]) m = true;
-  }
-  const bool xlo = !per && x == 0, xhi = !per && x == nx - 1, ylo = !per && y == 0, yhi = !per && y == ny - 1;
-  const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
-  const T jr = npmax(rel_jump(pxl.rho, c.rho, pxr.rho, er, per, xlo, xhi),
-                     rel_jump(pyl.rho, c.rho, pyr.rho, er, per, ylo, yhi));
-  const T jp = npmax(rel_jump(pxl.p, c.p, pxr.p, ep, per, xlo, xhi),
-                     rel_jump(pyl.p, c.p, pyr.p, ep, per, ylo, yhi));
-  const T e_c = c.p / (c.rho * P.gm1), e_xl = pxl.p / (pxl.rho * P.gm1), e_xr = pxr.p / (pxr.rho * P.gm1);
-  const T e_yl = pyl.p / (pyl.rho * P.gm1), e_yr = pyr.p / (pyr.rho * P.gm1);
-  const T je = npmax(rel_jump(e_xl, e_c, e_xr, ee, per, xlo, xhi), rel_jump(e_yl, e_c, e_yr, ee, per, ylo, yhi));
-  const double thr = P.jump_thr;
-  if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
-  if (!per && P.bguard > 0) {
-    const int g = P.bguard;
-    if (y < g || y >= ny - g || x < g || x >= nx - g) m = true;
-  }
-  raw[y * nx + x] = m ? 1 : 0;
-}
-
-// (2w+1)^2 OR-dilation (periodic wrap, or truncated at the edges), written
-// both as mask [ny][nx] and transposed maskT [nx][ny].  Tile 32x32.
-__global__ void __launch_bounds__(kBlock) k_dilate(int nx, int ny, int w, int periodic,
-                                                   const uint8_t* __restrict__ raw, uint8_t* __restrict__ mask,
-                                                   uint8_t* __restrict__ maskT) {
-  constexpr int TS = 32, HMAX = 8, E = TS + 2 * HMAX;
-  __shared__ uint8_t in[E][E];
-  __shared__ uint8_t mid[E][TS];
-  __shared__ uint8_t outt[TS][TS + 1];
-  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS, ext = TS + 2 * w;
-  for (int t = threadIdx.x; t < ext * ext; t += kBlock) {
-    const int r = t / ext, c = t - r * ext;
-    int gy = y0 - w + r, gx = x0 - w + c;
-    uint8_t val = 0;
-    if (periodic) {
-      gy = ((gy % ny) + ny) % ny;
-      gx = ((gx % nx) + nx) % nx;
-      val = raw[gy * nx + gx];
-    } else if (gy >= 0 && gy < ny && gx >= 0 && gx < nx) {
-      val = raw[gy * nx + gx];
-    }
-    in[r][c] = val;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < ext * TS; t += kBlock) {
-    const int r = t / TS, c = t - r * TS;
-    uint8_t o = 0;
-    for (int d = 0; d <= 2 * w; ++d) o |= in[r][c + d];
-    mid[r][c] = o;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < TS * TS; t += kBlock) {
-    const int r = t / TS, c = t - r * TS;
-    uint8_t o = 0;
-    for (int d = 0; d <= 2 * w; ++d) o |= mid[r + d][c];
-    outt[r][c] = o;
-    const int gy = y0 + r, gx = x0 + c;
-    if (gy < ny && gx < nx) mask[gy * nx + gx] = o;
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < TS * TS; t += kBlock) {
-    const int c = t / TS, r = t - c * TS;  // consecutive threads -> consecutive y
-    const int gy = y0 + r, gx = x0 + c;
-    if (gy < ny && gx < nx) maskT[(size_t)gx * ny + gy] = outt[r][c];
-  }
-}
-
-// [4][ny][nx] -> [4][nx][ny], 32x32 tiles through LDS.
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_transpose4(int nx, int ny, const T* __restrict__ a, T* __restrict__ at) {
-  __shared__ T tile[32][33];
-  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-  const size_t N = (size_t)nx * ny;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int c = 0; c < 4; ++c) {
-    for (int r = ty; r < 32; r += 8) {
-      const int gy = y0 + r, gx = x0 + tx;
-      if (gy < ny && gx < nx) tile[r][tx] = a[c * N + (size_t)gy * nx + gx];
-    }
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-      const int gx = x0 + r, gy = y0 + tx;
-      if (gy < ny && gx < nx) at[c * N + (size_t)gx * ny + gy] = tile[tx][r];
-    }
-    __syncthreads();
-  }
-}
-
 // Fused sensor on a 32x16 output tile: primitives of (enforced) q on the tile
 // plus a (w+1) halo in LDS, undilated mask on the tile plus a w halo, the
 // separable (2w+1)^2 OR dilation, mask and maskT stores, and (when qT is not
@@ -341,13 +225,13 @@ __global__ void __launch_bounds__(kBlock) k_transpose4(int nx, int ny, const T* 
 // in the dilation, as in OOP/problems/riemann_config3.py:128-162.
 constexpr int kSTX = 32, kSTY = 16;
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restrict__ q, int enforce_first,
+__device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
                                                    const double* __restrict__ scal, uint8_t* __restrict__ mask,
-                                                   uint8_t* __restrict__ maskT, T* __restrict__ qT) {
+                                                   uint8_t* __restrict__ maskT, T* __restrict__ qT, int bx, int by) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nx = P.nx, ny = P.ny, N = nx * ny, w = P.width, H = w + 1;
   const int EX = kSTX + 2 * H, EY = kSTY + 2 * H, RX = kSTX + 2 * w, RY = kSTY + 2 * w;
-  const int x0 = blockIdx.x * kSTX, y0 = blockIdx.y * kSTY;
+  const int x0 = bx * kSTX, y0 = by * kSTY;
   const bool per = P.periodic;
   T* Lr = reinterpret_cast<T*>(smem);
   T* Lu = Lr + EX * EY;
@@ -460,131 +344,179 @@ __global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restr
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                   const double* __restrict__ scal, uint8_t* __restrict__ mask,
+                                                   uint8_t* __restrict__ maskT, T* __restrict__ qT) {
+  sensor_body<T>(P, q, enforce_first, scal, mask, maskT, qT, blockIdx.x, blockIdx.y);
+}
+
 // ================================================================ line sweep
 // Power of rho from the LDS table (zero beyond the table: |rho|^66 < 1e-22).
 template <class T>
 __device__ __forceinline__ T pwk(const T* PW, int k) { return k <= 65 ? PW[k] : T(0); }
 
-// Solve the four (alpha, 1, alpha) systems held in r[c*S + i], i < n, in place.
+// Divided difference of the interface flux F[c*S + .] and compact solve of
+// the four (alpha, 1, alpha) systems, stored to out[c*NL + g0 + i].
 // Periodic: cyclic matrix (OOP/spatial_operator.py:227-249 solves it by FFT).
 // Non-periodic: identity end rows (OOP/problems/riemann_config3.py:237-265,
 // Thomas).  Method: A = K (1 - rho S^-1)(1 - rho S), K = 1/(1+rho^2), so
 // A^-1 r is a causal then an anti-causal first-order recurrence with
 // coefficient rho = (-1 + sqrt(1 - 4 alpha^2)) / (2 alpha) = -0.4514.  Each
-// recurrence runs as a chunked scan: every thread sweeps one chunk of L
-// points with zero carry-in, then adds rho^k times the exact carry, which is a
-// short geometric sum over the preceding chunks' local results (|rho|^52 <
-// 1e-18 truncates it; the periodic wrap is summed in closed form).  The
-// non-periodic system is the zero-Dirichlet interior problem plus the
-// homogeneous solutions a rho^i + b rho^(n-1-i) that restore the end rows.
+// recurrence runs as a chunked scan: every thread holds one chunk of L <= 8
+// points of one line in registers, sweeps it with zero carry-in, then adds
+// rho^k times the exact carry, a short geometric sum over the preceding
+// chunks' local results read from LDS (|rho|^52 < 1e-18 truncates it; the
+// periodic wrap is summed in closed form).  The non-periodic system is the
+// zero-Dirichlet interior problem plus the homogeneous solutions
+// a rho^i + b rho^(n-1-i) that restore the end rows.
+// Requires 4*n/L <= 2*blockDim.x (at most two chunks per thread).
 template <class T, bool PER>
-__device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scratch) {
-  const int n = K.n, L = K.L, nch = K.nch, M = K.M;
-  T* e1 = scratch;             // [4][nch]
-  T* e2 = scratch + 4 * nch;   // [4][nch]
-  T* bnd = scratch + 8 * nch;  // [4][5]: r0, r_{n-1}, z_n, p0, p_{n-1}
+__device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h, const LineK<T>& K,
+                                            const T* __restrict__ PW, T* __restrict__ scratch, T* __restrict__ out,
+                                            size_t NL, size_t g0) {
+  const int n = K.n, L = K.L, nch = K.nch, M = K.M, ntask = 4 * nch, B = blockDim.x;
   const T rho = K.rho;
-  const int ntask = 4 * nch, B = blockDim.x;
-  if (!PER) {
-    if (threadIdx.x < 4) {
-      bnd[threadIdx.x * 5 + 0] = r[threadIdx.x * S];
-      bnd[threadIdx.x * 5 + 1] = r[threadIdx.x * S + n - 1];
-    }
-    __syncthreads();
-  }
+  T* e1 = scratch;             // [4*nch] local causal results
+  T* e2 = scratch + 4 * nch;   // [4*nch] local anti-causal results
+  T* bnd = scratch + 8 * nch;  // [4][3]: z_n, p_0, p_{n-1}
+  auto raw = [&](int c, int i) -> T {
+    const T* f = F + c * S;
+    return PER ? (f[i] - f[wrap(i - 1, n)]) / h : (f[i + 1] - f[i]) / h;
+  };
+  T y[2][8];
   // causal pass, local
-  for (int t = threadIdx.x; t < ntask; t += B) {
-    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
-    T* rl = r + line * S;
-    T acc = T(0);
-    for (int k = 0; k < L; ++k) {
-      const int i = s0 + k;
-      T xv = rl[i];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * B;
+    if (t < ntask) {
+      const int line = t / nch, s0 = (t - line * nch) * L;
+      T r0 = T(0), rn1 = T(0);
       if (!PER) {
-        if (i == 0 || i == n - 1) {
-          xv = T(0);
-        } else {
-          if (i == 1) xv = xv - K.alpha_c * bnd[line * 5 + 0];
-          if (i == n - 2) xv = xv - K.alpha_c * bnd[line * 5 + 1];
+        r0 = raw(line, 0);
+        rn1 = raw(line, n - 1);
+      }
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (k < L) {
+          const int i = s0 + k;
+          T xv = raw(line, i);
+          if (!PER) {
+            if (i == 0 || i == n - 1) {
+              xv = T(0);
+            } else {
+              if (i == 1) xv = xv - K.alpha_c * r0;
+              if (i == n - 2) xv = xv - K.alpha_c * rn1;
+            }
+          }
+          acc = xv + rho * acc;
+          y[u][k] = acc;
         }
       }
-      acc = xv + rho * acc;
-      rl[i] = acc;
-    }
-    e1[line * nch + ch] = acc;
-  }
-  __syncthreads();
-  // causal pass, carries
-  for (int t = threadIdx.x; t < ntask; t += B) {
-    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
-    T* rl = r + line * S;
-    const T* el = e1 + line * nch;
-    T carry = T(0);
-    if (PER) {
-      const int mm = nch < M ? nch : M;
-      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[(ch - m + nch) % nch];
-      carry = carry * K.inv1mrn;
-    } else {
-      const int mm = ch < M ? ch : M;
-      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch - m];
-    }
-    for (int k = 0; k < L; ++k) rl[s0 + k] = rl[s0 + k] + pwk(PW, k + 1) * carry;
-    if (!PER && ch == nch - 1) bnd[line * 5 + 2] = (rho * rl[n - 1]) / K.one_m_rho2;
-  }
-  __syncthreads();
-  // anti-causal pass, local
-  for (int t = threadIdx.x; t < ntask; t += B) {
-    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
-    T* rl = r + line * S;
-    T acc = T(0);
-    for (int k = L - 1; k >= 0; --k) {
-      acc = rl[s0 + k] + rho * acc;
-      rl[s0 + k] = acc;
-    }
-    e2[line * nch + ch] = acc;
-  }
-  __syncthreads();
-  // anti-causal pass, carries, scale by 1 + rho^2
-  for (int t = threadIdx.x; t < ntask; t += B) {
-    const int line = t / nch, ch = t - line * nch, s0 = ch * L;
-    T* rl = r + line * S;
-    const T* el = e2 + line * nch;
-    T carry = T(0);
-    if (PER) {
-      const int mm = nch < M ? nch : M;
-      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[(ch + m) % nch];
-      carry = carry * K.inv1mrn;
-    } else {
-      const int rem = nch - 1 - ch;
-      const int mm = rem < M ? rem : M;
-      for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch + m];
-      if (rem < M) carry = carry + pwk(PW, rem * L) * bnd[line * 5 + 2];
-    }
-    for (int k = 0; k < L; ++k) rl[s0 + k] = K.c1p * (rl[s0 + k] + pwk(PW, L - k) * carry);
-    if (!PER) {
-      if (ch == 0) bnd[line * 5 + 3] = rl[0];
-      if (ch == nch - 1) bnd[line * 5 + 4] = rl[n - 1];
+      e1[t] = acc;
     }
   }
   __syncthreads();
-  if (!PER) {
-    for (int t = threadIdx.x; t < ntask; t += B) {
-      const int line = t / nch, ch = t - line * nch, s0 = ch * L;
-      T* rl = r + line * S;
-      const T p0 = bnd[line * 5 + 3], pn = bnd[line * 5 + 4];
-      const T det = T(1) - K.rnm1 * K.rnm1;
-      const T a = (-p0 + pn * K.rnm1) / det, b = (-pn + p0 * K.rnm1) / det;
-      for (int k = 0; k < L; ++k) {
-        const int i = s0 + k;
-        if (i == 0)
-          rl[0] = bnd[line * 5 + 0];
-        else if (i == n - 1)
-          rl[n - 1] = bnd[line * 5 + 1];
-        else
-          rl[i] = (rl[i] + a * pwk(PW, i)) + b * pwk(PW, n - 1 - i);
+  // causal carries, then the anti-causal local pass
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * B;
+    if (t < ntask) {
+      const int line = t / nch, ch = t - line * nch;
+      const T* el = e1 + line * nch;
+      T carry = T(0);
+      if (PER) {
+        const int mm = nch < M ? nch : M;
+        int j = ch;
+        for (int m = 1; m <= mm; ++m) {
+          j = (j == 0) ? nch - 1 : j - 1;
+          carry = carry + pwk(PW, (m - 1) * L) * el[j];
+        }
+        carry = carry * K.inv1mrn;
+      } else {
+        const int mm = ch < M ? ch : M;
+        for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch - m];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < L) y[u][k] = y[u][k] + pwk(PW, k + 1) * carry;
+      if (!PER && ch == nch - 1) bnd[line * 3 + 0] = (rho * y[u][L - 1]) / K.one_m_rho2;
+      T acc = T(0);
+#pragma unroll
+      for (int k = 7; k >= 0; --k) {
+        if (k < L) {
+          acc = y[u][k] + rho * acc;
+          y[u][k] = acc;
+        }
+      }
+      e2[t] = acc;
+    }
+  }
+  __syncthreads();
+  // anti-causal carries, scale by 1 + rho^2
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * B;
+    if (t < ntask) {
+      const int line = t / nch, ch = t - line * nch;
+      const T* el = e2 + line * nch;
+      T carry = T(0);
+      if (PER) {
+        const int mm = nch < M ? nch : M;
+        int j = ch;
+        for (int m = 1; m <= mm; ++m) {
+          j = (j == nch - 1) ? 0 : j + 1;
+          carry = carry + pwk(PW, (m - 1) * L) * el[j];
+        }
+        carry = carry * K.inv1mrn;
+      } else {
+        const int rem = nch - 1 - ch;
+        const int mm = rem < M ? rem : M;
+        for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch + m];
+        if (rem < M) carry = carry + pwk(PW, rem * L) * bnd[line * 3 + 0];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < L) y[u][k] = K.c1p * (y[u][k] + pwk(PW, L - k) * carry);
+      if (!PER) {
+        if (ch == 0) bnd[line * 3 + 1] = y[u][0];
+        if (ch == nch - 1) bnd[line * 3 + 2] = y[u][L - 1];
       }
     }
-    __syncthreads();
+  }
+  if (!PER) __syncthreads();
+  // (non-periodic: end-row corrections) and store
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int t = threadIdx.x + u * B;
+    if (t < ntask) {
+      const int line = t / nch, s0 = (t - line * nch) * L;
+      T* o = out + line * NL + g0;
+      if (PER) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < L) o[s0 + k] = y[u][k];
+      } else {
+        const T p0 = bnd[line * 3 + 1], pn = bnd[line * 3 + 2];
+        const T det = T(1) - K.rnm1 * K.rnm1;
+        const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (k < L) {
+            const int i = s0 + k;
+            T v;
+            if (i == 0)
+              v = raw(line, 0);
+            else if (i == n - 1)
+              v = raw(line, n - 1);
+            else
+              v = (y[u][k] + a * pwk(PW, i)) + bb * pwk(PW, n - 1 - i);
+            o[i] = v;
+          }
+        }
+      }
+    }
   }
 }
 
@@ -597,12 +529,12 @@ __device__ void solve_lines(T* r, int S, const LineK<T>& K, const T* PW, T* scra
 // OOP/problems/riemann_config3.py:165-265, :344-357.
 constexpr int kLineBlock = 512;
 template <class T, bool PER>
-__global__ void __launch_bounds__(kLineBlock) k_line(Params<T> P, LineK<T> K, int axis, int enforced_input,
+__device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K, int axis, int enforced_input,
                                                      const T* __restrict__ ql, const uint8_t* __restrict__ ml,
-                                                     const double* __restrict__ scal, T* __restrict__ out) {
+                                                     const double* __restrict__ scal, T* __restrict__ out, int line, int nl) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int BL = kLineBlock;
-  const int n = K.n, S = n + 1, line = blockIdx.x;
+  const int n = K.n, S = n + 1;
   T* A = reinterpret_cast<T*>(smem);
   T* B = A + 4 * S;
   T* C = B + 4 * S;
@@ -611,7 +543,7 @@ __global__ void __launch_bounds__(kLineBlock) k_line(Params<T> P, LineK<T> K, in
   int* wl = reinterpret_cast<int*>(scratch + 8 * K.nch + 20);  // WENO work list, n+1
   uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + S);
   __shared__ int wcnt;
-  const size_t NL = (size_t)gridDim.x * n;
+  const size_t NL = (size_t)nl * n;
   const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
   const T h = axis == 0 ? P.dx : P.dy;
   for (int k = threadIdx.x; k < 66; k += BL) PW[k] = K.pw[k];
@@ -677,8 +609,8 @@ __global__ void __launch_bounds__(kLineBlock) k_line(Params<T> P, LineK<T> K, in
   // 3. hybrid interface flux into A
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
-  for (int t = threadIdx.x; t < 4 * nI; t += BL) {
-    const int c = t / nI, k = t - c * nI;
+  for (int c = 0; c < 4; ++c)
+  for (int k = threadIdx.x; k < nI; k += BL) {
     const bool fl = P.weno_only || iflag(k);
     const int il = PER ? k : k - 1;
     T F;
@@ -695,19 +627,15 @@ __global__ void __launch_bounds__(kLineBlock) k_line(Params<T> P, LineK<T> K, in
     A[c * S + k] = F;
   }
   __syncthreads();
-  // 4. divided difference into B
-  for (int t = threadIdx.x; t < 4 * n; t += BL) {
-    const int c = t / n, i = t - c * n;
-    const T* F = A + c * S;
-    B[c * S + i] = PER ? (F[i] - F[wrap(i - 1, n)]) / h : (F[i + 1] - F[i]) / h;
-  }
-  __syncthreads();
-  // 5. compact solve, 6. store
-  solve_lines<T, PER>(B, S, K, PW, scratch);
-  for (int i = threadIdx.x; i < n; i += BL) {
-    const size_t g = (size_t)line * n + i;
-    for (int c = 0; c < 4; ++c) out[c * NL + g] = B[c * S + i];
-  }
+  // 4. divided difference, compact solve, store
+  solve_store<T, PER>(A, S, h, K, PW, scratch, out, NL, (size_t)line * n);
+}
+
+template <class T, bool PER>
+__global__ void __launch_bounds__(kLineBlock, 4) k_line(Params<T> P, LineK<T> K, int axis, int enforced_input,
+                                                     const T* __restrict__ ql, const uint8_t* __restrict__ ml,
+                                                     const double* __restrict__ scal, T* __restrict__ out) {
+  line_body<T, PER>(P, K, axis, enforced_input, ql, ml, scal, out, blockIdx.x, gridDim.x);
 }
 
 // ================================================================ stage update
@@ -757,13 +685,13 @@ enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
 // enforce (+ outflow guard: every node takes the value of its clamped
 // interior source, OOP/problems/riemann_config3.py:419-429).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int stage, const T* __restrict__ q0,
+__device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
                                                    const T* __restrict__ qs, const T* __restrict__ dfx,
                                                    const T* __restrict__ dfyT, const T* __restrict__ fx,
                                                    const T* __restrict__ fy, const double* __restrict__ scal, T dt,
-                                                   T* __restrict__ out) {
+                                                   T* __restrict__ out, int bx, int by) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
   if (x >= nx || y >= ny) return;
   int cx = x, cy = y;
   if (P.guard > 0) {
@@ -839,6 +767,15 @@ __global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int st
   for (int c = 0; c < 4; ++c) out[c * N + oidx] = o[c];
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int stage, const T* __restrict__ q0,
+                                                   const T* __restrict__ qs, const T* __restrict__ dfx,
+                                                   const T* __restrict__ dfyT, const T* __restrict__ fx,
+                                                   const T* __restrict__ fy, const double* __restrict__ scal, T dt,
+                                                   T* __restrict__ out) {
+  update_body<T>(P, mode, stage, q0, qs, dfx, dfyT, fx, fy, scal, dt, out, blockIdx.x, blockIdx.y);
+}
+
 // ================================================================ hyperviscosity
 // q - mn*w*h^4*active*lap(lap(q)), then enforce (+ guard).  Periodic
 // Laplacian with wrap (OOP/spatial_operator.py:394-435) or edge-padded
@@ -860,10 +797,10 @@ __device__ __forceinline__ T lap_at(const Params<T>& P, const T* __restrict__ a,
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict__ q, const uint8_t* __restrict__ active,
-                                               int guard, int invert, T* __restrict__ out) {
+__device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict__ q, const uint8_t* __restrict__ active,
+                                               int guard, int invert, T* __restrict__ out, int bx, int by) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
   if (x >= nx || y >= ny) return;
   int cx = x, cy = y;
   if (guard && P.guard > 0) {
@@ -894,12 +831,18 @@ __global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict_
   for (int c = 0; c < 4; ++c) out[(size_t)c * N + y * nx + x] = o[c];
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict__ q, const uint8_t* __restrict__ active,
+                                               int guard, int invert, T* __restrict__ out) {
+  hv_body<T>(P, q, active, guard, invert, out, blockIdx.x, blockIdx.y);
+}
+
 // clean(): enforce_physical_state, then the outflow guard if requested.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_enforce(Params<T> P, const T* __restrict__ q, int guard,
-                                                    T* __restrict__ out) {
+__device__ __forceinline__ void enforce_body(const Params<T>& P, const T* __restrict__ q, int guard,
+                                                    T* __restrict__ out, int bx, int by) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
   if (x >= nx || y >= ny) return;
   int cx = x, cy = y;
   if (guard && P.guard > 0) {
@@ -916,20 +859,26 @@ __global__ void __launch_bounds__(kBlock) k_enforce(Params<T> P, const T* __rest
   out[3 * N + o] = d;
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_enforce(Params<T> P, const T* __restrict__ q, int guard,
+                                                    T* __restrict__ out) {
+  enforce_body<T>(P, q, guard, out, blockIdx.x, blockIdx.y);
+}
+
 // ================================================================ forcing
 // Forward DFT of the noise at each shell mode (OOP/forcing.py:78-79 keeps only
 // these coefficients of fft2).  PPT points per thread held in registers; one
 // block reduction per mode.
 constexpr int kPPT = 8;
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_fproject(Params<T> P, const double* __restrict__ noise,
-                                                     double* __restrict__ part) {
+__device__ __forceinline__ void fproject_body(const Params<T>& P, const double* __restrict__ noise,
+                                                     double* __restrict__ part, int bid, int nblk) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   double val[kPPT];
   int xs[kPPT], ys[kPPT];
 #pragma unroll
   for (int j = 0; j < kPPT; ++j) {
-    const int idx = blockIdx.x * kBlock * kPPT + j * kBlock + threadIdx.x;
+    const int idx = bid * kBlock * kPPT + j * kBlock + threadIdx.x;
     if (idx < N) {
       val[j] = noise[idx];
       ys[j] = idx / nx;
@@ -950,22 +899,28 @@ __global__ void __launch_bounds__(kBlock) k_fproject(Params<T> P, const double* 
       v[0] += val[j] * cr;
       v[1] -= val[j] * si;
     }
-    block_reduce_write<2, 0>(v, part + ((size_t)blockIdx.x * P.nforce + m) * 2);
+    block_reduce_write<2, 0>(v, part + ((size_t)bid * P.nforce + m) * 2);
     __syncthreads();
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fproject(Params<T> P, const double* __restrict__ noise,
+                                                     double* __restrict__ part) {
+  fproject_body<T>(P, noise, part, blockIdx.x, gridDim.x);
 }
 
 // fx, fy = real inverse DFT of (fx_hat, fy_hat) on the shell modes
 // (OOP/forcing.py:82-85), plus sum(fx^2+fy^2) and sum(rho*(fx*u+fy*v)).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_fsynth(Params<T> P, const double* __restrict__ fhat, T* __restrict__ fxo,
+__device__ __forceinline__ void fsynth_body(const Params<T>& P, const double* __restrict__ fhat, T* __restrict__ fxo,
                                                    T* __restrict__ fyo, const T* __restrict__ rho,
                                                    const T* __restrict__ u, const T* __restrict__ v,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part, int bid, int nblk) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const double invN = 1.0 / ((double)nx * (double)ny);
   double acc[2] = {0.0, 0.0};
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+  for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
     const int y = idx / nx, x = idx - y * nx;
     double ax = 0.0, ay = 0.0;
     for (int m = 0; m < P.nforce; ++m) {
@@ -985,18 +940,26 @@ __global__ void __launch_bounds__(kBlock) k_fsynth(Params<T> P, const double* __
     acc[0] += (double)(fxv * fxv + fyv * fyv);
     acc[1] += (double)(rho[idx] * (fxv * u[idx] + fyv * v[idx]));
   }
-  block_reduce_write<2, 0>(acc, part + (size_t)blockIdx.x * 2);
+  block_reduce_write<2, 0>(acc, part + (size_t)bid * 2);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fsynth(Params<T> P, const double* __restrict__ fhat, T* __restrict__ fxo,
+                                                   T* __restrict__ fyo, const T* __restrict__ rho,
+                                                   const T* __restrict__ u, const T* __restrict__ v,
+                                                   double* __restrict__ part) {
+  fsynth_body<T>(P, fhat, fxo, fyo, rho, u, v, part, blockIdx.x, gridDim.x);
 }
 
 // fx = (fx*scale)*alpha, fy likewise (OOP/forcing.py:91-112), and the
 // statistics of OOP/forcing.py:113-118.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_fscale(Params<T> P, T* __restrict__ fx, T* __restrict__ fy, T scale,
+__device__ __forceinline__ void fscale_body(const Params<T>& P, T* __restrict__ fx, T* __restrict__ fy, T scale,
                                                    T alpha, const T* __restrict__ rho, const T* __restrict__ u,
-                                                   const T* __restrict__ v, double* __restrict__ part) {
+                                                   const T* __restrict__ v, double* __restrict__ part, int bid, int nblk) {
   const int N = P.nx * P.ny;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < N; idx += gridDim.x * kBlock) {
+  for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
     const T a = (fx[idx] * scale) * alpha, b = (fy[idx] * scale) * alpha;
     fx[idx] = a;
     fy[idx] = b;
@@ -1005,7 +968,14 @@ __global__ void __launch_bounds__(kBlock) k_fscale(Params<T> P, T* __restrict__ 
     acc[2] += (double)(a * b);
     acc[3] += (double)(rho[idx] * (a * u[idx] + b * v[idx]));
   }
-  block_reduce_write<4, 0>(acc, part + (size_t)blockIdx.x * 4);
+  block_reduce_write<4, 0>(acc, part + (size_t)bid * 4);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_fscale(Params<T> P, T* __restrict__ fx, T* __restrict__ fy, T scale,
+                                                   T alpha, const T* __restrict__ rho, const T* __restrict__ u,
+                                                   const T* __restrict__ v, double* __restrict__ part) {
+  fscale_body<T>(P, fx, fy, scale, alpha, rho, u, v, part, blockIdx.x, gridDim.x);
 }
 
 // OOP/equations.py:136-145 on the whole grid.

@@ -75,41 +75,46 @@ class PlanSpec:
         return True if self.periodic else self.shear_threshold > 0.0
 
 
+def make_desc(spec: PlanSpec):
+    """hitx_desc for a spec; returns (desc, arrays that must outlive it)."""
+    s = spec
+    d = HitxDesc()
+    d.abi_version = _lib.HITX_ABI_VERSION
+    d.nx, d.ny = s.nx, s.ny
+    d.periodic = int(bool(s.periodic))
+    d.dtype = _lib.HITX_F64 if s.dtype == torch.float64 else _lib.HITX_F32
+    d.scheme = _lib.HITX_WENO if s.scheme == "weno" else _lib.HITX_HYBRID
+    d.dx, d.dy = s.dx, s.dy
+    d.dx2, d.dy2 = s.dx**2, s.dy**2
+    d.gamma, d.rho_floor, d.p_floor = s.gamma, s.rho_floor, s.p_floor
+    d.sensor_width = s.sensor_width
+    d.boundary_guard = s.boundary_guard
+    d.shear_enabled = int(s.shear_enabled())
+    d.compression_threshold = s.compression_threshold
+    d.jump_threshold = s.jump_threshold
+    d.shear_threshold = 0.0 if s.shear_threshold is None else float(s.shear_threshold)
+    d.viscosity, d.prandtl, d.gas_constant = s.viscosity, s.prandtl, s.gas_constant
+    for k in range(4):
+        d.hv_coef[k] = s.hv_coef[k]
+    d.guard_cells = s.guard_cells
+    d.forcing_enabled = int(s.forcing)
+    d.drag_enabled = int(s.drag_coef > 0.0 and len(s.drag_modes) > 0)
+    d.drag_coef = s.drag_coef
+    d.cooling_enabled = int(s.cooling_time is not None and s.cooling_time > 0.0)
+    d.cooling_time = 0.0 if s.cooling_time is None else s.cooling_time
+    d.cooling_p_ref = (1.0 / s.gamma) if s.cooling_p_ref is None else s.cooling_p_ref
+    dm = np.ascontiguousarray(np.array(s.drag_modes, dtype=np.int32).reshape(-1, 2))
+    fm = np.ascontiguousarray(np.array(s.force_modes, dtype=np.int32).reshape(-1, 2))
+    d.n_drag_modes, d.n_force_modes = len(dm), len(fm)
+    d.drag_modes_host = dm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    d.force_modes_host = fm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    return d, (dm, fm)
+
+
 class Plan:
     def __init__(self, spec: PlanSpec):
         self.spec = spec
-        s = spec
-        d = HitxDesc()
-        d.abi_version = _lib.HITX_ABI_VERSION
-        d.nx, d.ny = s.nx, s.ny
-        d.periodic = int(bool(s.periodic))
-        d.dtype = _lib.HITX_F64 if s.dtype == torch.float64 else _lib.HITX_F32
-        d.scheme = _lib.HITX_WENO if s.scheme == "weno" else _lib.HITX_HYBRID
-        d.dx, d.dy = s.dx, s.dy
-        d.dx2, d.dy2 = s.dx**2, s.dy**2
-        d.gamma, d.rho_floor, d.p_floor = s.gamma, s.rho_floor, s.p_floor
-        d.sensor_width = s.sensor_width
-        d.boundary_guard = s.boundary_guard
-        d.shear_enabled = int(s.shear_enabled())
-        d.compression_threshold = s.compression_threshold
-        d.jump_threshold = s.jump_threshold
-        d.shear_threshold = 0.0 if s.shear_threshold is None else float(s.shear_threshold)
-        d.viscosity, d.prandtl, d.gas_constant = s.viscosity, s.prandtl, s.gas_constant
-        for k in range(4):
-            d.hv_coef[k] = s.hv_coef[k]
-        d.guard_cells = s.guard_cells
-        d.forcing_enabled = int(s.forcing)
-        d.drag_enabled = int(s.drag_coef > 0.0 and len(s.drag_modes) > 0)
-        d.drag_coef = s.drag_coef
-        d.cooling_enabled = int(s.cooling_time is not None and s.cooling_time > 0.0)
-        d.cooling_time = 0.0 if s.cooling_time is None else s.cooling_time
-        d.cooling_p_ref = (1.0 / s.gamma) if s.cooling_p_ref is None else s.cooling_p_ref
-        dm = np.ascontiguousarray(np.array(s.drag_modes, dtype=np.int32).reshape(-1, 2))
-        fm = np.ascontiguousarray(np.array(s.force_modes, dtype=np.int32).reshape(-1, 2))
-        d.n_drag_modes, d.n_force_modes = len(dm), len(fm)
-        d.drag_modes_host = dm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        d.force_modes_host = fm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-        self._keep = (dm, fm)
+        d, self._keep = make_desc(spec)
         h = ctypes.c_void_p()
         check(lib().hitx_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h

@@ -96,9 +96,7 @@ typedef struct hitx_plan hitx_plan;
 int hitx_plan_create(const hitx_desc* desc, hitx_plan** out);
 int hitx_plan_destroy(hitx_plan* plan);
 /* Bytes of device workspace that every compute entry point below needs.
- * The workspace must be zero-filled before its first use (it holds a
- * reduction ticket that every call leaves at zero again) and must not be
- * shared by calls that can run concurrently. */
+ * A workspace must not be shared by calls that can run concurrently. */
 size_t hitx_workspace_bytes(const hitx_plan* plan);
 const char* hitx_last_error(void);
 int hitx_abi_version(void);
@@ -171,6 +169,44 @@ int hitx_viscous_rhs(hitx_plan* plan, const void* q, void* out, void* stream);
  * kernel time and the number of launches since the last read, and resets. */
 int hitx_timing_enable(hitx_plan* plan, int32_t on);
 int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
+
+/* ---------------------------------------------------------------- batches
+ * Many instances (grids of different sizes, same periodic/dtype/scheme/
+ * sensor width/drag setting) advance together: one launch per kernel kind
+ * per stage covers all of them, and per-instance scalar control (dt, Mach
+ * control, the forcing's OU recurrence and power rescale, step counting)
+ * runs on the device, so hitx_batch_step needs no host synchronisation and
+ * can be captured in a HIP graph.  No reference counterpart: the reference
+ * runs one instance per process (OOP/hit2d.py:764, OOP/problems/riemann_config3.py:681).
+ *
+ * kpar: [n][HITX_KPAR] run parameters per instance, doubles, NaN = None:
+ *   0 cfl, 1 viscous dt clip 0.25 h^2/nu (NaN: none), 2 fixed dt (1: Riemann loop),
+ *   3 hyperviscosity interval, 4 hyperviscosity on, 5 Mach control on, 6 Mach target,
+ *   7 Mach memory, 8 Mach exponent, 9 min power, 10 max power, 11 target power,
+ *   12 min forcing power, 13 max rescale, 14 alpha memory, 15 force rms,
+ *   16 correlation time, 17 forcing on, 18 tfinal, 19 max steps.
+ * Control scalars (HITX_BUF_CTL, doubles): 0 active, 1 time, 2 tfinal, 3 dt, 4 step,
+ *   7 hyperviscosity applications, 11 target power, 12 alpha, 18-21 Fxx Fyy Fxy P_in,
+ *   22 power before rescale, 23 alpha target, 24 Mach, 25 desired power, 26 n steps (fixed dt).
+ * The caller writes each instance's initial state (HITX_BUF_STATE), control
+ * scalars and, for forced HIT, the noise of each step (HITX_BUF_NOISE, the
+ * next nx*ny standard normals of the instance's stream) before hitx_batch_step. */
+#define HITX_KPAR 24
+#define HITX_CTL 32
+enum { HITX_BUF_STATE = 0, HITX_BUF_NOISE, HITX_BUF_CTL, HITX_BUF_PHI, HITX_BUF_RNG, HITX_BUF_FX, HITX_BUF_FY };
+
+typedef struct hitx_batch hitx_batch;
+int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hitx_batch** out);
+int hitx_batch_destroy(hitx_batch* batch);
+size_t hitx_batch_workspace_bytes(const hitx_batch* batch);
+/* Carve the instances' buffers from ws and upload the instance table (blocking). */
+int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
+/* Device pointer of one of instance i's buffers inside the bound workspace. */
+int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
+/* Fixed dt of every instance from its current state (Riemann loop). */
+int hitx_batch_fixed_dt(hitx_batch* batch, void* stream);
+/* One full step of every active instance. */
+int hitx_batch_step(hitx_batch* batch, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,85 @@
+"""Batched runs (hitx_batch_*) against single-instance runs and the oracle.
+
+Riemann instances in a batch must reproduce the single-instance solver bit
+for bit (same kernels, same fixed dt, no host-side scalars).  Forced HIT
+instances differ from the single-instance driver only through summation
+order of the forcing statistics and the device exp/pow of the control
+scalars (relative 1e-16 per step), so they are held to 1e-12 after a few
+steps; against the oracle the sensitivity-based bound applies.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def test_riemann_batch_matches_single_instances():
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannConfig3Offset08, RiemannConfig6, RiemannSolver
+
+    cfgs = [RiemannConfig3(nx=64, ny=48, shear_threshold=None, progress_every=10**9),
+            RiemannConfig6(nx=48, ny=64, shear_threshold=None, progress_every=10**9),
+            RiemannConfig3Offset08(nx=56, ny=56, shear_threshold=None, progress_every=10**9, backend="hip")]
+    b = BatchRunner(cfgs, max_steps=7)
+    b.run()
+    ctl = b.control()
+    for i, c in enumerate(cfgs):
+        s = RiemannSolver(c)
+        s.run(max_steps=7)
+        assert ctl[i, 4] == 7 and ctl[i, 3] == s.dt and ctl[i, 7] == s.hv_applications
+        np.testing.assert_array_equal(host(b.q[i]), host(s.q))
+
+
+def test_hit_batch_matches_single_instances():
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig, HITSolver
+
+    base = dict(cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0, viscosity=7.5e-4,
+                forcing_correlation_time=0.5, mach_control=True, large_scale_drag=0.10, cooling_time=5.0,
+                hyperviscosity_interval=2, tfinal=1.0, max_steps=5)
+    cfgs = [HIT2DConfig(nx=32, ny=32, target_mach=0.1, forcing_seed=0, **base),
+            HIT2DConfig(nx=48, ny=40, target_mach=0.25, forcing_seed=1, **base),
+            HIT2DConfig(nx=64, ny=64, target_mach=0.4, forcing_seed=2, **base)]
+    b = BatchRunner(cfgs)
+    b.run()
+    ctl = b.control()
+    for i, c in enumerate(cfgs):
+        s = HITSolver(c)
+        s.run()
+        assert ctl[i, 4] == s.steps
+        assert ctl[i, 1] == pytest.approx(s.time, rel=1e-14)
+        assert ctl[i, 12] == pytest.approx(s.info["alpha"], rel=1e-11)
+        for k in range(4):
+            assert rel_l2(host(b.q[i])[k], host(s.q)[k]) <= 1e-12, (i, k)
+
+
+def test_hit_batch_vs_oracle():
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+    from oracle import hit as OH
+
+    kw = dict(nx=48, ny=32, cfl=0.05, target_mach=0.25, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0,
+              forcing_kmax=5.0, viscosity=7.5e-4, forcing_correlation_time=0.5, mach_control=True,
+              large_scale_drag=0.10, cooling_time=5.0, tfinal=1.0, max_steps=6, forcing_seed=7)
+    b = BatchRunner([HIT2DConfig(**kw)])
+    b.run()
+    ref = OH.HITRun(OH.HITParams(**kw))
+    ref.run()
+    for k in range(4):
+        assert rel_l2(host(b.q[0])[k], ref.q[k]) <= 1e-10, k
