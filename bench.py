@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--dtype", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the host oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run 3 plain device copies of this many MiB first (scale for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -78,6 +80,13 @@ def main():
     from hit_solver_amd.riemann import RiemannConfig3, RiemannSolver
 
     torch.cuda.set_device(0)
+    if args.copy_calibration_mb > 0:
+        src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nx = args.nx
     cfg = RiemannConfig3(nx=nx, ny=nx, cfl=0.4, tfinal=0.3, sensor_width=4, jump_threshold=0.025,
                          shear_threshold=None, mn=0.001, hyperviscosity_interval=1, progress_every=10**9,

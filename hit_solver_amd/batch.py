@@ -5,8 +5,8 @@ all instances, per-instance scalar control on the device).
 Each instance follows the reference's step loop for its own configuration
 (OOP/hit2d.py:764-918 or OOP/problems/riemann_config3.py:681-749).  The
 forcing noise of each HIT instance is its own NumPy PCG64 stream
-(default_rng(forcing_seed + 1)), drawn on the host and copied into the
-instance's noise buffer before each step.
+(default_rng(forcing_seed + 1)), generated on the device inside the step, so a
+step needs no host work and no synchronisation.
 """
 
 from __future__ import annotations
@@ -22,6 +22,7 @@ from ._lib import check, lib
 from .forcing import shell_modes
 from .hit2d import HIT2DConfig, _make_equation, create_grid, drag_modes, initialize_hit_2d
 from .riemann import RiemannConfig3, hyperviscosity_enabled_for_config
+from .rng import pcg64_words
 from .runtime import PlanSpec, get_plan, make_desc
 from .spatial_operator import PeriodicHyperviscosity2D
 
@@ -30,7 +31,7 @@ NAN = float("nan")
 C_ACTIVE, C_TIME, C_TFINAL, C_DT, C_STEP, C_MAXSTEPS = 0, 1, 2, 3, 4, 5
 C_HVAPPS, C_POWER, C_ALPHA, C_NSTEPS = 7, 11, 12, 26
 C_FXX, C_FYY, C_FXY, C_PIN, C_PBEFORE, C_ATGT, C_MACH, C_PDES = 18, 19, 20, 21, 22, 23, 24, 25
-BUF_STATE, BUF_NOISE, BUF_CTL = 0, 1, 2
+BUF_STATE, BUF_NOISE, BUF_CTL, BUF_PHI, BUF_RNG = 0, 1, 2, 3, 4
 
 
 def _none(v):
@@ -118,7 +119,7 @@ class BatchRunner:
                 spec, k, domain, eq = hit_spec_kpar(c)
                 rng = np.random.default_rng(c.forcing_seed)
                 raw.append(initialize_hit_2d(c, domain, eq, rng))
-                self.rngs.append(np.random.default_rng(c.forcing_seed + 1))
+                self.rngs.append(pcg64_words(np.random.default_rng(c.forcing_seed + 1).bit_generator))
             else:
                 spec, k = riemann_spec_kpar(c, max_steps)
                 raw.append(None)
@@ -144,6 +145,7 @@ class BatchRunner:
         self.ctl_views = [self._view(i, BUF_CTL, torch.float64, (32,)) for i in range(n)]
         self.noise = [self._view(i, BUF_NOISE, torch.float64, (sp.ny, sp.nx)) for i, sp in enumerate(specs)] \
             if self.hit else None
+        self.rng_state = [self._view(i, BUF_RNG, torch.int64, (4,)) for i in range(n)] if self.hit else None
         # initial states: enforce (+ outflow guard), as the single-instance drivers
         for i, (c, sp) in enumerate(zip(self.configs, specs)):
             q0 = torch.from_numpy(raw[i]).to("cuda", sp.dtype) if self.hit else c.initial_state()
@@ -155,6 +157,8 @@ class BatchRunner:
             ctl[C_POWER] = kpars[i][11]
             ctl[C_ALPHA] = 1.0
             self.ctl_views[i].copy_(ctl)
+            if self.hit:
+                self.rng_state[i].copy_(torch.from_numpy(self.rngs[i].view(np.int64).copy()))
         if not self.hit:
             check(lib().hitx_batch_fixed_dt(h, self._stream()))
         self._pinned = None
@@ -179,15 +183,8 @@ class BatchRunner:
         nb = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
         return self.ws[off:off + nb].view(dtype).view(*shape)
 
-    def fill_noise(self):
-        """Draw each HIT instance's next nx*ny normals on the host."""
-        for i, sp in enumerate(self.specs):
-            a = self.rngs[i].standard_normal((sp.ny, sp.nx))
-            self.noise[i].copy_(torch.from_numpy(a), non_blocking=False)
-
     def step(self):
-        if self.hit:
-            self.fill_noise()
+        """One step of every active instance (asynchronous, no host work)."""
         check(lib().hitx_batch_step(self.handle, self._stream()))
 
     def control(self):

@@ -36,6 +36,39 @@ int fail(int code, const std::string& msg) {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Workspace of one normal-variate request (see hitx_rng.h).
+struct RngLayout {
+  size_t vals, att, map, entry, base, stat, total;
+};
+RngLayout rng_layout(long long n) {
+  const size_t nch = (size_t)rng_chunks(n), words = nch * kRngChunk;
+  RngLayout L;
+  L.vals = 0;
+  L.att = align256(L.vals + words * sizeof(double));
+  L.map = align256(L.att + words);
+  L.entry = align256(L.map + nch * kRngEntries * sizeof(uint16_t));
+  L.base = align256(L.entry + nch);
+  L.stat = align256(L.base + nch * sizeof(int));
+  L.total = align256(L.stat + 4 * sizeof(int));
+  return L;
+}
+RngJob make_rngjob(uint8_t* ws, const RngLayout& L, uint64_t* state, double* out, long long n) {
+  RngJob J;
+  J.state = state;
+  J.out = out;
+  J.vals = reinterpret_cast<double*>(ws + L.vals);
+  J.att = ws + L.att;
+  J.map = reinterpret_cast<uint16_t*>(ws + L.map);
+  J.entry = ws + L.entry;
+  J.base = reinterpret_cast<int*>(ws + L.base);
+  J.stat = reinterpret_cast<int*>(ws + L.stat);
+  J.n = n;
+  J.W = rng_words(n);
+  J.nch = rng_chunks(n);
+  return J;
+}
+constexpr long long kRngMaxN = 1ll << 30;
+
 constexpr double kAlphaC = 3.0 / 8.0;  // OOP/spatial_operator.py:240
 
 template <class T>
@@ -602,6 +635,24 @@ int hitx_viscous_rhs(hitx_plan* pl, const void* q, void* out, void* stream) {
   return HITX_OK;
 }
 
+size_t hitx_normals_workspace_bytes(int64_t n) { return n > 0 && n <= kRngMaxN ? rng_layout(n).total : 0; }
+
+int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* ws, size_t ws_bytes, void* stream) {
+  if (!state || !out || !ws) return fail(HITX_EINVAL, "null argument");
+  if (n <= 0 || n > kRngMaxN) return fail(HITX_EINVAL, "normal count out of range (1 .. 2^30)");
+  const RngLayout L = rng_layout(n);
+  if (ws_bytes < L.total) return fail(HITX_ENOMEM, "workspace too small for the normal variates");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const RngJob J = make_rngjob(static_cast<uint8_t*>(ws), L, state, out, n);
+  const int nb = (J.nch + kRngAttemptThreads - 1) / kRngAttemptThreads;
+  k_rng_attempts<<<nb, kRngAttemptThreads, 0, s>>>(J);
+  k_rng_scan<<<1, kRngScan, 0, s>>>(J, (flags & HITX_NORMALS_SERIAL) ? 1 : 0);
+  k_rng_scatter<<<nb, kRngAttemptThreads, 0, s>>>(J);
+  k_rng_finish<<<1, 64, 0, s>>>(J);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
 int hitx_timing_enable(hitx_plan* pl, int32_t on) {
   if (!pl) return fail(HITX_EINVAL, "null plan");
   pl->timing = on ? 1 : 0;
@@ -632,7 +683,8 @@ struct hitx_batch {
   std::vector<hitx_plan*> plans;
   std::vector<double> kpar;  // n * K_COUNT
   struct Buf {
-    size_t Q, Q1, Q2, S, qT, dfx, dfyT, fx, fy, mask, maskT, part, scal, ctl, kpar, phi, fhat, noise, fpart, kxy, rng;
+    size_t Q, Q1, Q2, S, qT, dfx, dfyT, fx, fy, mask, maskT, part, scal, ctl, kpar, phi, fhat, noise, fpart, kxy, rng,
+        rngws;
   };
   std::vector<Buf> buf;
   size_t off_tab = 0, off_offs = 0, ws_bytes = 0;
@@ -653,6 +705,8 @@ int batch_blocks(const hitx_plan* pl, int kind) {
     case OFF_LY: return nx;
     case OFF_T16: return ((nx + 15) / 16) * ((ny + 15) / 16);
     case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
+    case OFF_RNG:
+      return pl->d.n_force_modes > 0 ? (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads : 0;
     default: return 0;
   }
 }
@@ -672,6 +726,7 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   I.part = d(o.part); I.scal = d(o.scal); I.ctl = d(o.ctl); I.kpar = d(o.kpar); I.phi = d(o.phi);
   I.fhat = d(o.fhat); I.noise = d(o.noise); I.fpart = d(o.fpart); I.kxy = d(o.kxy);
   I.rng = reinterpret_cast<uint64_t*>(base + o.rng);
+  I.R = make_rngjob(base + o.rngws, rng_layout(pl->N), I.rng, I.noise, pl->N);
   I.nblk_stage = pl->nblk_stage;
   I.nblk_proj = pl->nblk_proj;
 }
@@ -742,6 +797,10 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
   b_step_reduce<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
   b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab);
   if (b->periodic && b->total[OFF_PROJ] > 0) {
+    b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
+    b_rng_scan<T><<<n, kRngScan, 0, s>>>(tab);
+    b_rng_scatter<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
+    b_rng_finish<T><<<n, 64, 0, s>>>(tab);
     b_fproject<T><<<b->total[OFF_PROJ], kBlock, 0, s>>>(tab, offs, n);
     b_fou<T><<<n, kBlock, 0, s>>>(tab);
     b_fsynth<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
@@ -846,6 +905,7 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     u.fpart = take((size_t)pl->nblk_proj * 2 * nf * sizeof(double));
     u.kxy = take(2 * nf * sizeof(double));
     u.rng = take(4 * sizeof(uint64_t));
+    u.rngws = take(pl->d.n_force_modes > 0 ? rng_layout(N).total : 8);
     b->buf.push_back(u);
   }
   b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "hitx_kernels.h"
+#include "hitx_rng.h"
 
 namespace hitx {
 
@@ -35,6 +36,7 @@ struct Inst {
   uint8_t *mask, *maskT;
   double *part, *scal, *ctl, *kpar, *phi, *fhat, *noise, *fpart, *kxy;
   uint64_t* rng;  // PCG64 state: state hi, state lo, increment hi, increment lo
+  RngJob R;       // forcing noise request: nx*ny normals of the rng stream into noise
   int nblk_stage, nblk_proj;
 };
 
@@ -236,6 +238,39 @@ __global__ void __launch_bounds__(kBlock) b_fproject(const Inst<T>* __restrict__
   HITX_BATCH_INST(OFF_PROJ)
   if (I.kpar[K_FORCE] == 0.0) return;
   fproject_body<T>(I.P, I.noise, I.fpart, lb, I.nblk_proj);
+}
+
+// The instance's next nx*ny standard normals (OOP/forcing.py:77), in place
+// of the host draw: four launches, see hitx_rng.h.
+template <class T>
+__global__ void __launch_bounds__(kRngAttemptThreads) b_rng_attempts(const Inst<T>* __restrict__ tab,
+                                                                     const int* __restrict__ offs, int ninst) {
+  HITX_BATCH_INST(OFF_RNG)
+  if (I.kpar[K_FORCE] == 0.0) return;
+  __shared__ uint8_t lds[kRngAttemptThreads * kRngChunk];
+  rng_attempt_body(I.R, lb * kRngAttemptThreads + threadIdx.x, lds + threadIdx.x * kRngChunk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
+  rng_scan_body(I.R, 0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRngAttemptThreads) b_rng_scatter(const Inst<T>* __restrict__ tab,
+                                                                    const int* __restrict__ offs, int ninst) {
+  HITX_BATCH_INST(OFF_RNG)
+  if (I.kpar[K_FORCE] == 0.0) return;
+  rng_scatter_body(I.R, lb * kRngAttemptThreads + threadIdx.x);
+}
+
+template <class T>
+__global__ void b_rng_finish(const Inst<T>* __restrict__ tab) {
+  const Inst<T>& I = tab[blockIdx.x];
+  if (threadIdx.x != 0 || I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
+  rng_finish_body(I.R);
 }
 
 // One block per instance: sum the projection partials, OU recurrence of the

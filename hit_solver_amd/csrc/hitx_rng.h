@@ -1,0 +1,303 @@
+// Device reproduction of NumPy's Generator(PCG64).standard_normal stream, the
+// forcing noise of the reference (OOP/forcing.py:66 default_rng(seed), :77
+// standard_normal((ny, nx))).
+//
+// Generator: PCG64 (O'Neill 2014), 128-bit LCG state s <- s*M + inc, output
+// XSL-RR of the new state; jump-ahead by the arbitrary-stride LCG recurrence
+// (F. Brown, "Random number generation with arbitrary strides", 1994).
+// Normals: 256-layer ziggurat (Marsaglia & Tsang 2000) in NumPy's form: one
+// 64-bit word per attempt gives the layer (low 8 bits), the sign (bit 8) and
+// a 52-bit magnitude; the wedge test takes one more word as a double, the
+// tail two per try.
+//
+// The stream is sequential (a rejected attempt shifts every later normal), so
+// the parallel form works on attempts, not normals:
+//   1. every word position p of the stream is evaluated as if an attempt
+//      started there: its length in words and whether it yields a value;
+//   2. positions are cut in chunks of kRngChunk words; for each chunk and each
+//      entry offset e < kRngEntries (how far the previous chunk's last attempt
+//      spilled into it) the attempt chain is walked to the chunk end, giving
+//      the spill into the next chunk and the number of values;
+//   3. these small maps are composed by a block-wide scan, which fixes the
+//      true entry offset and the index of the first value of every chunk;
+//   4. each chunk writes its values at their indices; the thread that writes
+//      value n-1 records how many words the n values consumed.
+// A spill of kRngEntries words or more, or too few values in the words
+// evaluated, sets a flag, and one thread then regenerates the whole request
+// sequentially.  Both forms give the same numbers.
+#pragma once
+
+#include "hitx_device.h"
+#include "hitx_normal_tables.h"
+
+namespace hitx {
+
+typedef unsigned __int128 u128;
+
+constexpr int kRngChunk = 32;    // words per chunk (one thread)
+constexpr int kRngEntries = 8;   // entry offsets tracked per chunk
+constexpr int kRngScan = 1024;   // threads of the scan block
+constexpr uint64_t kMask52 = (1ull << 52) - 1;
+
+__device__ __forceinline__ u128 pcg_mult() {
+  return ((u128)0x2360ED051FC65DA4ull << 64) | (u128)0x4385DF649FCCF645ull;
+}
+
+struct Pcg {
+  u128 s, inc;
+  __device__ __forceinline__ uint64_t next() {
+    s = s * pcg_mult() + inc;
+    const uint64_t x = (uint64_t)(s >> 64) ^ (uint64_t)s;
+    const unsigned rot = (unsigned)(s >> 122);
+    return (x >> rot) | (x << ((64u - rot) & 63u));
+  }
+};
+
+// s after `delta` steps of s <- s*M + inc.
+__device__ __forceinline__ u128 pcg_advance(u128 s, u128 inc, unsigned long long delta) {
+  u128 cm = pcg_mult(), cp = inc, am = 1, ap = 0;
+  while (delta) {
+    if (delta & 1ull) {
+      am = am * cm;
+      ap = ap * cm + cp;
+    }
+    cp = (cm + 1) * cp;
+    cm = cm * cm;
+    delta >>= 1;
+  }
+  return am * s + ap;
+}
+
+__device__ __forceinline__ Pcg pcg_load(const uint64_t* st) {
+  Pcg g;
+  g.s = ((u128)st[0] << 64) | (u128)st[1];
+  g.inc = ((u128)st[2] << 64) | (u128)st[3];
+  return g;
+}
+
+__device__ __forceinline__ double pcg_double(uint64_t w) { return (double)(w >> 11) * (1.0 / 9007199254740992.0); }
+
+struct ZigAttempt {
+  int len;     // words consumed by the attempt
+  bool out;    // the attempt yields a value
+  double v;
+};
+
+// Attempt starting with word w; g is positioned after w and is advanced only
+// on the local copy.  CAP bounds the tail loop so that the length fits the
+// 7-bit code of the parallel form (a tail run that long is reported as a
+// spill and the request is regenerated serially without the bound).
+template <bool CAP>
+__device__ __forceinline__ ZigAttempt zig_attempt(uint64_t w, Pcg g) {
+  const int idx = (int)(w & 0xff);
+  const uint64_t r = w >> 8;
+  const uint64_t rabs = (r >> 1) & kMask52;
+  double x = (double)rabs * kZigWi[idx];
+  if (r & 1ull) x = -x;
+  ZigAttempt a;
+  a.v = x;
+  if (rabs < kZigKi[idx]) {
+    a.len = 1;
+    a.out = true;
+    return a;
+  }
+  if (idx == 0) {
+    int len = 1;
+    for (;;) {
+      const double xx = -kZigInvR * log1p(-pcg_double(g.next()));
+      const double yy = -log1p(-pcg_double(g.next()));
+      len += 2;
+      if (yy + yy > xx * xx) {
+        a.v = ((rabs >> 8) & 1ull) ? -(kZigR + xx) : kZigR + xx;
+        a.len = len;
+        a.out = true;
+        return a;
+      }
+      if (CAP && len > 125) {
+        a.len = 127;
+        a.out = false;
+        return a;
+      }
+    }
+  }
+  const double u = pcg_double(g.next());
+  a.len = 2;
+  a.out = ((kZigFi[idx - 1] - kZigFi[idx]) * u + kZigFi[idx]) < exp(-0.5 * x * x);
+  return a;
+}
+
+// Buffers of one request of n normals (all in device memory).
+struct RngJob {
+  uint64_t* state;  // s hi, s lo, inc hi, inc lo; advanced past the n values
+  double* out;      // n values
+  double* vals;     // [kRngChunk][nch] attempt values (transposed: coalesced)
+  uint8_t* att;     // [kRngChunk][nch] attempt length | yields << 7
+  uint16_t* map;    // [nch][kRngEntries] spill | count << 8 (spill 0xff: overflow)
+  uint8_t* entry;   // [nch] resolved entry offset
+  int* base;        // [nch] index of the chunk's first value
+  int* stat;        // [0] flag: 0 parallel result, 1 overflow / too few words, 2 forced serial;
+                    // [1] words consumed by the n values
+  long long n;      // values requested
+  int W, nch;       // words evaluated, chunks
+};
+
+__host__ __device__ inline int rng_words(long long n) { return (int)(n + n / 16 + 64); }
+__host__ __device__ inline int rng_chunks(long long n) { return (rng_words(n) + kRngChunk - 1) / kRngChunk; }
+
+// Steps 1 and 2 for chunk c (one thread).  lds: kRngChunk bytes per thread.
+__device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c, uint8_t* lds) {
+  if (c >= J.nch) return;
+  const int p0 = c * kRngChunk;
+  Pcg g = pcg_load(J.state);
+  g.s = pcg_advance(g.s, g.inc, (unsigned long long)p0);
+  for (int j = 0; j < kRngChunk; ++j) {
+    const uint64_t w = g.next();
+    const ZigAttempt a = zig_attempt<true>(w, g);
+    const uint8_t code = (uint8_t)(a.len | (a.out ? 0x80 : 0));
+    J.vals[(size_t)j * J.nch + c] = a.v;
+    J.att[(size_t)j * J.nch + c] = code;
+    lds[j] = code;
+  }
+  for (int e = 0; e < kRngEntries; ++e) {
+    int p = e, cnt = 0;
+    while (p < kRngChunk) {
+      const uint8_t code = lds[p];
+      cnt += code >> 7;
+      p += code & 0x7f;
+    }
+    const int spill = p - kRngChunk;
+    J.map[(size_t)c * kRngEntries + e] = (uint16_t)((spill < kRngEntries ? spill : 0xff) | (cnt << 8));
+  }
+}
+
+// Step 3 (one block of kRngScan threads per request).  A chunk's map sends
+// the entry offset e to (spill, count); spill 0xff marks an overflow.  Each
+// thread composes the maps of its run of chunks, the block scans the run maps
+// (Hillis-Steele in LDS), and every run then knows its entry offset and the
+// index of its first value.  force_serial makes the last step regenerate
+// serially (a check of that path).
+__device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial) {
+  __shared__ uint8_t sx[kRngScan * kRngEntries];
+  __shared__ int sn[kRngScan * kRngEntries];
+  const int t = threadIdx.x;
+  const int per = (J.nch + kRngScan - 1) / kRngScan;
+  const int c0 = min(t * per, J.nch), c1 = min(c0 + per, J.nch);
+  uint8_t mx[kRngEntries];
+  int mn[kRngEntries];
+#pragma unroll
+  for (int e = 0; e < kRngEntries; ++e) {
+    mx[e] = (uint8_t)e;
+    mn[e] = 0;
+  }
+  for (int c = c0; c < c1; ++c) {
+#pragma unroll
+    for (int e = 0; e < kRngEntries; ++e) {
+      if (mx[e] < kRngEntries) {
+        const uint16_t v = J.map[(size_t)c * kRngEntries + mx[e]];
+        mx[e] = (uint8_t)(v & 0xff);
+        mn[e] += v >> 8;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < kRngEntries; ++e) {
+    sx[t * kRngEntries + e] = mx[e];
+    sn[t * kRngEntries + e] = mn[e];
+  }
+  __syncthreads();
+  for (int d = 1; d < kRngScan; d <<= 1) {
+    const bool has = t >= d;
+    if (has) {
+      // (left run then own run): own map read from LDS at the left's spill
+#pragma unroll
+      for (int e = 0; e < kRngEntries; ++e) {
+        const int x1 = sx[(t - d) * kRngEntries + e];
+        if (x1 < kRngEntries) {
+          mx[e] = sx[t * kRngEntries + x1];
+          mn[e] = sn[(t - d) * kRngEntries + e] + sn[t * kRngEntries + x1];
+        } else {
+          mx[e] = 0xff;
+          mn[e] = 0;
+        }
+      }
+    }
+    __syncthreads();
+    if (has) {
+#pragma unroll
+      for (int e = 0; e < kRngEntries; ++e) {
+        sx[t * kRngEntries + e] = mx[e];
+        sn[t * kRngEntries + e] = mn[e];
+      }
+    }
+    __syncthreads();
+  }
+  // exclusive prefix: the stream enters chunk 0 at offset 0
+  int e = 0, cnt = 0;
+  if (t > 0) {
+    e = sx[(t - 1) * kRngEntries];
+    cnt = sn[(t - 1) * kRngEntries];
+  }
+  for (int c = c0; c < c1 && e < kRngEntries; ++c) {
+    J.entry[c] = (uint8_t)e;
+    J.base[c] = cnt;
+    const uint16_t v = J.map[(size_t)c * kRngEntries + e];
+    e = v & 0xff;
+    cnt += v >> 8;
+  }
+  if (t == kRngScan - 1) {
+    const bool overflow = sx[t * kRngEntries] >= kRngEntries;
+    const bool short_ = sn[t * kRngEntries] < J.n;
+    J.stat[0] = (overflow || short_) ? 1 : (force_serial ? 2 : 0);
+  }
+}
+
+// Step 4 for chunk c (one thread).
+__device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c) {
+  if (c >= J.nch || J.stat[0] != 0) return;
+  int p = J.entry[c];
+  long long k = J.base[c];
+  while (p < kRngChunk) {
+    const uint8_t code = J.att[(size_t)p * J.nch + c];
+    const int len = code & 0x7f;
+    if (code & 0x80) {
+      if (k < J.n) J.out[k] = J.vals[(size_t)p * J.nch + c];
+      if (k == J.n - 1) J.stat[1] = c * kRngChunk + p + len;  // words consumed
+      ++k;
+    }
+    p += len;
+  }
+}
+
+// Last step (one thread): advance the state, or regenerate serially.
+__device__ __forceinline__ void rng_finish_body(const RngJob& J) {
+  Pcg g = pcg_load(J.state);
+  if (J.stat[0] == 0) {
+    g.s = pcg_advance(g.s, g.inc, (unsigned long long)(unsigned)J.stat[1]);
+  } else {
+    for (long long k = 0; k < J.n;) {
+      const uint64_t w = g.next();
+      const ZigAttempt a = zig_attempt<false>(w, g);
+      for (int j = 1; j < a.len; ++j) g.next();
+      if (a.out) J.out[k++] = a.v;
+    }
+  }
+  J.state[0] = (uint64_t)(g.s >> 64);
+  J.state[1] = (uint64_t)g.s;
+  J.stat[0] = 0;
+}
+
+constexpr int kRngAttemptThreads = 256;
+
+__global__ void __launch_bounds__(kRngAttemptThreads) k_rng_attempts(RngJob J) {
+  __shared__ uint8_t lds[kRngAttemptThreads * kRngChunk];
+  rng_attempt_body(J, blockIdx.x * kRngAttemptThreads + threadIdx.x, lds + threadIdx.x * kRngChunk);
+}
+__global__ void __launch_bounds__(kRngScan) k_rng_scan(RngJob J, int force_serial) { rng_scan_body(J, force_serial); }
+__global__ void __launch_bounds__(kRngAttemptThreads) k_rng_scatter(RngJob J) {
+  rng_scatter_body(J, blockIdx.x * kRngAttemptThreads + threadIdx.x);
+}
+__global__ void k_rng_finish(RngJob J) {
+  if (threadIdx.x == 0) rng_finish_body(J);
+}
+
+}  // namespace hitx

@@ -1,9 +1,10 @@
 """Solenoidal OU shell forcing on the hip backend (reference OOP/forcing.py:10-130).
 
-The noise stream is NumPy PCG64 ``default_rng(seed).standard_normal`` on the
-host, as on the reference's NumPy path (the reference's CuPy path draws a
-different stream, OOP/forcing.py:67-68).  Everything grid-sized runs on the
-device: the forward DFT of the noise restricted to the shell modes
+The noise stream is NumPy PCG64 ``default_rng(seed).standard_normal``, as on
+the reference's NumPy path (the reference's CuPy path draws a different
+stream, OOP/forcing.py:67-68), generated bit for bit on the device
+(``hitx_normals``).  Everything grid-sized runs on the device: the noise, the
+forward DFT of the noise restricted to the shell modes
 (``hitx_forcing_project``), the curl and inverse DFT (``hitx_forcing_synthesize``),
 the rescale and the statistics (``hitx_forcing_scale``).  The OU recurrence
 and the power-rescale logic act on the ~56 shell coefficients and scalars on
@@ -18,6 +19,7 @@ import numpy as np
 import torch
 
 from .domain import Domain2D
+from .rng import DeviceNormalStream
 from .runtime import PlanSpec, get_plan
 
 
@@ -67,8 +69,8 @@ class IsotropicShellOUForcing2D:
         if np.abs(self.modes).max() > 16 or len(self.modes) > 116:
             raise ValueError("forcing shell too wide for the device projection (|k index| <= 16)")
         self._phi = np.zeros(len(self.modes), dtype=np.complex128)
-        self._rng = np.random.default_rng(self.seed)
-        self._noise_host = None
+        self._rng = DeviceNormalStream(self.seed)
+        self._noise = None
 
     @property
     def mode_tuple(self):
@@ -80,12 +82,10 @@ class IsotropicShellOUForcing2D:
                                  force_modes=self.mode_tuple))
 
     def draw_noise(self):
-        """The next N normals of the PCG64 stream, on the device (float64)."""
-        n = self._rng.standard_normal((self.domain.ny, self.domain.nx))
-        if self._noise_host is None:
-            self._noise_host = torch.empty(n.shape, dtype=torch.float64, pin_memory=True)
-        self._noise_host.numpy()[...] = n
-        return self._noise_host.to("cuda", non_blocking=True)
+        """The next nx*ny normals of the PCG64 stream (float64, on the device)."""
+        if self._noise is None:
+            self._noise = torch.empty((self.domain.ny, self.domain.nx), dtype=torch.float64, device="cuda")
+        return self._rng.normals(self._noise.shape, out=self._noise)
 
     def update(self, dt: float, rho, u, v):
         """OOP/forcing.py:70-130: one OU step; returns (fx, fy, info)."""

@@ -163,6 +163,18 @@ int hitx_primitives(hitx_plan* plan, const void* q, void* rho, void* u, void* v,
  * adds when desc.viscosity > 0): out [4][ny][nx], out[0] = 0. */
 int hitx_viscous_rhs(hitx_plan* plan, const void* q, void* out, void* stream);
 
+/* Forcing noise: the next n standard normals of a NumPy Generator(PCG64)
+ * stream, bit for bit as numpy.random.default_rng(seed).standard_normal(n)
+ * returns them; replaces the host draw self._rng.standard_normal((ny, nx))
+ * of OOP/forcing.py:77 (generator created at OOP/forcing.py:66).
+ * state: device uint64[4] = PCG64 state (high, low word) and increment (high,
+ * low word), as bit_generator.state holds them; advanced past the n values.
+ * out: device double[n].  flags: HITX_NORMALS_SERIAL forces the one-thread
+ * form (the same numbers; a check of the parallel form).  n in 1 .. 2^30. */
+#define HITX_NORMALS_SERIAL 1
+size_t hitx_normals_workspace_bytes(int64_t n);
+int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* ws, size_t ws_bytes, void* stream);
+
 /* Measurement support (no reference counterpart): when enabled, every
  * line-sweep kernel launch of this plan is bracketed by HIP events on its
  * stream.  hitx_timing_read synchronises those events, returns the summed
@@ -189,8 +201,9 @@ int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
  *   7 hyperviscosity applications, 11 target power, 12 alpha, 18-21 Fxx Fyy Fxy P_in,
  *   22 power before rescale, 23 alpha target, 24 Mach, 25 desired power, 26 n steps (fixed dt).
  * The caller writes each instance's initial state (HITX_BUF_STATE), control
- * scalars and, for forced HIT, the noise of each step (HITX_BUF_NOISE, the
- * next nx*ny standard normals of the instance's stream) before hitx_batch_step. */
+ * scalars and, for forced HIT, the PCG64 state of its noise stream
+ * (HITX_BUF_RNG, 4 uint64 words as for hitx_normals); each step draws the
+ * instance's next nx*ny normals into HITX_BUF_NOISE on the device. */
 #define HITX_KPAR 24
 #define HITX_CTL 32
 enum { HITX_BUF_STATE = 0, HITX_BUF_NOISE, HITX_BUF_CTL, HITX_BUF_PHI, HITX_BUF_RNG, HITX_BUF_FX, HITX_BUF_FY };

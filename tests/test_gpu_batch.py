@@ -83,3 +83,34 @@ def test_hit_batch_vs_oracle():
     ref.run()
     for k in range(4):
         assert rel_l2(host(b.q[0])[k], ref.q[k]) <= 1e-10, k
+
+
+def assert_normals_match(got, ref):
+    """Bit-exact, except values from the ziggurat tail (|x| > r = 3.654...),
+    which go through log1p: the device's log1p and the host libm's may differ
+    in the last bit (1 ulp)."""
+    tail = np.abs(ref) > 3.6541528853610088
+    np.testing.assert_array_equal(got[~tail], ref[~tail])
+    np.testing.assert_array_max_ulp(got[tail], ref[tail], maxulp=1)
+
+
+def test_device_normals_match_numpy():
+    from hit_solver_amd.rng import DeviceNormalStream
+
+    for seed, sizes in [(1235, [1, 7, 100, 4096]), (0, [262144, 262144]), (63, [37 * 41, 5000, 3])]:
+        g = np.random.default_rng(seed)
+        d = DeviceNormalStream(seed)
+        for n in sizes:
+            ref = g.standard_normal(n)
+            assert_normals_match(host(d.normals(n)), ref)
+        s = g.bit_generator.state["state"]
+        assert d.host_state() == (s["state"], s["inc"])
+
+
+def test_device_normals_serial_form_agrees():
+    from hit_solver_amd.rng import DeviceNormalStream
+
+    a, b = DeviceNormalStream(99), DeviceNormalStream(99)
+    for n in (3000, 70000):
+        np.testing.assert_array_equal(host(a.normals(n)), host(b.normals(n, serial=True)))
+    assert a.host_state() == b.host_state()
