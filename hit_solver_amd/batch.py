@@ -187,6 +187,38 @@ class BatchRunner:
         """One step of every active instance (asynchronous, no host work)."""
         check(lib().hitx_batch_step(self.handle, self._stream()))
 
+    def tune(self, rb: int, rbp: int):
+        """Column-walk tile heights of every instance (rows per wave of the
+        stage reductions and of the forcing projection)."""
+        torch.cuda.synchronize()
+        check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp)))
+
+    def autotune(self, candidates=None, steps: int = 3):
+        """Time a few steps per candidate (rb, rbp) on a copy of the batch's
+        device state, restore the state and keep the fastest candidate.
+        Returns (best, {candidate: ms per step})."""
+        if candidates is None:
+            candidates = [(rb, rbp) for rb in (2, 4, 8, 16, 32) for rbp in ((16, 64) if self.hit else (16,))]
+        torch.cuda.synchronize()
+        backup = self.ws.clone()
+        times = {}
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for cand in candidates:
+            self.tune(*cand)
+            self.step()
+            e0.record()
+            for _ in range(steps):
+                self.step()
+            e1.record()
+            torch.cuda.synchronize()
+            times[cand] = e0.elapsed_time(e1) / steps
+            self.ws.copy_(backup)
+        torch.cuda.synchronize()
+        del backup
+        best = min(times, key=times.get)
+        self.tune(*best)
+        return best, times
+
     def control(self):
         """Control scalars of all instances (host numpy, [n, 32]); synchronises."""
         return torch.stack(self.ctl_views).cpu().numpy()

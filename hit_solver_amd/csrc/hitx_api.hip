@@ -2,6 +2,7 @@
 // Declarations and the reference interfaces they replace: include/hitx.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +36,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// bounds of the column-walk tile heights a batch may be tuned to
+constexpr int kTuneRbMin = 2, kTuneRbMax = 64, kTuneRbpMin = 8, kTuneRbpMax = 256;
 
 // Workspace of one normal-variate request (see hitx_rng.h).
 struct RngLayout {
@@ -117,7 +121,10 @@ struct hitx_plan {
   size_t off_raw = 0, off_mask = 0, off_maskT = 0, off_qT = 0, off_dfx = 0, off_dfyT = 0, off_part = 0,
          off_scal = 0, off_misc = 0, ws_bytes = 0;
   size_t sensor_lds = 0;
-  int nblk_red = 0, nblk_proj = 0, nblk_stage = 0;
+  int nblk_red = 0, nblk_proj = 0, nblk_stage = 0, nblk_sr = 0;
+  int rb = 2, rbp = 4;                // rows per wave of the column walks
+  int dkx = 0, dky = 0, fkx = 0, fky = 0;
+  size_t off_drows = 0, off_frows = 0;
   size_t line_lds_x = 0, line_lds_y = 0;
   // optional per-launch timing of the line-sweep kernel (HIP events on the launch stream)
   int timing = 0;
@@ -177,8 +184,17 @@ Params<T> make_params(const hitx_plan* pl) {
   P.twy = pl->d_tw ? pl->d_tw + (size_t)(2 * pl->kmx + 1) * d.nx * 2 : nullptr;
   P.drag_modes = pl->d_modes;
   P.force_modes = pl->d_modes ? pl->d_modes + 2 * d.n_drag_modes : nullptr;
+  P.rb = pl->rb;
+  P.rbp = pl->rbp;
+  P.dkx = pl->dkx;
+  P.dky = pl->dky;
+  P.fkx = pl->fkx;
+  P.fky = pl->fky;
   return P;
 }
+
+// dynamic LDS of the forcing projection: the lanes' column sums, 2*fky+1 per lane
+size_t proj_lds_bytes(const hitx_plan* pl) { return (size_t)kWaves * 64 * (2 * pl->fky + 1) * sizeof(double); }
 
 size_t line_lds_bytes(int n, int nch, int esize) {
   return (size_t)(12 * (n + 1) + 66 + 8 * nch + 20) * esize + 4 * (size_t)(n + 1) + n + 64;
@@ -200,7 +216,7 @@ const LineK<float>& linek<float>(const hitx_plan* pl, int axis) { return axis ==
 struct WS {
   uint8_t *raw, *mask, *maskT;
   void *qT, *dfx, *dfyT;
-  double *part, *scal, *misc;
+  double *part, *scal, *misc, *drows, *frows;
 };
 
 WS carve(const hitx_plan* pl, void* ws) {
@@ -215,6 +231,8 @@ WS carve(const hitx_plan* pl, void* ws) {
   w.part = reinterpret_cast<double*>(b + pl->off_part);
   w.scal = reinterpret_cast<double*>(b + pl->off_scal);
   w.misc = reinterpret_cast<double*>(b + pl->off_misc);
+  w.drows = reinterpret_cast<double*>(b + pl->off_drows);
+  w.frows = reinterpret_cast<double*>(b + pl->off_frows);
   return w;
 }
 
@@ -233,11 +251,12 @@ template <class T>
 int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, T* qT, const WS& w, hipStream_t s) {
   const Params<T> P = make_params<T>(pl);
   if (P.drag_on) {
-    k_stage_reduce<T, true><<<pl->nblk_stage, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_stage, w.scal);
+    k_stage_reduce<T, true><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_sr, w.scal);
+    k_drows<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.scal, w.drows);
   } else {
-    k_stage_reduce<T, false><<<pl->nblk_stage, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_stage, w.scal);
+    k_stage_reduce<T, false><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_sr, w.scal);
   }
   dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
   k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
@@ -319,6 +338,10 @@ int validate_desc(const hitx_desc* d) {
     return fail(HITX_EINVAL, "mode list too long (drag <= 12, total <= HITX_MAX_MODES)");
   if ((d->n_drag_modes && !d->drag_modes_host) || (d->n_force_modes && !d->force_modes_host))
     return fail(HITX_EINVAL, "mode list pointer is null");
+  for (int m = 0; m < 2 * d->n_drag_modes; ++m)
+    if (std::abs(d->drag_modes_host[m]) > 2) return fail(HITX_EINVAL, "drag mode index beyond 2");
+  for (int m = 0; m < 2 * d->n_force_modes; ++m)  // projection LDS holds 2*|ky|+1 sums per lane
+    if (std::abs(d->force_modes_host[m]) > 12) return fail(HITX_EINVAL, "forcing mode index beyond 12");
   return HITX_OK;
 }
 
@@ -331,8 +354,8 @@ int rhs_or_stage(hitx_plan* pl, int mode, int stage, const void* q0, const void*
   const Params<T> P = make_params<T>(pl);
   k_update<T><<<grid16(pl), kBlock, 0, s>>>(P, mode, stage, static_cast<const T*>(q0), static_cast<const T*>(qs),
                                             static_cast<const T*>(w.dfx), static_cast<const T*>(w.dfyT),
-                                            static_cast<const T*>(fx), static_cast<const T*>(fy), w.scal, (T)dt,
-                                            static_cast<T*>(out));
+                                            static_cast<const T*>(fx), static_cast<const T*>(fy), w.scal, w.drows,
+                                            (T)dt, static_cast<T*>(out));
   LAUNCHCHK();
   return HITX_OK;
 }
@@ -375,13 +398,21 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
     km_x = std::max(km_x, std::abs(all[i]));
     km_y = std::max(km_y, std::abs(all[i + 1]));
   }
+  for (size_t i = 0; i < pl->drag_modes.size(); i += 2) {
+    pl->dkx = std::max(pl->dkx, std::abs(pl->drag_modes[i]));
+    pl->dky = std::max(pl->dky, std::abs(pl->drag_modes[i + 1]));
+  }
+  for (size_t i = 0; i < pl->force_modes.size(); i += 2) {
+    pl->fkx = std::max(pl->fkx, std::abs(pl->force_modes[i]));
+    pl->fky = std::max(pl->fky, std::abs(pl->force_modes[i + 1]));
+  }
   if (km_x > 16 || km_y > 16) {
     delete pl;
     return fail(HITX_EINVAL, "mode index beyond 16");
   }
   pl->kmx = km_x;
   pl->kmy = km_y;
-  if (!all.empty()) {
+  if (desc->n_drag_modes + desc->n_force_modes > 0) {
     const int nx = desc->nx, ny = desc->ny;
     std::vector<double> tw((size_t)(2 * km_x + 1) * nx * 2 + (size_t)(2 * km_y + 1) * ny * 2);
     const double two_pi = 2.0 * M_PI;
@@ -423,8 +454,13 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   // workspace layout
   const size_t N = pl->N, E = pl->esize;
   pl->nblk_red = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 1024);
-  pl->nblk_stage = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 512);  // stage reductions: fewer partials
-  pl->nblk_proj = (int)((N + (size_t)kBlock * kPPT - 1) / ((size_t)kBlock * kPPT));
+  pl->nblk_stage = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 512);  // grid-stride reductions
+  // column walks: rows per wave so that a plan alone still gives ~2048 waves
+  const int nseg1 = (desc->nx + 61) / 62, nseg0 = (desc->nx + 63) / 64;
+  pl->rb = std::max(2, std::min(32, nseg1 * desc->ny / 2048));
+  pl->rbp = std::max(8, std::min(64, nseg0 * desc->ny / 1024));
+  pl->nblk_sr = walk_blocks(desc->nx, desc->ny, pl->rb, 1);
+  pl->nblk_proj = walk_blocks(desc->nx, desc->ny, pl->rbp, 0);
   size_t o = 0;
   pl->off_raw = o; o = align256(o + N);
   pl->off_mask = o; o = align256(o + N);
@@ -432,11 +468,14 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->off_qT = o; o = align256(o + 4 * N * E);
   pl->off_dfx = o; o = align256(o + 4 * N * E);
   pl->off_dfyT = o; o = align256(o + 4 * N * E);
-  const size_t part = std::max<size_t>((size_t)pl->nblk_red * 64,
-                                       (size_t)pl->nblk_proj * 2 * std::max(1, desc->n_force_modes));
+  const size_t part = std::max({(size_t)pl->nblk_red * 64, (size_t)pl->nblk_sr * 64,
+                                 (size_t)pl->nblk_proj * 2 * std::max(1, desc->n_force_modes)});
   pl->off_part = o; o = align256(o + part * sizeof(double));
   pl->off_scal = o; o = align256(o + S_COUNT * sizeof(double));
   pl->off_misc = o; o = align256(o + 64 * sizeof(double));
+  pl->off_drows = o; o = align256(o + (pl->dkx + pl->dky > 0 ? (size_t)desc->ny * 10 : 1) * sizeof(double));
+  pl->off_frows = o; o = align256(o + (desc->n_force_modes > 0 ? (size_t)desc->ny * 2 * (2 * pl->fkx + 1) : 1) *
+                                          sizeof(double));
   pl->ws_bytes = o;
   *out = pl;
   return HITX_OK;
@@ -545,10 +584,11 @@ int hitx_forcing_project(hitx_plan* pl, const double* noise, double* modes_out, 
   if (pl->d.n_force_modes <= 0) return fail(HITX_EINVAL, "plan has no forcing modes");
   const WS w = carve(pl, ws);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t lds = proj_lds_bytes(pl);
   if (pl->esize == 8)
-    k_fproject<double><<<pl->nblk_proj, kBlock, 0, s>>>(make_params<double>(pl), noise, w.part);
+    k_fproject<double><<<pl->nblk_proj, kBlock, lds, s>>>(make_params<double>(pl), noise, w.part);
   else
-    k_fproject<float><<<pl->nblk_proj, kBlock, 0, s>>>(make_params<float>(pl), noise, w.part);
+    k_fproject<float><<<pl->nblk_proj, kBlock, lds, s>>>(make_params<float>(pl), noise, w.part);
   k_finalize_sum<<<2 * pl->d.n_force_modes, kBlock, 0, s>>>(w.part, pl->nblk_proj, 2 * pl->d.n_force_modes,
                                                             modes_out);
   LAUNCHCHK();
@@ -563,13 +603,18 @@ int hitx_forcing_synthesize(hitx_plan* pl, const double* fhat, void* fx, void* f
   if (pl->d.n_force_modes <= 0) return fail(HITX_EINVAL, "plan has no forcing modes");
   const WS w = carve(pl, ws);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nbr = (pl->d.ny * (pl->fkx + 1) + kBlock - 1) / kBlock;
   if (pl->esize == 8)
-    k_fsynth<double><<<pl->nblk_red, kBlock, 0, s>>>(make_params<double>(pl), fhat, static_cast<double*>(fx),
+    k_frows<double><<<nbr, kBlock, 0, s>>>(make_params<double>(pl), fhat, w.frows);
+  else
+    k_frows<float><<<nbr, kBlock, 0, s>>>(make_params<float>(pl), fhat, w.frows);
+  if (pl->esize == 8)
+    k_fsynth<double><<<pl->nblk_red, kBlock, 0, s>>>(make_params<double>(pl), w.frows, static_cast<double*>(fx),
                                                      static_cast<double*>(fy), static_cast<const double*>(rho),
                                                      static_cast<const double*>(u), static_cast<const double*>(v),
                                                      w.part);
   else
-    k_fsynth<float><<<pl->nblk_red, kBlock, 0, s>>>(make_params<float>(pl), fhat, static_cast<float*>(fx),
+    k_fsynth<float><<<pl->nblk_red, kBlock, 0, s>>>(make_params<float>(pl), w.frows, static_cast<float*>(fx),
                                                     static_cast<float*>(fy), static_cast<const float*>(rho),
                                                     static_cast<const float*>(u), static_cast<const float*>(v), w.part);
   k_finalize_sum<<<2, kBlock, 0, s>>>(w.part, pl->nblk_red, 2, stats_out);
@@ -624,12 +669,14 @@ int hitx_viscous_rhs(hitx_plan* pl, const void* q, void* out, void* stream) {
     Params<double> P = make_params<double>(pl);
     P.visc_on = 1;
     k_update<double><<<grid16(pl), kBlock, 0, s>>>(P, MODE_VISC, 0, nullptr, static_cast<const double*>(q), nullptr,
-                                                   nullptr, nullptr, nullptr, nullptr, 0.0, static_cast<double*>(out));
+                                                   nullptr, nullptr, nullptr, nullptr, nullptr, 0.0,
+                                                   static_cast<double*>(out));
   } else {
     Params<float> P = make_params<float>(pl);
     P.visc_on = 1;
     k_update<float><<<grid16(pl), kBlock, 0, s>>>(P, MODE_VISC, 0, nullptr, static_cast<const float*>(q), nullptr,
-                                                  nullptr, nullptr, nullptr, nullptr, 0.0f, static_cast<float*>(out));
+                                                  nullptr, nullptr, nullptr, nullptr, nullptr, 0.0f,
+                                                  static_cast<float*>(out));
   }
   LAUNCHCHK();
   return HITX_OK;
@@ -684,13 +731,13 @@ struct hitx_batch {
   std::vector<double> kpar;  // n * K_COUNT
   struct Buf {
     size_t Q, Q1, Q2, S, qT, dfx, dfyT, fx, fy, mask, maskT, part, scal, ctl, kpar, phi, fhat, noise, fpart, kxy, rng,
-        rngws;
+        rngws, drows, frows;
   };
   std::vector<Buf> buf;
   size_t off_tab = 0, off_offs = 0, ws_bytes = 0;
   int total[OFF_KINDS] = {0};
   std::vector<int> offs;  // OFF_KINDS * (n + 1)
-  size_t line_lds = 0, sensor_lds = 0;
+  size_t line_lds = 0, sensor_lds = 0, proj_lds = 0;
   void* ws = nullptr;
 };
 
@@ -700,6 +747,9 @@ int batch_blocks(const hitx_plan* pl, int kind) {
   const int nx = pl->d.nx, ny = pl->d.ny;
   switch (kind) {
     case OFF_RED: return pl->nblk_stage;
+    case OFF_SR: return pl->nblk_sr;
+    case OFF_DROWS: return pl->d.drag_enabled && pl->d.n_drag_modes > 0 ? (2 * ny + kBlock - 1) / kBlock : 0;
+    case OFF_FROWS: return pl->d.n_force_modes > 0 ? (ny * (pl->fkx + 1) + kBlock - 1) / kBlock : 0;
     case OFF_SENS: return ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
     case OFF_LX: return ny;
     case OFF_LY: return nx;
@@ -708,6 +758,20 @@ int batch_blocks(const hitx_plan* pl, int kind) {
     case OFF_RNG:
       return pl->d.n_force_modes > 0 ? (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads : 0;
     default: return 0;
+  }
+}
+
+void batch_offsets(hitx_batch* b) {
+  const int n = b->n;
+  b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);
+  for (int k = 0; k < OFF_KINDS; ++k) {
+    int acc = 0;
+    for (int i = 0; i < n; ++i) {
+      b->offs[(size_t)k * (n + 1) + i] = acc;
+      acc += batch_blocks(b->plans[i], k);
+    }
+    b->offs[(size_t)k * (n + 1) + n] = acc;
+    b->total[k] = acc;
   }
 }
 
@@ -727,17 +791,21 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   I.fhat = d(o.fhat); I.noise = d(o.noise); I.fpart = d(o.fpart); I.kxy = d(o.kxy);
   I.rng = reinterpret_cast<uint64_t*>(base + o.rng);
   I.R = make_rngjob(base + o.rngws, rng_layout(pl->N), I.rng, I.noise, pl->N);
+  I.drows = d(o.drows);
+  I.frows = d(o.frows);
   I.nblk_stage = pl->nblk_stage;
+  I.nblk_sr = pl->nblk_sr;
   I.nblk_proj = pl->nblk_proj;
 }
 
 template <class T>
-int batch_bind_t(hitx_batch* b, void* ws) {
+int batch_bind_t(hitx_batch* b, void* ws, bool constants) {
   uint8_t* base = static_cast<uint8_t*>(ws);
   std::vector<Inst<T>> tab(b->n);
   for (int i = 0; i < b->n; ++i) fill_inst<T>(b, i, base, tab[i]);
   HIPCHK(hipMemcpy(base + b->off_tab, tab.data(), tab.size() * sizeof(Inst<T>), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(base + b->off_offs, b->offs.data(), b->offs.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!constants) return HITX_OK;
   for (int i = 0; i < b->n; ++i) {
     const hitx_plan* pl = b->plans[i];
     HIPCHK(hipMemcpy(base + b->buf[i].kpar, &b->kpar[(size_t)i * K_COUNT], K_COUNT * sizeof(double),
@@ -770,8 +838,9 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const Inst<T>* tab = btab<T>(b);
   const int* offs = boffs(b);
   const int n = b->n;
-  b_stage_reduce<T, DRAG><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n, st);
+  b_stage_reduce<T, DRAG><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
   b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st);
+  if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
   b_sensor<T><<<b->total[OFF_SENS], kBlock, b->sensor_lds, s>>>(tab, offs, n, st);
   if (st == 3) {
     LAUNCHCHK();
@@ -801,8 +870,9 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
     b_rng_scan<T><<<n, kRngScan, 0, s>>>(tab);
     b_rng_scatter<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
     b_rng_finish<T><<<n, 64, 0, s>>>(tab);
-    b_fproject<T><<<b->total[OFF_PROJ], kBlock, 0, s>>>(tab, offs, n);
+    b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, s>>>(tab, offs, n);
     b_fou<T><<<n, kBlock, 0, s>>>(tab);
+    b_frows<T><<<b->total[OFF_FROWS], kBlock, 0, s>>>(tab, offs, n);
     b_fsynth<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
     b_falpha<T><<<n, kBlock, 0, s>>>(tab);
     b_fscale<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
@@ -873,6 +943,7 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     b->plans.push_back(pl);
     b->line_lds = std::max({b->line_lds, pl->line_lds_x, pl->line_lds_y});
     b->sensor_lds = std::max(b->sensor_lds, pl->sensor_lds);
+    b->proj_lds = std::max(b->proj_lds, proj_lds_bytes(pl));
   }
   b->drag = make_params<double>(b->plans[0]).drag_on;
   int rc = b->esize == 8 ? batch_set_lds<double>(b) : batch_set_lds<float>(b);
@@ -895,29 +966,25 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     u.qT = take(4 * N * E); u.dfx = take(4 * N * E); u.dfyT = take(4 * N * E);
     u.fx = take(N * E); u.fy = take(N * E);
     u.mask = take(N); u.maskT = take(N);
-    u.part = take((size_t)pl->nblk_stage * 64 * sizeof(double));
+    // partials sized for the smallest tile heights hitx_batch_tune accepts
+    const int sr_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbMin, 1);
+    const int pj_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbpMin, 0);
+    u.part = take((size_t)std::max({pl->nblk_stage, pl->nblk_sr, sr_max}) * 64 * sizeof(double));
     u.scal = take(S_COUNT * sizeof(double));
     u.ctl = take(C_COUNT * sizeof(double));
     u.kpar = take(K_COUNT * sizeof(double));
     u.phi = take(2 * nf * sizeof(double));
     u.fhat = take(4 * nf * sizeof(double));
     u.noise = take(pl->d.n_force_modes > 0 ? N * sizeof(double) : 8);
-    u.fpart = take((size_t)pl->nblk_proj * 2 * nf * sizeof(double));
+    u.fpart = take((size_t)std::max(pl->nblk_proj, pj_max) * 2 * nf * sizeof(double));
     u.kxy = take(2 * nf * sizeof(double));
     u.rng = take(4 * sizeof(uint64_t));
     u.rngws = take(pl->d.n_force_modes > 0 ? rng_layout(N).total : 8);
+    u.drows = take((pl->dkx + pl->dky > 0 ? (size_t)pl->d.ny * 10 : 1) * sizeof(double));
+    u.frows = take((pl->d.n_force_modes > 0 ? (size_t)pl->d.ny * 2 * (2 * pl->fkx + 1) : 1) * sizeof(double));
     b->buf.push_back(u);
   }
-  b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);
-  for (int k = 0; k < OFF_KINDS; ++k) {
-    int acc = 0;
-    for (int i = 0; i < n; ++i) {
-      b->offs[(size_t)k * (n + 1) + i] = acc;
-      acc += batch_blocks(b->plans[i], k);
-    }
-    b->offs[(size_t)k * (n + 1) + n] = acc;
-    b->total[k] = acc;
-  }
+  batch_offsets(b);
   const size_t inst_bytes = b->esize == 8 ? sizeof(Inst<double>) : sizeof(Inst<float>);
   b->off_tab = o; o = align256(o + inst_bytes * n);
   b->off_offs = o; o = align256(o + b->offs.size() * sizeof(int));
@@ -939,7 +1006,7 @@ int hitx_batch_bind(hitx_batch* b, void* ws, size_t ws_bytes) {
   if (!b || !ws) return fail(HITX_EINVAL, "null argument");
   if (ws_bytes < b->ws_bytes) return fail(HITX_ENOMEM, "batch workspace too small");
   b->ws = ws;
-  return b->esize == 8 ? batch_bind_t<double>(b, ws) : batch_bind_t<float>(b, ws);
+  return b->esize == 8 ? batch_bind_t<double>(b, ws, true) : batch_bind_t<float>(b, ws, true);
 }
 
 int hitx_batch_buffer(const hitx_batch* b, int32_t i, int32_t which, void** ptr) {
@@ -958,6 +1025,21 @@ int hitx_batch_buffer(const hitx_batch* b, int32_t i, int32_t which, void** ptr)
     default: return fail(HITX_EINVAL, "unknown buffer");
   }
   *ptr = static_cast<uint8_t*>(b->ws) + off;
+  return HITX_OK;
+}
+
+int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp) {
+  if (!b) return fail(HITX_EINVAL, "null batch");
+  if (rb < kTuneRbMin || rb > kTuneRbMax || rbp < kTuneRbpMin || rbp > kTuneRbpMax)
+    return fail(HITX_EINVAL, "tile heights out of range (rb 2..64, rbp 8..256)");
+  for (hitx_plan* pl : b->plans) {
+    pl->rb = rb;
+    pl->rbp = rbp;
+    pl->nblk_sr = walk_blocks(pl->d.nx, pl->d.ny, rb, 1);
+    pl->nblk_proj = walk_blocks(pl->d.nx, pl->d.ny, rbp, 0);
+  }
+  batch_offsets(b);
+  if (b->ws) return b->esize == 8 ? batch_bind_t<double>(b, b->ws, false) : batch_bind_t<float>(b, b->ws, false);
   return HITX_OK;
 }
 

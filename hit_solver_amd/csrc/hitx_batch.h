@@ -37,11 +37,12 @@ struct Inst {
   double *part, *scal, *ctl, *kpar, *phi, *fhat, *noise, *fpart, *kxy;
   uint64_t* rng;  // PCG64 state: state hi, state lo, increment hi, increment lo
   RngJob R;       // forcing noise request: nx*ny normals of the rng stream into noise
-  int nblk_stage, nblk_proj;
+  double *drows, *frows;  // per-row synthesis tables: drag low-pass, forcing
+  int nblk_stage, nblk_sr, nblk_proj;
 };
 
 // offsets kinds (rows of the offsets table, each ninst+1 ints)
-enum { OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_KINDS };
+enum { OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_KINDS };
 
 __device__ __forceinline__ int find_inst(const int* __restrict__ o, int n, int b) {
   int lo = 0, hi = n;  // o[0] = 0 <= b < o[n]
@@ -77,16 +78,24 @@ __device__ __forceinline__ T* stage_out(const Inst<T>& I, int st) {
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                          int ninst, int st) {
-  HITX_BATCH_INST(OFF_RED)
+  HITX_BATCH_INST(OFF_SR)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
-  stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb, I.nblk_stage);
+  stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb);
 }
 
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __restrict__ tab, int st) {
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || (st == 3 && I.ctl[C_HVNOW] == 0.0)) return;
-  stage_finalize_body<T, DRAG>(I.P, I.part, I.nblk_stage, I.scal);
+  stage_finalize_body<T, DRAG>(I.P, I.part, I.nblk_sr, I.scal);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_drows(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                  int ninst, int st) {
+  HITX_BATCH_INST(OFF_DROWS)
+  if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  drows_body<T>(I.P, I.scal, I.drows, lb * kBlock + threadIdx.x);
 }
 
 template <class T>
@@ -115,7 +124,7 @@ __global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ t
   HITX_BATCH_INST(OFF_T16)
   const int ntx = (I.P.nx + 15) / 16;
   const int by = lb / ntx, bx = lb - by * ntx;
-  update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, (T)I.ctl[C_DT],
+  update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
                  stage_out(I, st), bx, by);
 }
 
@@ -237,7 +246,7 @@ __global__ void __launch_bounds__(kBlock) b_fproject(const Inst<T>* __restrict__
                                                      int ninst) {
   HITX_BATCH_INST(OFF_PROJ)
   if (I.kpar[K_FORCE] == 0.0) return;
-  fproject_body<T>(I.P, I.noise, I.fpart, lb, I.nblk_proj);
+  fproject_body<T>(I.P, I.noise, I.fpart, lb);
 }
 
 // The instance's next nx*ny standard normals (OOP/forcing.py:77), in place
@@ -299,6 +308,14 @@ __global__ void __launch_bounds__(kBlock) b_fou(const Inst<T>* __restrict__ tab)
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_frows(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                  int ninst) {
+  HITX_BATCH_INST(OFF_FROWS)
+  if (I.kpar[K_FORCE] == 0.0) return;
+  frows_body<T>(I.P, I.fhat, I.frows, lb * kBlock + threadIdx.x);
+}
+
 // fx, fy from the shell coefficients, with the primitives of Q for the power.
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
@@ -311,16 +328,8 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
   double acc[2] = {0.0, 0.0};
   for (int idx = lb * kBlock + threadIdx.x; idx < N; idx += I.nblk_stage * kBlock) {
     const int y = idx / nx, x = idx - y * nx;
-    double ax = 0.0, ay = 0.0;
-    for (int m = 0; m < P.nforce; ++m) {
-      const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
-      const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
-      const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + y) * 2;
-      const double cr = tx[0] * ty[0] - tx[1] * ty[1];
-      const double si = tx[1] * ty[0] + tx[0] * ty[1];
-      ax += I.fhat[2 * m] * cr - I.fhat[2 * m + 1] * si;
-      ay += I.fhat[2 * P.nforce + 2 * m] * cr - I.fhat[2 * P.nforce + 2 * m + 1] * si;
-    }
+    double ax, ay;
+    fsynth_at(P, I.frows, y, x, ax, ay);
     const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
     I.fx[idx] = fxv;
     I.fy[idx] = fyv;

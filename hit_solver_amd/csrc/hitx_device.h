@@ -44,6 +44,10 @@ struct Params {
   const double* twy;                  // [(2*kmy+1)][ny][2]
   const int* drag_modes;              // [ndrag][2]
   const int* force_modes;             // [nforce][2]
+  // separable mode sums (column walks and per-row tables)
+  int rb, rbp;                        // rows per wave: stage reductions, forcing projection
+  int dkx, dky;                       // max |kx|, |ky| of the drag modes (<= 2)
+  int fkx, fky;                       // max |kx|, |ky| of the forcing modes (<= 12)
 };
 
 // Constants of the line solve along one axis (see solve_lines()).

@@ -6,6 +6,7 @@
 // sweeps stage whole lines through LDS with unit-stride global access.
 #pragma once
 
+#include "../../include/hitx.h"
 #include "hitx_device.h"
 
 namespace hitx {
@@ -17,87 +18,196 @@ template <class T>
 __device__ __forceinline__ T absT(T a) { return fabs(a); }
 
 // ================================================================ reductions
+// ---------------------------------------------------------------- column walks
+// A wave covers 64 adjacent columns and walks down a run of rows.  With a
+// halo of one column the edge lanes only supply x-neighbours (62 columns
+// owned per wave); rows are walked with the previous and next row held in
+// registers, so every node's primitives are formed once (plus one halo row
+// above and below the run).  Twiddles of a row are wave-uniform.
+struct Walk {
+  int x, xl, ya, yb;  // lane column, column loaded (wrapped or clamped), rows [ya, yb)
+  bool own, live;     // lane owns column x; wave has rows
+};
+
+__device__ __forceinline__ Walk walk_setup(int nx, int ny, int periodic, int rb, int halo, int bid) {
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gw = bid * kWaves + wid;
+  const int segw = 64 - 2 * halo;
+  const int nseg = (nx + segw - 1) / segw;
+  const int seg = gw % nseg, rc = gw / nseg;
+  Walk w;
+  w.ya = rc * rb;
+  w.yb = min(w.ya + rb, ny);
+  w.live = w.ya < ny;
+  w.x = seg * segw - halo + lane;
+  w.own = w.live && lane >= halo && lane < 64 - halo && w.x < nx;
+  w.xl = periodic ? ((w.x % nx) + nx) % nx : clampi(w.x, 0, nx - 1);
+  return w;
+}
+
+__host__ __device__ inline int walk_blocks(int nx, int ny, int rb, int halo) {
+  const int segw = 64 - 2 * halo;
+  const int waves = ((nx + segw - 1) / segw) * ((ny + rb - 1) / rb);
+  return (waves + kWaves - 1) / kWaves;
+}
+
+template <class T>
+__device__ __forceinline__ T lane_from(T v, int src) {
+  return __shfl(v, src & 63, 64);
+}
+
+// Sum (or max when MAX) over the 64 lanes of a wave; every lane gets it.
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double a) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(a, off, 64);
+    a = MAX ? fmax(a, o) : a + o;
+  }
+  return a;
+}
+
 // Per-stage reductions that gate the sensor and the flux split
 // (OOP/spatial_operator.py:148 eps, :271 div_rms, :279 vort_rms, :332 alpha),
 // plus the HIT closure's cooling mean pressure (OOP/hit2d.py:423) and drag
 // projection onto the low-k modes (OOP/hit2d.py:391-394).  Sensor inputs use
 // enforce(q) when enforce_first (the operator rhs); cooling and drag use q as
 // given (the RK stage state), as the reference does.
+//
+// Column walk (halo 1, P.rb rows per wave).  The drag projection is
+// separable: each lane sums rho*u and rho*v times cos/sin(ky*y) for
+// ky = 0..dky over its rows, and multiplies by exp(-i*kx*x) once at the end.
 template <class T, bool DRAG>
 __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* __restrict__ q,
-                                                         int enforce_first, double* __restrict__ part, int bid, int nblk) {
+                                                  int enforce_first, double* __restrict__ part, int bid) {
   constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
-  double v[NR];
+  double v[R_DRAG];
 #pragma unroll
-  for (int k = 0; k < NR; ++k) v[k] = 0.0;
+  for (int k = 0; k < R_DRAG; ++k) v[k] = 0.0;
+  double bs[2][5];  // [rho u, rho v][1, cos y, sin y, cos 2y, sin 2y] column sums
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) bs[f][j] = 0.0;
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
-    const int y = idx / nx, x = idx - y * nx;
-    const Prim<T> c = prim_at(P, q, N, idx, enforce_first);
-    const T cs = sqrt((P.gamma * c.p) / c.rho);
-    v[R_MAXR] = fmax(v[R_MAXR], (double)absT(c.rho));
-    v[R_MAXP] = fmax(v[R_MAXP], (double)absT(c.p));
-    const T e = c.p / (c.rho * P.gm1);
-    v[R_MAXE] = fmax(v[R_MAXE], (double)absT(e));
-    v[R_AX] = fmax(v[R_AX], (double)(absT(c.u) + cs));
-    v[R_AY] = fmax(v[R_AY], (double)(absT(c.v) + cs));
-    T ux, uy, vx, vy;
-    if (P.periodic) {
-      const Prim<T> xr = prim_at(P, q, N, y * nx + wrap(x + 1, nx), enforce_first);
-      const Prim<T> xl = prim_at(P, q, N, y * nx + wrap(x - 1, nx), enforce_first);
-      const Prim<T> yr = prim_at(P, q, N, wrap(y + 1, ny) * nx + x, enforce_first);
-      const Prim<T> yl = prim_at(P, q, N, wrap(y - 1, ny) * nx + x, enforce_first);
-      ux = (xr.u - xl.u) / P.two_dx;
-      vx = (xr.v - xl.v) / P.two_dx;
-      uy = (yr.u - yl.u) / P.two_dy;
-      vy = (yr.v - yl.v) / P.two_dy;
-    } else {  // OOP/problems/riemann_config3.py:268-275
-      const int xa = x == 0 ? 0 : x - 1, xb = x == nx - 1 ? nx - 1 : x + 1;
-      const int ya = y == 0 ? 0 : y - 1, yb = y == ny - 1 ? ny - 1 : y + 1;
-      const Prim<T> xr = prim_at(P, q, N, y * nx + xb, enforce_first);
-      const Prim<T> xl = prim_at(P, q, N, y * nx + xa, enforce_first);
-      const Prim<T> yr = prim_at(P, q, N, yb * nx + x, enforce_first);
-      const Prim<T> yl = prim_at(P, q, N, ya * nx + x, enforce_first);
-      const bool xin = (x > 0 && x < nx - 1), yin = (y > 0 && y < ny - 1);
-      ux = xin ? (xr.u - xl.u) / P.two_dx : (xr.u - xl.u) / P.dx;
-      vx = xin ? (xr.v - xl.v) / P.two_dx : (xr.v - xl.v) / P.dx;
-      uy = yin ? (yr.u - yl.u) / P.two_dy : (yr.u - yl.u) / P.dy;
-      vy = yin ? (yr.v - yl.v) / P.two_dy : (yr.v - yl.v) / P.dy;
-    }
-    const T div = ux + vy;
-    v[R_DIV2] += (double)div * (double)div;
-    if (P.shear_on) {
-      const T w = vx - uy;
-      v[R_VORT2] += (double)w * (double)w;
-    }
-    if (P.cool_on || DRAG) {
-      T a, b, cc, d;
-      load4(q, N, idx, a, b, cc, d);
-      if (P.cool_on) v[R_PSUM] += (double)prim(P, a, b, cc, d).p;
-      if (DRAG) {
-#pragma unroll
-        for (int m = 0; m < 12; ++m) {
-          const bool on = m < P.ndrag;
-          const int kx = on ? P.drag_modes[2 * m] : 0, ky = on ? P.drag_modes[2 * m + 1] : 0;
-          const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
-          const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + y) * 2;
-          const double cr = on ? tx[0] * ty[0] - tx[1] * ty[1] : 0.0;
-          const double si = on ? tx[1] * ty[0] + tx[0] * ty[1] : 0.0;
-          v[R_DRAG + 4 * m + 0] += (double)b * cr;
-          v[R_DRAG + 4 * m + 1] -= (double)b * si;
-          v[R_DRAG + 4 * m + 2] += (double)cc * cr;
-          v[R_DRAG + 4 * m + 3] -= (double)cc * si;
+  const Walk W = walk_setup(nx, ny, P.periodic, P.rb, 1, bid);
+  const int lane = threadIdx.x & 63;
+  auto row_of = [&](int y) { return P.periodic ? (y < 0 ? y + ny : (y >= ny ? y - ny : y)) : clampi(y, 0, ny - 1); };
+  if (W.live) {
+    Prim<T> pv = prim_at(P, q, N, row_of(W.ya - 1) * nx + W.xl, enforce_first);
+    Prim<T> cu = prim_at(P, q, N, W.ya * nx + W.xl, enforce_first);
+    Prim<T> nn = prim_at(P, q, N, row_of(W.ya + 1) * nx + W.xl, enforce_first);
+    for (int y = W.ya; y < W.yb; ++y) {
+      // next row's loads in flight while this row is reduced
+      T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0);
+      if (y + 1 < W.yb) load4(q, N, row_of(y + 2) * nx + W.xl, n0, n1, n2, n3);
+      const T ul = lane_from(cu.u, lane - 1), ur = lane_from(cu.u, lane + 1);
+      const T vl = lane_from(cu.v, lane - 1), vr = lane_from(cu.v, lane + 1);
+      if (W.own) {
+        const int x = W.x;
+        const T cs = sqrt((P.gamma * cu.p) / cu.rho);
+        v[R_MAXR] = fmax(v[R_MAXR], (double)absT(cu.rho));
+        v[R_MAXP] = fmax(v[R_MAXP], (double)absT(cu.p));
+        const T e = cu.p / (cu.rho * P.gm1);
+        v[R_MAXE] = fmax(v[R_MAXE], (double)absT(e));
+        v[R_AX] = fmax(v[R_AX], (double)(absT(cu.u) + cs));
+        v[R_AY] = fmax(v[R_AY], (double)(absT(cu.v) + cs));
+        T ux, uy, vx, vy;
+        if (P.periodic) {
+          ux = (ur - ul) / P.two_dx;
+          vx = (vr - vl) / P.two_dx;
+          uy = (nn.u - pv.u) / P.two_dy;
+          vy = (nn.v - pv.v) / P.two_dy;
+        } else {  // OOP/problems/riemann_config3.py:268-275 (clamped neighbours)
+          const bool xin = (x > 0 && x < nx - 1), yin = (y > 0 && y < ny - 1);
+          ux = xin ? (ur - ul) / P.two_dx : (ur - ul) / P.dx;
+          vx = xin ? (vr - vl) / P.two_dx : (vr - vl) / P.dx;
+          uy = yin ? (nn.u - pv.u) / P.two_dy : (nn.u - pv.u) / P.dy;
+          vy = yin ? (nn.v - pv.v) / P.two_dy : (nn.v - pv.v) / P.dy;
         }
+        const T div = ux + vy;
+        v[R_DIV2] += (double)div * (double)div;
+        if (P.shear_on) {
+          const T w = vx - uy;
+          v[R_VORT2] += (double)w * (double)w;
+        }
+        if (P.cool_on || DRAG) {
+          T a, b, c, d;
+          load4(q, N, y * nx + x, a, b, c, d);
+          if (P.cool_on) v[R_PSUM] += (double)prim(P, a, b, c, d).p;
+          if (DRAG) {
+            const double fb = (double)b, fc = (double)c;
+            bs[0][0] += fb;
+            bs[1][0] += fc;
+#pragma unroll
+            for (int k = 1; k <= 2; ++k) {
+              if (k <= P.dky) {
+                const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + y) * 2;
+                const double cy = ty[0], sy = ty[1];
+                bs[0][2 * k - 1] += fb * cy;
+                bs[0][2 * k] += fb * sy;
+                bs[1][2 * k - 1] += fc * cy;
+                bs[1][2 * k] += fc * sy;
+              }
+            }
+          }
+        }
+      }
+      pv = cu;
+      cu = nn;
+      if (y + 1 < W.yb) {
+        if (enforce_first) enforce(P, n0, n1, n2, n3);
+        nn = prim(P, n0, n1, n2, n3);
       }
     }
   }
-  block_reduce_write<NR, 0x1Full>(v, part + (size_t)bid * NR);
+  block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * NR);
+  if (DRAG) {
+    // Mode (kx, ky), field f: sum over the wave's columns of
+    // exp(-i kx x) * sum_y f exp(-i ky y).  The lanes' column sums go to LDS;
+    // lane j < 2*ndrag then forms output (mode j/2, field j%2) over 64 columns.
+    __shared__ double cs_[kWaves][10][64];
+    __shared__ double ms_[kWaves][48];
+    const int wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) cs_[wid][f * 5 + j][lane] = W.own ? bs[f][j] : 0.0;
+    __syncthreads();
+    if (lane < 2 * P.ndrag) {
+      const int m = lane >> 1, f = lane & 1;
+      const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
+      const int ka = ky < 0 ? -ky : ky;
+      const int sc = f * 5 + (ka == 0 ? 0 : 2 * ka - 1);
+      double re = 0.0, im = 0.0;
+      const int x0 = W.x - lane;  // column of lane 0
+      for (int l = 0; l < 64; ++l) {
+        const int x = x0 + l;
+        if (x < 0 || x >= nx) continue;
+        const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
+        const double cx = tx[0], sx = tx[1];
+        const double br = cs_[wid][sc][l];
+        const double bi = ka == 0 ? 0.0 : (ky > 0 ? -cs_[wid][sc + 1][l] : cs_[wid][sc + 1][l]);
+        re += cx * br + sx * bi;
+        im += cx * bi - sx * br;
+      }
+      ms_[wid][4 * m + 2 * f] = re;
+      ms_[wid][4 * m + 2 * f + 1] = im;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 4 * P.ndrag; j += blockDim.x) {
+      double a = ms_[0][j];
+      for (int w = 1; w < kWaves; ++w) a += ms_[w][j];
+      part[(size_t)bid * NR + R_DRAG + j] = a;
+    }
+  }
 }
 
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* __restrict__ q,
                                                          int enforce_first, double* __restrict__ part) {
-  stage_reduce_body<T, DRAG>(P, q, enforce_first, part, blockIdx.x, gridDim.x);
+  stage_reduce_body<T, DRAG>(P, q, enforce_first, part, blockIdx.x);
 }
 
 // One block: reduce the partial rows in a fixed order and write the stage
@@ -131,6 +241,44 @@ __device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const do
   }
   if (DRAG)
     for (int k = threadIdx.x; k < 4 * P.ndrag; k += blockDim.x) scal[S_DRAG + k] = r[R_DRAG + k] / N;
+}
+
+// Per-row table of the drag low-pass (one thread per row and field): for row
+// y and field f (rho u, rho v), G(kx) = sum over modes with that kx of
+// A(m)*exp(i ky y), A = the projection / N, and drows[y][f] = {Re G(0),
+// Re(G(1)+G(-1)), Im(G(1)-G(-1)), Re(G(2)+G(-2)), Im(G(2)-G(-2))}, so the
+// low-pass field at (x, y) is H0 + sum_k (Hc_k cos(k x) - Hs_k sin(k x)) (the
+// real part of the inverse transform, OOP/hit2d.py:366-367).
+template <class T>
+__device__ __forceinline__ void drows_body(const Params<T>& P, const double* __restrict__ scal,
+                                           double* __restrict__ drows, int t) {
+  if (t >= 2 * P.ny) return;
+  const int y = t >> 1, f = t & 1;
+  double h[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int m = 0; m < P.ndrag; ++m) {
+    const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
+    const double ar = scal[S_DRAG + 4 * m + 2 * f], ai = scal[S_DRAG + 4 * m + 2 * f + 1];
+    const double* ty = P.twy + ((size_t)(ky + P.kmy) * P.ny + y) * 2;
+    const double gr = ar * ty[0] - ai * ty[1], gi = ar * ty[1] + ai * ty[0];
+    const double gs = kx > 0 ? gi : -gi;
+    if (kx == 0) {
+      h[0] += gr;
+    } else if (kx == 1 || kx == -1) {
+      h[1] += gr;
+      h[2] += gs;
+    } else {
+      h[3] += gr;
+      h[4] += gs;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 5; ++j) drows[((size_t)y * 2 + f) * 5 + j] = h[j];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_drows(Params<T> P, const double* __restrict__ scal,
+                                                  double* __restrict__ drows) {
+  drows_body<T>(P, scal, drows, blockIdx.x * kBlock + threadIdx.x);
 }
 
 template <class T, bool DRAG>
@@ -688,8 +836,9 @@ template <class T>
 __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
                                                    const T* __restrict__ qs, const T* __restrict__ dfx,
                                                    const T* __restrict__ dfyT, const T* __restrict__ fx,
-                                                   const T* __restrict__ fy, const double* __restrict__ scal, T dt,
-                                                   T* __restrict__ out, int bx, int by) {
+                                                   const T* __restrict__ fy, const double* __restrict__ scal,
+                                                   const double* __restrict__ drows, T dt, T* __restrict__ out,
+                                                   int bx, int by) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
   if (x >= nx || y >= ny) return;
@@ -731,17 +880,14 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       Lv[2] = Lv[2] + st.rho * f2;
       Lv[3] = Lv[3] + st.rho * (st.u * f1 + st.v * f2);
     }
-    if (P.drag_on) {  // OOP/hit2d.py:370-399
-      double lx = 0.0, ly = 0.0;
-      for (int m = 0; m < P.ndrag; ++m) {
-        const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
-        const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + cx) * 2;
-        const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + cy) * 2;
-        const double cr = tx[0] * ty[0] - tx[1] * ty[1];
-        const double si = tx[1] * ty[0] + tx[0] * ty[1];
-        const double* A = scal + S_DRAG + 4 * m;
-        lx += A[0] * cr - A[1] * si;
-        ly += A[2] * cr - A[3] * si;
+    if (P.drag_on) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
+      const double* H = drows + (size_t)cy * 10;
+      double lx = H[0], ly = H[5];
+      for (int k = 1; k <= P.dkx; ++k) {
+        const double* tx = P.twx + ((size_t)(k + P.kmx) * nx + cx) * 2;
+        const double c = tx[0], sn = tx[1];
+        lx += H[2 * k - 1] * c - H[2 * k] * sn;
+        ly += H[5 + 2 * k - 1] * c - H[5 + 2 * k] * sn;
       }
       const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
       Lv[1] = Lv[1] + d1;
@@ -771,9 +917,9 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int stage, const T* __restrict__ q0,
                                                    const T* __restrict__ qs, const T* __restrict__ dfx,
                                                    const T* __restrict__ dfyT, const T* __restrict__ fx,
-                                                   const T* __restrict__ fy, const double* __restrict__ scal, T dt,
-                                                   T* __restrict__ out) {
-  update_body<T>(P, mode, stage, q0, qs, dfx, dfyT, fx, fy, scal, dt, out, blockIdx.x, blockIdx.y);
+                                                   const T* __restrict__ fy, const double* __restrict__ scal,
+                                                   const double* __restrict__ drows, T dt, T* __restrict__ out) {
+  update_body<T>(P, mode, stage, q0, qs, dfx, dfyT, fx, fy, scal, drows, dt, out, blockIdx.x, blockIdx.y);
 }
 
 // ================================================================ hyperviscosity
@@ -866,74 +1012,161 @@ __global__ void __launch_bounds__(kBlock) k_enforce(Params<T> P, const T* __rest
 }
 
 // ================================================================ forcing
-// Forward DFT of the noise at each shell mode (OOP/forcing.py:78-79 keeps only
-// these coefficients of fft2).  PPT points per thread held in registers; one
-// block reduction per mode.
-constexpr int kPPT = 8;
+constexpr int kFK = 12;  // largest |kx|, |ky| index of a forcing mode
+
+// fft2(noise) at the shell modes (OOP/forcing.py:78-79): per block and mode
+// the partial sum of noise*exp(-i(kx x + ky y)), re and im.  Column walk
+// (no halo, P.rbp rows per wave): each lane sums noise*cos/sin(ky y) for
+// ky = 0..fky over its rows; the lanes' sums go to LDS (dynamic, 2*fky+1
+// rows of 64 per wave) and lane j then forms mode j over the wave's columns
+// with exp(-i kx x).
 template <class T>
 __device__ __forceinline__ void fproject_body(const Params<T>& P, const double* __restrict__ noise,
-                                                     double* __restrict__ part, int bid, int nblk) {
-  const int nx = P.nx, ny = P.ny, N = nx * ny;
-  double val[kPPT];
-  int xs[kPPT], ys[kPPT];
+                                              double* __restrict__ part, int bid) {
+  extern __shared__ double dsh[];
+  __shared__ double sh[kWaves][2 * HITX_MAX_MODES];
+  const int nx = P.nx, ny = P.ny;
+  const Walk W = walk_setup(nx, ny, 1, P.rbp, 0, bid);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int NS = 2 * P.fky + 1;
+  double ac[kFK + 1], as[kFK + 1];
 #pragma unroll
-  for (int j = 0; j < kPPT; ++j) {
-    const int idx = bid * kBlock * kPPT + j * kBlock + threadIdx.x;
-    if (idx < N) {
-      val[j] = noise[idx];
-      ys[j] = idx / nx;
-      xs[j] = idx - ys[j] * nx;
-    } else {
-      val[j] = 0.0; ys[j] = 0; xs[j] = 0;
+  for (int k = 0; k <= kFK; ++k) {
+    ac[k] = 0.0;
+    as[k] = 0.0;
+  }
+  if (W.live) {
+#pragma unroll 4
+    for (int y = W.ya; y < W.yb; ++y) {
+      const double n = W.own ? noise[(size_t)y * nx + W.x] : 0.0;
+      ac[0] += n;
+#pragma unroll
+      for (int k = 1; k <= kFK; ++k) {
+        if (k <= P.fky) {
+          const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + y) * 2;
+          ac[k] += n * ty[0];
+          as[k] += n * ty[1];
+        }
+      }
     }
   }
-  for (int m = 0; m < P.nforce; ++m) {
-    const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
-    double v[2] = {0.0, 0.0};
+  double* L = dsh + (size_t)wid * NS * 64;
+  L[lane] = ac[0];
 #pragma unroll
-    for (int j = 0; j < kPPT; ++j) {
-      const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + xs[j]) * 2;
-      const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + ys[j]) * 2;
-      const double cr = tx[0] * ty[0] - tx[1] * ty[1];
-      const double si = tx[1] * ty[0] + tx[0] * ty[1];
-      v[0] += val[j] * cr;
-      v[1] -= val[j] * si;
+  for (int k = 1; k <= kFK; ++k) {
+    if (k <= P.fky) {
+      L[(2 * k - 1) * 64 + lane] = ac[k];
+      L[2 * k * 64 + lane] = as[k];
     }
-    block_reduce_write<2, 0>(v, part + ((size_t)bid * P.nforce + m) * 2);
-    __syncthreads();
+  }
+  __syncthreads();
+  const int x0 = W.x - lane;
+  for (int m = lane; m < P.nforce; m += 64) {
+    const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
+    const int ka = ky < 0 ? -ky : ky;
+    double re = 0.0, im = 0.0;
+    if (W.live) {
+      for (int l = 0; l < 64; ++l) {
+        const int x = x0 + l;
+        if (x >= nx) break;
+        const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
+        const double cx = tx[0], sx = tx[1];
+        const double br = ka == 0 ? L[l] : L[(2 * ka - 1) * 64 + l];
+        const double bi = ka == 0 ? 0.0 : (ky > 0 ? -L[2 * ka * 64 + l] : L[2 * ka * 64 + l]);
+        re += cx * br + sx * bi;
+        im += cx * bi - sx * br;
+      }
+    }
+    sh[wid][2 * m] = re;
+    sh[wid][2 * m + 1] = im;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < 2 * P.nforce; j += blockDim.x) {
+    double a = sh[0][j];
+    for (int w = 1; w < kWaves; ++w) a += sh[w][j];
+    part[(size_t)bid * 2 * P.nforce + j] = a;
   }
 }
 
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_fproject(Params<T> P, const double* __restrict__ noise,
                                                      double* __restrict__ part) {
-  fproject_body<T>(P, noise, part, blockIdx.x, gridDim.x);
+  fproject_body<T>(P, noise, part, blockIdx.x);
+}
+
+// Per-row table of the forcing synthesis (one block): for row y and field f
+// (fx, fy), G(kx) = sum over the modes with that kx of fhat_f(m)*exp(i ky y),
+// frows[y][f] = {Re G(0), then Re(G(k)+G(-k)), Im(G(k)-G(-k)) for k = 1..fkx},
+// so that the real inverse DFT at (x, y) is H0 + sum_k (Hc_k cos(k x) -
+// Hs_k sin(k x)) (OOP/forcing.py:84-85, before the 1/N).
+template <class T>
+__device__ __forceinline__ void frows_body(const Params<T>& P, const double* __restrict__ fhat,
+                                           double* __restrict__ frows, int t) {
+  const int kx1 = P.fkx + 1, FR = 2 * P.fkx + 1, nf = P.nforce;
+  if (t >= P.ny * kx1) return;
+  const int y = t / kx1, k = t - y * kx1;
+  double hr[2] = {0.0, 0.0}, hs[2] = {0.0, 0.0};
+  for (int m = 0; m < nf; ++m) {
+    const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
+    if (kx != k && kx != -k) continue;
+    const double* ty = P.twy + ((size_t)(ky + P.kmy) * P.ny + y) * 2;
+    const double cy = ty[0], sy = ty[1];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const double fr = fhat[f * 2 * nf + 2 * m], fi = fhat[f * 2 * nf + 2 * m + 1];
+      const double gr = fr * cy - fi * sy, gi = fr * sy + fi * cy;
+      hr[f] += gr;
+      hs[f] += kx > 0 ? gi : -gi;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    double* o = frows + ((size_t)y * 2 + f) * FR;
+    if (k == 0) {
+      o[0] = hr[f];
+    } else {
+      o[2 * k - 1] = hr[f];
+      o[2 * k] = hs[f];
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_frows(Params<T> P, const double* __restrict__ fhat,
+                                                  double* __restrict__ frows) {
+  frows_body<T>(P, fhat, frows, blockIdx.x * kBlock + threadIdx.x);
+}
+
+// Real inverse DFT of the shell coefficients at one node from the row table.
+template <class T>
+__device__ __forceinline__ void fsynth_at(const Params<T>& P, const double* __restrict__ frows, int y, int x,
+                                          double& ax, double& ay) {
+  const int FR = 2 * P.fkx + 1;
+  const double* H = frows + (size_t)y * 2 * FR;
+  ax = H[0];
+  ay = H[FR];
+  for (int k = 1; k <= P.fkx; ++k) {
+    const double* tx = P.twx + ((size_t)(k + P.kmx) * P.nx + x) * 2;
+    const double c = tx[0], sn = tx[1];
+    ax += H[2 * k - 1] * c - H[2 * k] * sn;
+    ay += H[FR + 2 * k - 1] * c - H[FR + 2 * k] * sn;
+  }
 }
 
 // fx, fy = real inverse DFT of (fx_hat, fy_hat) on the shell modes
 // (OOP/forcing.py:82-85), plus sum(fx^2+fy^2) and sum(rho*(fx*u+fy*v)).
 template <class T>
-__device__ __forceinline__ void fsynth_body(const Params<T>& P, const double* __restrict__ fhat, T* __restrict__ fxo,
-                                                   T* __restrict__ fyo, const T* __restrict__ rho,
-                                                   const T* __restrict__ u, const T* __restrict__ v,
-                                                   double* __restrict__ part, int bid, int nblk) {
+__device__ __forceinline__ void fsynth_body(const Params<T>& P, const double* __restrict__ frows, T* __restrict__ fxo,
+                                            T* __restrict__ fyo, const T* __restrict__ rho,
+                                            const T* __restrict__ u, const T* __restrict__ v,
+                                            double* __restrict__ part, int bid, int nblk) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const double invN = 1.0 / ((double)nx * (double)ny);
   double acc[2] = {0.0, 0.0};
   for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
     const int y = idx / nx, x = idx - y * nx;
-    double ax = 0.0, ay = 0.0;
-    for (int m = 0; m < P.nforce; ++m) {
-      const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
-      const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
-      const double* ty = P.twy + ((size_t)(ky + P.kmy) * ny + y) * 2;
-      const double cr = tx[0] * ty[0] - tx[1] * ty[1];
-      const double si = tx[1] * ty[0] + tx[0] * ty[1];
-      const double* fxm = fhat + 2 * m;
-      const double* fym = fhat + 2 * P.nforce + 2 * m;
-      ax += fxm[0] * cr - fxm[1] * si;
-      ay += fym[0] * cr - fym[1] * si;
-    }
+    double ax, ay;
+    fsynth_at(P, frows, y, x, ax, ay);
     const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
     fxo[idx] = fxv;
     fyo[idx] = fyv;
@@ -944,11 +1177,11 @@ __device__ __forceinline__ void fsynth_body(const Params<T>& P, const double* __
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_fsynth(Params<T> P, const double* __restrict__ fhat, T* __restrict__ fxo,
+__global__ void __launch_bounds__(kBlock) k_fsynth(Params<T> P, const double* __restrict__ frows, T* __restrict__ fxo,
                                                    T* __restrict__ fyo, const T* __restrict__ rho,
                                                    const T* __restrict__ u, const T* __restrict__ v,
                                                    double* __restrict__ part) {
-  fsynth_body<T>(P, fhat, fxo, fyo, rho, u, v, part, blockIdx.x, gridDim.x);
+  fsynth_body<T>(P, frows, fxo, fyo, rho, u, v, part, blockIdx.x, gridDim.x);
 }
 
 // fx = (fx*scale)*alpha, fy likewise (OOP/forcing.py:91-112), and the

@@ -66,8 +66,8 @@ class IsotropicShellOUForcing2D:
         self.modes, self._kx, self._ky = shell_modes(self.domain, self.k_min, self.k_max)
         if len(self.modes) == 0:
             raise ValueError("forcing shell contains no Fourier modes on this grid")
-        if np.abs(self.modes).max() > 16 or len(self.modes) > 116:
-            raise ValueError("forcing shell too wide for the device projection (|k index| <= 16)")
+        if np.abs(self.modes).max() > 12 or len(self.modes) > 116:
+            raise ValueError("forcing shell too wide for the device projection (|k index| <= 12)")
         self._phi = np.zeros(len(self.modes), dtype=np.complex128)
         self._rng = DeviceNormalStream(self.seed)
         self._noise = None

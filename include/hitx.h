@@ -214,6 +214,12 @@ int hitx_batch_destroy(hitx_batch* batch);
 size_t hitx_batch_workspace_bytes(const hitx_batch* batch);
 /* Carve the instances' buffers from ws and upload the instance table (blocking). */
 int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
+/* Tile heights of the column-walk kernels for every instance (no reference
+ * counterpart; the batch autotuner's knob): rb rows per wave of the stage
+ * reductions (2..64), rbp rows per wave of the forcing projection (8..256).
+ * Changes the summation order of those reductions only.  Blocking when bound
+ * (re-uploads the instance table); no work may be in flight on the batch. */
+int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp);
 /* Device pointer of one of instance i's buffers inside the bound workspace. */
 int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
 /* Fixed dt of every instance from its current state (Riemann loop). */
