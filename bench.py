@@ -1,5 +1,13 @@
 """Throughput of the fused RHS + SSP-RK3 step on one MI355X.
 
+Headline (value): the single-instance FP64 Riemann workload below.  The same
+JSON line also carries the forced-HIT measurements ("hit"): production HIT
+512x512 FP64 as one instance (through the Python operator API with its host
+control, and on device-side control replayed as a HIP graph), and the
+batched FP32 form: 64 instances, 16 each at 128^2, 256^2, 384^2 and 512^2,
+seeds 0-63, Mach (0.1, 0.25, 0.4)[seed % 3], production settings otherwise,
+autotuned column-walk tiles, steps replayed as a HIP graph.
+
 Default workload: Lax-Liu Riemann configuration 3, 512x512, FP64, hybrid
 compact/WENO7 scheme, CFL 0.4, T 0.3 (fixed dt, 1343 steps for the full
 solve), sensor width 4, jump threshold 0.025, shear term off, compact-node
@@ -32,6 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # reading q and writing dF/dx, one y-pass reading q, dF/dx, q0 and writing q,
 # the uint8 mask written and read once; plus the hyperviscosity pass).
 STEP_BYTES = {"fp64": 614.0, "fp32": 310.0}
+# forced HIT (forcing, drag, cooling, viscous terms; hyperviscosity every 5 steps)
+HIT_STEP_BYTES = {"fp64": 635.0, "fp32": 320.0}
 
 
 def parse():
@@ -44,6 +54,8 @@ def parse():
     ap.add_argument("--dtype", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the host oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-hit", action="store_true", help="skip the forced-HIT measurements")
+    ap.add_argument("--hit-steps", type=int, default=60)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run 3 plain device copies of this many MiB first (scale for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -67,6 +79,89 @@ def cpu_baseline(nx, seconds):
     return {"value": round(nx * nx * n / el / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"NumPy oracle, Riemann C3 {nx}x{nx} FP64, {n} RK3 steps (+HV) after 1 warm-up step, "
                       f"{el:.1f} s on 1 host core"}
+
+
+def _timed(fn, k):
+    import torch
+
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(k):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return max(time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3)
+
+
+def hit_configs_batched(dtype="fp32"):
+    from hit_solver_amd.hit2d import HIT2DConfig
+
+    out, seed = [], 0
+    for n in (128, 256, 384, 512):
+        for _ in range(16):
+            mt = (0.1, 0.25, 0.4)[seed % 3]
+            out.append(HIT2DConfig.production_512(nx=n, ny=n, forcing_seed=seed, target_mach=mt,
+                                                  mach_control_target=mt, dtype=dtype, tfinal=1e9))
+            seed += 1
+    return out
+
+
+def hit_measurements(steps):
+    """Forced HIT: plain single-instance FP64 (API path and device-control
+    graph) and the batched FP32 form."""
+    import torch
+
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig, HITSolver
+
+    res = {}
+    cfg = HIT2DConfig.production_512(dtype="fp64", tfinal=1e9)
+    s = HITSolver(cfg)
+    for _ in range(5):
+        s.step()
+    k = max(10, steps // 3)
+    T = _timed(s.step, k)
+    res["hit512_fp64_api"] = {"value": round(512 * 512 * k / T / 1e6, 3), "unit": "Mcell-updates/s",
+                              "ms_per_step": round(T / k * 1e3, 4),
+                              "form": "HITSolver.step: Python operator API, host-side dt/Mach/forcing control"}
+    del s
+    b = BatchRunner([cfg])
+    for _ in range(5):
+        b.step()
+    b.capture(10)
+    b.replay(1)
+    T = _timed(lambda: b.replay(1), max(1, steps // 10))
+    n = max(1, steps // 10) * 10
+    plain = 512 * 512 * n / T / 1e6
+    res["hit512_fp64_plain"] = {"value": round(plain, 3), "unit": "Mcell-updates/s", "ms_per_step": round(T / n * 1e3, 4),
+                                "form": "one instance, device-side control, 10 steps per HIP graph"}
+    del b
+    torch.cuda.empty_cache()
+    cs = hit_configs_batched("fp32")
+    cells = sum(c.nx * c.ny for c in cs)
+    b = BatchRunner(cs)
+    for _ in range(2):
+        b.step()
+    best, _times = b.autotune()
+    b.capture(5)
+    b.replay(1)
+    reps = max(1, steps // 5)
+    T = _timed(lambda: b.replay(1), reps)
+    n = reps * 5
+    v = cells * n / T / 1e6
+    ach = v * 1e6 * HIT_STEP_BYTES["fp32"] / 1e9
+    res["hit_batched_fp32"] = {
+        "value": round(v, 3), "unit": "Mcell-updates/s", "ms_per_step": round(T / n * 1e3, 4),
+        "instances": len(cs), "cells": cells, "tiles": {"rb": best[0], "rbp": best[1]},
+        "vs_hit512_fp64_plain": round(v / plain, 3),
+        "step_roofline": {"alg_bytes_per_cell_update": HIT_STEP_BYTES["fp32"], "achieved_GBs": round(ach, 1),
+                          "frac": round(ach / HBM_PEAK_GBS, 4)},
+        "form": "64 instances (16 each 128^2..512^2), FP32 state, FP64 reductions, 5 steps per HIP graph"}
+    del b
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -148,6 +243,10 @@ def main():
                           "achieved_GBs": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9, 1),
                           "frac": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9 / HBM_PEAK_GBS, 4)},
     }
+    if not args.no_hit:
+        del s
+        torch.cuda.empty_cache()
+        out["hit"] = hit_measurements(args.hit_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nx, args.cpu_seconds)
     print(json.dumps(out), flush=True)

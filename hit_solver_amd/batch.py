@@ -187,11 +187,34 @@ class BatchRunner:
         """One step of every active instance (asynchronous, no host work)."""
         check(lib().hitx_batch_step(self.handle, self._stream()))
 
+    def capture(self, steps: int = 1):
+        """Record `steps` batch steps as one HIP graph (the step has no host
+        work: dt, control and noise are on the device).  Replaying it advances
+        every active instance; finished instances idle through it."""
+        torch.cuda.synchronize()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(steps):
+                self.step()
+        torch.cuda.synchronize()
+        self._graph, self._graph_steps = g, steps
+        return g
+
+    def replay(self, n: int = 1):
+        """Replay the captured graph n times (n * captured steps)."""
+        if getattr(self, "_graph", None) is None:
+            raise RuntimeError("capture() first")
+        for _ in range(n):
+            self._graph.replay()
+
     def tune(self, rb: int, rbp: int):
         """Column-walk tile heights of every instance (rows per wave of the
         stage reductions and of the forcing projection)."""
         torch.cuda.synchronize()
         check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp)))
+        self._graph = None  # grid sizes changed
 
     def autotune(self, candidates=None, steps: int = 3):
         """Time a few steps per candidate (rb, rbp) on a copy of the batch's

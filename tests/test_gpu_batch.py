@@ -133,3 +133,31 @@ def test_hit_batch_tuned_tiles_agree():
     for i in range(2):
         for k in range(4):
             assert rel_l2(host(b.q[i])[k], host(a.q[i])[k]) <= 1e-12, (i, k)
+
+
+def test_batch_graph_replay_matches_eager():
+    """A captured batch step replayed k times equals k eager steps bit for bit
+    (HIT with device noise and control, and Riemann), with eager work and
+    synchronisations between the replays."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannConfig6
+
+    kw = dict(cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0, viscosity=7.5e-4,
+              mach_control=True, large_scale_drag=0.10, cooling_time=5.0, tfinal=1.0, hyperviscosity_interval=2)
+    hit = [HIT2DConfig(nx=64, ny=64, target_mach=0.25, forcing_seed=5, **kw),
+           HIT2DConfig(nx=48, ny=40, target_mach=0.1, forcing_seed=6, **kw)]
+    rie = [RiemannConfig3(nx=64, ny=64, shear_threshold=None, progress_every=10**9),
+           RiemannConfig6(nx=48, ny=56, shear_threshold=None, progress_every=10**9)]
+    for cfgs in (hit, rie):
+        a, b = BatchRunner(cfgs), BatchRunner(cfgs)
+        b.capture(2)
+        for _ in range(3):
+            a.step()
+            a.step()
+            b.replay(1)
+            torch.cuda.synchronize()
+            _ = torch.ones(8, device="cuda").sum().item()  # eager work between replays
+            for i in range(len(cfgs)):
+                np.testing.assert_array_equal(host(b.q[i]), host(a.q[i]))
+            np.testing.assert_array_equal(b.control(), a.control())
