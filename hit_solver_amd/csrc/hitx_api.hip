@@ -11,6 +11,7 @@
 
 #include "../../include/hitx.h"
 #include "hitx_batch.h"
+#include "hitx_line.h"
 
 using namespace hitx;
 
@@ -123,6 +124,7 @@ struct hitx_plan {
   size_t sensor_lds = 0;
   int nblk_red = 0, nblk_proj = 0, nblk_stage = 0, nblk_sr = 0;
   int rb = 2, rbp = 4;                // rows per wave of the column walks
+  int line_form = 0;                  // 0: auto, 1: block form, 2: wave form where the line allows it
   int dkx = 0, dky = 0, fkx = 0, fky = 0;
   size_t off_drows = 0, off_frows = 0;
   size_t line_lds_x = 0, line_lds_y = 0;
@@ -277,28 +279,54 @@ int tmark(hitx_plan* pl, hipStream_t s) {
 }
 
 // both sweeps of the operator on state qs: dfx and dfyT in the workspace
+size_t wave_slice_bytes(int n, int esize) { return (line_wave_lds(n, esize) + 15) & ~size_t(15); }
+// The wave form needs many lines to fill the device (one wavefront each);
+// with few lines the block form's 8 wavefronts per line finish sooner.
+constexpr int kWaveFormMinLines = 2048;
+bool use_wave_form(int form, int n, long long lines) {
+  if (!line_wave_ok(n) || form == 1) return false;
+  return form == 2 || lines >= kWaveFormMinLines;
+}
+bool use_wave(const hitx_plan* pl, int n) {
+  const int nl = n == pl->d.nx ? pl->d.ny : pl->d.nx;
+  return use_wave_form(pl->line_form, n, nl);
+}
+
+// One line sweep along axis 0 (lines of nx, ny of them) or axis 1 (the
+// transposed state: lines of ny, nx of them).
+template <class T>
+void launch_line(const hitx_plan* pl, const Params<T>& P, int axis, int enforced, const T* in, const uint8_t* mask,
+                 const double* scal, T* out, hipStream_t s) {
+  const int n = axis == 0 ? pl->d.nx : pl->d.ny, nl = axis == 0 ? pl->d.ny : pl->d.nx;
+  const LineK<T>& K = linek<T>(pl, axis);
+  if (use_wave(pl, n)) {
+    const int sb = (int)wave_slice_bytes(n, pl->esize);
+    const int nb = (nl + kLineWaves - 1) / kLineWaves;
+    if (P.periodic)
+      k_line_wave<T, true><<<nb, 64 * kLineWaves, kLineWaves * sb, s>>>(P, K, axis, enforced, in, mask, scal, out,
+                                                                          nl, sb);
+    else
+      k_line_wave<T, false><<<nb, 64 * kLineWaves, kLineWaves * sb, s>>>(P, K, axis, enforced, in, mask, scal, out,
+                                                                           nl, sb);
+    return;
+  }
+  const size_t lds = axis == 0 ? pl->line_lds_x : pl->line_lds_y;
+  if (P.periodic)
+    k_line<T, true><<<nl, kLineBlock, lds, s>>>(P, K, axis, enforced, in, mask, scal, out);
+  else
+    k_line<T, false><<<nl, kLineBlock, lds, s>>>(P, K, axis, enforced, in, mask, scal, out);
+}
+
 template <class T>
 int run_sweeps(hitx_plan* pl, const T* qs, const WS& w, hipStream_t s) {
   int rc = run_sensor<T>(pl, qs, 1, w.mask, static_cast<T*>(w.qT), w, s);
   if (rc) return rc;
   const Params<T> P = make_params<T>(pl);
   if ((rc = tmark(pl, s))) return rc;
-  if (P.periodic)
-    k_line<T, true><<<pl->d.ny, kLineBlock, pl->line_lds_x, s>>>(P, linek<T>(pl, 0), 0, 0, qs, w.mask, w.scal,
-                                                                 static_cast<T*>(w.dfx));
-  else
-    k_line<T, false><<<pl->d.ny, kLineBlock, pl->line_lds_x, s>>>(P, linek<T>(pl, 0), 0, 0, qs, w.mask, w.scal,
-                                                                  static_cast<T*>(w.dfx));
+  launch_line<T>(pl, P, 0, 0, qs, w.mask, w.scal, static_cast<T*>(w.dfx), s);
   if ((rc = tmark(pl, s))) return rc;
   if ((rc = tmark(pl, s))) return rc;
-  if (P.periodic)
-    k_line<T, true><<<pl->d.nx, kLineBlock, pl->line_lds_y, s>>>(P, linek<T>(pl, 1), 1, 1,
-                                                                 static_cast<const T*>(w.qT), w.maskT, w.scal,
-                                                                 static_cast<T*>(w.dfyT));
-  else
-    k_line<T, false><<<pl->d.nx, kLineBlock, pl->line_lds_y, s>>>(P, linek<T>(pl, 1), 1, 1,
-                                                                  static_cast<const T*>(w.qT), w.maskT, w.scal,
-                                                                  static_cast<T*>(w.dfyT));
+  launch_line<T>(pl, P, 1, 1, static_cast<const T*>(w.qT), w.maskT, w.scal, static_cast<T*>(w.dfyT), s);
   if ((rc = tmark(pl, s))) return rc;
   LAUNCHCHK();
   return HITX_OK;
@@ -700,6 +728,13 @@ int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* w
   return HITX_OK;
 }
 
+int hitx_set_line_form(hitx_plan* pl, int32_t form) {
+  if (!pl) return fail(HITX_EINVAL, "null plan");
+  if (form < 0 || form > 2) return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block) or 2 (wave)");
+  pl->line_form = form;
+  return HITX_OK;
+}
+
 int hitx_timing_enable(hitx_plan* pl, int32_t on) {
   if (!pl) return fail(HITX_EINVAL, "null plan");
   pl->timing = on ? 1 : 0;
@@ -737,22 +772,27 @@ struct hitx_batch {
   size_t off_tab = 0, off_offs = 0, ws_bytes = 0;
   int total[OFF_KINDS] = {0};
   std::vector<int> offs;  // OFF_KINDS * (n + 1)
-  size_t line_lds = 0, sensor_lds = 0, proj_lds = 0;
+  size_t line_lds = 0, sensor_lds = 0, proj_lds = 0, wave_slice = 16;
+  int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave
+  bool wave_x = false, wave_y = false;
   void* ws = nullptr;
 };
 
 namespace {
 
-int batch_blocks(const hitx_plan* pl, int kind) {
+int batch_blocks(const hitx_batch* b, const hitx_plan* pl, int kind) {
   const int nx = pl->d.nx, ny = pl->d.ny;
+  const bool wx = b->wave_x && line_wave_ok(nx), wy = b->wave_y && line_wave_ok(ny);
   switch (kind) {
     case OFF_RED: return pl->nblk_stage;
     case OFF_SR: return pl->nblk_sr;
     case OFF_DROWS: return pl->d.drag_enabled && pl->d.n_drag_modes > 0 ? (2 * ny + kBlock - 1) / kBlock : 0;
     case OFF_FROWS: return pl->d.n_force_modes > 0 ? (ny * (pl->fkx + 1) + kBlock - 1) / kBlock : 0;
     case OFF_SENS: return ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
-    case OFF_LX: return ny;
-    case OFF_LY: return nx;
+    case OFF_LX: return wx ? 0 : ny;
+    case OFF_LY: return wy ? 0 : nx;
+    case OFF_LXW: return wx ? (ny + kLineWaves - 1) / kLineWaves : 0;
+    case OFF_LYW: return wy ? (nx + kLineWaves - 1) / kLineWaves : 0;
     case OFF_T16: return ((nx + 15) / 16) * ((ny + 15) / 16);
     case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
     case OFF_RNG:
@@ -763,12 +803,20 @@ int batch_blocks(const hitx_plan* pl, int kind) {
 
 void batch_offsets(hitx_batch* b) {
   const int n = b->n;
+  long long lx = 0, ly = 0;  // wave-capable lines per axis
+  for (const hitx_plan* pl : b->plans) {
+    if (line_wave_ok(pl->d.nx)) lx += pl->d.ny;
+    if (line_wave_ok(pl->d.ny)) ly += pl->d.nx;
+  }
+  b->wave_x = b->line_form == 2 || (b->line_form == 0 && lx >= kWaveFormMinLines);
+  b->wave_y = b->line_form == 2 || (b->line_form == 0 && ly >= kWaveFormMinLines);
+  if (b->line_form == 1) b->wave_x = b->wave_y = false;
   b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);
   for (int k = 0; k < OFF_KINDS; ++k) {
     int acc = 0;
     for (int i = 0; i < n; ++i) {
       b->offs[(size_t)k * (n + 1) + i] = acc;
-      acc += batch_blocks(b->plans[i], k);
+      acc += batch_blocks(b, b->plans[i], k);
     }
     b->offs[(size_t)k * (n + 1) + n] = acc;
     b->total[k] = acc;
@@ -846,12 +894,21 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
     LAUNCHCHK();
     return HITX_OK;
   }
-  if (b->periodic) {
-    b_line<T, true><<<b->total[OFF_LX], kLineBlock, b->line_lds, s>>>(tab, offs, n, 0, st);
-    b_line<T, true><<<b->total[OFF_LY], kLineBlock, b->line_lds, s>>>(tab, offs, n, 1, st);
-  } else {
-    b_line<T, false><<<b->total[OFF_LX], kLineBlock, b->line_lds, s>>>(tab, offs, n, 0, st);
-    b_line<T, false><<<b->total[OFF_LY], kLineBlock, b->line_lds, s>>>(tab, offs, n, 1, st);
+  const int sb = (int)b->wave_slice;
+  for (int axis = 0; axis < 2; ++axis) {
+    const int kb = axis == 0 ? OFF_LX : OFF_LY, kw = axis == 0 ? OFF_LXW : OFF_LYW;
+    if (b->total[kw] > 0) {
+      if (b->periodic)
+        b_line_wave<T, true><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
+      else
+        b_line_wave<T, false><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
+    }
+    if (b->total[kb] > 0) {
+      if (b->periodic)
+        b_line<T, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+      else
+        b_line<T, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+    }
   }
   b_update<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n, st);
   LAUNCHCHK();
@@ -942,6 +999,8 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     }
     b->plans.push_back(pl);
     b->line_lds = std::max({b->line_lds, pl->line_lds_x, pl->line_lds_y});
+    if (line_wave_ok(pl->d.nx)) b->wave_slice = std::max(b->wave_slice, wave_slice_bytes(pl->d.nx, pl->esize));
+    if (line_wave_ok(pl->d.ny)) b->wave_slice = std::max(b->wave_slice, wave_slice_bytes(pl->d.ny, pl->esize));
     b->sensor_lds = std::max(b->sensor_lds, pl->sensor_lds);
     b->proj_lds = std::max(b->proj_lds, proj_lds_bytes(pl));
   }
@@ -1028,10 +1087,12 @@ int hitx_batch_buffer(const hitx_batch* b, int32_t i, int32_t which, void** ptr)
   return HITX_OK;
 }
 
-int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp) {
+int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form) {
   if (!b) return fail(HITX_EINVAL, "null batch");
   if (rb < kTuneRbMin || rb > kTuneRbMax || rbp < kTuneRbpMin || rbp > kTuneRbpMax)
     return fail(HITX_EINVAL, "tile heights out of range (rb 2..64, rbp 8..256)");
+  if (line_form < 0 || line_form > 2) return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block) or 2 (wave)");
+  b->line_form = line_form;
   for (hitx_plan* pl : b->plans) {
     pl->rb = rb;
     pl->rbp = rbp;

@@ -10,6 +10,7 @@
 
 #include "hitx_kernels.h"
 #include "hitx_rng.h"
+#include "hitx_line.h"
 
 namespace hitx {
 
@@ -42,7 +43,10 @@ struct Inst {
 };
 
 // offsets kinds (rows of the offsets table, each ninst+1 ints)
-enum { OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_KINDS };
+enum {
+  OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_LXW, OFF_LYW,
+  OFF_KINDS
+};
 
 __device__ __forceinline__ int find_inst(const int* __restrict__ o, int n, int b) {
   int lo = 0, hi = n;  // o[0] = 0 <= b < o[n]
@@ -116,6 +120,24 @@ __global__ void __launch_bounds__(kLineBlock, 4) b_line(const Inst<T>* __restric
     line_body<T, PER>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, lb, I.P.ny);
   else
     line_body<T, PER>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, lb, I.P.nx);
+}
+
+// Wave form of the sweep: kLineWaves lines per block, one per wavefront.
+template <class T, bool PER>
+__global__ void __launch_bounds__(64 * kLineWaves) b_line_wave(const Inst<T>* __restrict__ tab,
+                                                              const int* __restrict__ offs, int ninst, int axis, int st,
+                                                              int slice_bytes) {
+  extern __shared__ __align__(16) unsigned char wsm[];
+  HITX_BATCH_INST(axis == 0 ? OFF_LXW : OFF_LYW)
+  const int w = threadIdx.x >> 6;
+  const int line = lb * kLineWaves + w;
+  const int nl = axis == 0 ? I.P.ny : I.P.nx;
+  if (line >= nl) return;
+  if (axis == 0)
+    line_wave_any<T, PER>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, line, nl,
+                          wsm + (size_t)w * slice_bytes);
+  else
+    line_wave_any<T, PER>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, line, nl, wsm + (size_t)w * slice_bytes);
 }
 
 template <class T>

@@ -788,41 +788,49 @@ __global__ void __launch_bounds__(kLineBlock, 4) k_line(Params<T> P, LineK<T> K,
 
 // ================================================================ stage update
 // Periodic Navier-Stokes terms at one node (OOP/equations.py:232-267): nested
-// second-order central differences on the 13-point diamond.
+// second-order central differences on the 13-point diamond.  U, V, Tt are
+// the node primitives u, v, p/(rho R) of a tile with a 2-node halo in LDS
+// (row stride kVS); (ly, lx) is the node's position in that tile.
+constexpr int kVS = 21;
 template <class T>
-__device__ void viscous_at(const Params<T>& P, const T* __restrict__ q, int y, int x, T out[4]) {
-  const int nx = P.nx, ny = P.ny, N = nx * ny;
-  T U[5][5], V[5][5], Tt[5][5];
-#pragma unroll
-  for (int a = -2; a <= 2; ++a)
-#pragma unroll
-    for (int b = -2; b <= 2; ++b) {
-      if ((a < 0 ? -a : a) + (b < 0 ? -b : b) > 2) continue;
-      const Prim<T> pr = prim_at(P, q, N, wrap(y + a, ny) * nx + wrap(x + b, nx), 0);
-      U[a + 2][b + 2] = pr.u;
-      V[a + 2][b + 2] = pr.v;
-      Tt[a + 2][b + 2] = pr.p / (pr.rho * P.rgas);
-    }
-  // derivatives at node (a, b) relative to the centre
-  auto ux = [&](int a, int b) { return (U[a + 2][b + 3] - U[a + 2][b + 1]) / P.two_dx; };
-  auto uy = [&](int a, int b) { return (U[a + 3][b + 2] - U[a + 1][b + 2]) / P.two_dy; };
-  auto vx = [&](int a, int b) { return (V[a + 2][b + 3] - V[a + 2][b + 1]) / P.two_dx; };
-  auto vy = [&](int a, int b) { return (V[a + 3][b + 2] - V[a + 1][b + 2]) / P.two_dy; };
+__device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, const T* V, const T* Tt, int ly, int lx,
+                                            T out[4]) {
+  auto at = [&](const T* a, int dy, int dx) { return a[(ly + dy) * kVS + lx + dx]; };
+  auto ux = [&](int a, int b) { return (at(U, a, b + 1) - at(U, a, b - 1)) / P.two_dx; };
+  auto uy = [&](int a, int b) { return (at(U, a + 1, b) - at(U, a - 1, b)) / P.two_dy; };
+  auto vx = [&](int a, int b) { return (at(V, a, b + 1) - at(V, a, b - 1)) / P.two_dx; };
+  auto vy = [&](int a, int b) { return (at(V, a + 1, b) - at(V, a - 1, b)) / P.two_dy; };
   auto txx = [&](int a, int b) { return P.two_mu * ux(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
   auto tyy = [&](int a, int b) { return P.two_mu * vy(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
   auto txy = [&](int a, int b) { return P.mu * (uy(a, b) + vx(a, b)); };
   auto ex = [&](int a, int b) {
-    const T tx = (Tt[a + 2][b + 3] - Tt[a + 2][b + 1]) / P.two_dx;
-    return (U[a + 2][b + 2] * txx(a, b) + V[a + 2][b + 2] * txy(a, b)) + P.kcond * tx;
+    const T tx = (at(Tt, a, b + 1) - at(Tt, a, b - 1)) / P.two_dx;
+    return (at(U, a, b) * txx(a, b) + at(V, a, b) * txy(a, b)) + P.kcond * tx;
   };
   auto ey = [&](int a, int b) {
-    const T ty = (Tt[a + 3][b + 2] - Tt[a + 1][b + 2]) / P.two_dy;
-    return (U[a + 2][b + 2] * txy(a, b) + V[a + 2][b + 2] * tyy(a, b)) + P.kcond * ty;
+    const T ty = (at(Tt, a + 1, b) - at(Tt, a - 1, b)) / P.two_dy;
+    return (at(U, a, b) * txy(a, b) + at(V, a, b) * tyy(a, b)) + P.kcond * ty;
   };
   out[0] = T(0);
   out[1] = (txx(0, 1) - txx(0, -1)) / P.two_dx + (txy(1, 0) - txy(-1, 0)) / P.two_dy;
   out[2] = (txy(0, 1) - txy(0, -1)) / P.two_dx + (tyy(1, 0) - tyy(-1, 0)) / P.two_dy;
   out[3] = (ex(0, 1) - ex(0, -1)) / P.two_dx + (ey(1, 0) - ey(-1, 0)) / P.two_dy;
+}
+
+// Primitives u, v, p/(rho R) of the 20x20 tile around a 16x16 block
+// (periodic wrap) into LDS.
+template <class T>
+__device__ __forceinline__ void viscous_tile(const Params<T>& P, const T* __restrict__ q, int x0, int y0, T* U, T* V,
+                                             T* Tt) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  for (int i = threadIdx.x; i < 400; i += kBlock) {
+    const int ly = i / 20, lx = i - ly * 20;
+    const int gy = wrap(y0 - 2 + ly, ny), gx = wrap(x0 - 2 + lx, nx);
+    const Prim<T> pr = prim_at(P, q, N, gy * nx + gx, 0);
+    U[ly * kVS + lx] = pr.u;
+    V[ly * kVS + lx] = pr.v;
+    Tt[ly * kVS + lx] = pr.p / (pr.rho * P.rgas);
+  }
 }
 
 enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
@@ -832,33 +840,52 @@ enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
 // the Shu-Osher combination (OOP/time_operator.py:24-26) and clean =
 // enforce (+ outflow guard: every node takes the value of its clamped
 // interior source, OOP/problems/riemann_config3.py:419-429).
+// One 16x16 tile per block: the y-derivative (stored transposed, [c][x][y])
+// is staged through LDS so both derivative reads are coalesced.
 template <class T>
 __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
-                                                   const T* __restrict__ qs, const T* __restrict__ dfx,
-                                                   const T* __restrict__ dfyT, const T* __restrict__ fx,
-                                                   const T* __restrict__ fy, const double* __restrict__ scal,
-                                                   const double* __restrict__ drows, T dt, T* __restrict__ out,
-                                                   int bx, int by) {
+                                            const T* __restrict__ qs, const T* __restrict__ dfx,
+                                            const T* __restrict__ dfyT, const T* __restrict__ fx,
+                                            const T* __restrict__ fy, const double* __restrict__ scal,
+                                            const double* __restrict__ drows, T dt, T* __restrict__ out, int bx,
+                                            int by) {
+  __shared__ T sU[20 * kVS], sV[20 * kVS], sT[20 * kVS];
+  __shared__ T sD[4][16][17];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
-  if (x >= nx || y >= ny) return;
+  const int x0 = bx * 16, y0 = by * 16;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool in = x < nx && y < ny;
   int cx = x, cy = y;
   if (P.guard > 0) {
     cx = clampi(x, P.guard, nx - 1 - P.guard);
     cy = clampi(y, P.guard, ny - 1 - P.guard);
   }
   const int idx = cy * nx + cx, oidx = y * nx + x;
-  const size_t tidx = (size_t)cx * ny + cy;
+  const bool visc = mode == MODE_VISC || (mode == MODE_STAGE && P.visc_on);
+  if (visc) viscous_tile(P, qs, x0, y0, sU, sV, sT);
+  if (mode != MODE_VISC) {
+    // dfyT tile: thread (ty, tx) reads x-row x0+ty, y = y0+tx (unit stride in y)
+    const int rx = x0 + ty, ry = y0 + tx;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sD[c][tx][ty] = (rx < nx && ry < ny) ? dfyT[c * N + (size_t)rx * ny + ry] : T(0);
+  }
+  __syncthreads();
+  if (!in) return;
   if (mode == MODE_VISC) {  // standalone viscous_rhs of qs
     T vv[4];
-    viscous_at(P, qs, cy, cx, vv);
+    viscous_lds(P, sU, sV, sT, ty + 2, tx + 2, vv);
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c * N + oidx] = vv[c];
     return;
   }
+  const bool own = cx == x && cy == y;  // guard nodes take their interior source's values
   T Lv[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) Lv[c] = (-dfx[c * N + idx]) - dfyT[c * N + tidx];
+  for (int c = 0; c < 4; ++c) {
+    const T gy = own ? sD[c][ty][tx] : dfyT[c * N + (size_t)cx * ny + cy];
+    Lv[c] = (-dfx[c * N + idx]) - gy;
+  }
   if (mode == MODE_RHS) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c * N + oidx] = Lv[c];
@@ -868,7 +895,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   load4(qs, N, idx, s0, s1, s2, s3);
   if (P.visc_on) {
     T vv[4];
-    viscous_at(P, qs, cy, cx, vv);
+    viscous_lds(P, sU, sV, sT, ty + 2, tx + 2, vv);
 #pragma unroll
     for (int c = 0; c < 4; ++c) Lv[c] = Lv[c] + vv[c];
   }
@@ -884,8 +911,8 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       const double* H = drows + (size_t)cy * 10;
       double lx = H[0], ly = H[5];
       for (int k = 1; k <= P.dkx; ++k) {
-        const double* tx = P.twx + ((size_t)(k + P.kmx) * nx + cx) * 2;
-        const double c = tx[0], sn = tx[1];
+        const double* tw = P.twx + ((size_t)(k + P.kmx) * nx + cx) * 2;
+        const double c = tw[0], sn = tw[1];
         lx += H[2 * k - 1] * c - H[2 * k] * sn;
         ly += H[5 + 2 * k - 1] * c - H[5 + 2 * k] * sn;
       }

@@ -1,0 +1,321 @@
+// Line sweep with one wavefront per grid line (lines of n = 64*L nodes,
+// L = 1..8): the x- or y-derivative of the hybrid compact/WENO7 flux
+// (OOP/spatial_operator.py:321-342 periodic, OOP/problems/riemann_config3.py:329-360
+// non-periodic) without block barriers.
+//
+// Lane l holds nodes l*L .. l*L+L-1 of its line in registers.  Per
+// component, the conservative variable and the flux are staged in the wave's
+// LDS slice so interface stencils read neighbours directly; the WENO7 flux of
+// the interfaces around flagged ones is computed from a wave-compacted work
+// list (ballot + prefix count), so a shock cluster is shared by all 64 lanes.
+// The compact solve A w = raw, A = (1+rho^2)^-1 (1 - rho S^-1)(1 - rho S),
+// runs the causal and anti-causal first-order recurrences locally over each
+// lane's L nodes and joins the lanes with a 6-step affine scan
+// (C_{l+1} = e_l + rho^L C_l) across the wavefront; the periodic wrap is
+// closed in closed form, C_0 = (e_63 + rho^L P_63) / (1 - rho^n).
+// Non-periodic lines use identity end rows, with the end contributions moved
+// to the right-hand side and the homogeneous corrections a*rho^i +
+// b*rho^(n-1-i) fixed by the two end values, as the block form does.
+#pragma once
+
+#include "hitx_kernels.h"
+
+namespace hitx {
+
+constexpr int kLineWaveMaxL = 8;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS bytes of one wave's slice for a line of n nodes.
+__host__ __device__ inline size_t line_wave_lds(int n, int esize) {
+  return (size_t)(3 * (n + 1)) * esize + (size_t)(n + 1) * 2 + (size_t)n + 16;
+}
+
+// b^e for 0 <= e < 2^10 by squaring (uniform or per-lane e).
+template <class T>
+__device__ __forceinline__ T ipow(T b, int e) {
+  T r = T(1);
+#pragma unroll
+  for (int bit = 0; bit < 10; ++bit) {
+    if ((e >> bit) & 1) r = r * b;
+    b = b * b;
+  }
+  return r;
+}
+
+template <class T>
+__device__ __forceinline__ T shfl_up_rot(T v, int d) {  // value of lane (l - d) mod 64
+  return __shfl(v, (int)((threadIdx.x - d) & 63), 64);
+}
+template <class T>
+__device__ __forceinline__ T shfl_down_rot(T v, int d) {  // value of lane (l + d) mod 64
+  return __shfl(v, (int)((threadIdx.x + d) & 63), 64);
+}
+
+template <class T, bool PER, int L>
+__device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K, int axis, int enforced_input,
+                                          const T* __restrict__ ql, const uint8_t* __restrict__ ml,
+                                          const double* __restrict__ scal, T* __restrict__ out, int line, int nl,
+                                          unsigned char* slice) {
+  const int n = 64 * L, S = n + 1;
+  const int lane = threadIdx.x & 63;
+  T* qs = reinterpret_cast<T*>(slice);
+  T* fs = qs + S;
+  T* ws = fs + S;
+  uint16_t* wl = reinterpret_cast<uint16_t*>(ws + S);
+  uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + S);
+  const size_t NL = (size_t)nl * n;
+  const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
+  const T h = axis == 0 ? P.dx : P.dy;
+  const T rho = K.rho;
+  // 1. (enforce), primitives, point flux of the lane's L nodes
+  T q[4][L], f[4][L];
+  const size_t g0 = (size_t)line * n + lane * L;
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    T a = ql[g0 + j], b = ql[NL + g0 + j], c = ql[2 * NL + g0 + j], d = ql[3 * NL + g0 + j];
+    if (!enforced_input) enforce(P, a, b, c, d);
+    const Prim<T> pr = prim(P, a, b, c, d);
+    q[0][j] = a;
+    q[1][j] = b;
+    q[2][j] = c;
+    q[3][j] = d;
+    if (axis == 0) {  // OOP/equations.py:183-190
+      f[0][j] = b;
+      f[1][j] = pr.rho * (pr.u * pr.u) + pr.p;
+      f[2][j] = (pr.rho * pr.u) * pr.v;
+      f[3][j] = (d + pr.p) * pr.u;
+    } else {  // OOP/equations.py:192-199
+      f[0][j] = c;
+      f[1][j] = (pr.rho * pr.u) * pr.v;
+      f[2][j] = pr.rho * (pr.v * pr.v) + pr.p;
+      f[3][j] = (d + pr.p) * pr.v;
+    }
+    Mk[lane * L + j] = ml[g0 + j];
+  }
+  wave_sync();
+  const int nI = PER ? n : n + 1;
+  auto iflag = [&](int k) -> bool {
+    if (PER) {
+      k = wrap(k, n);
+      return (Mk[k] | Mk[wrap(k + 1, n)]) != 0;
+    }
+    k = clampi(k, 0, n);
+    return (Mk[clampi(k - 1, 0, n - 1)] | Mk[clampi(k, 0, n - 1)]) != 0;
+  };
+  auto nidx = [&](int j) -> int { return PER ? wrap(j, n) : clampi(j, 0, n - 1); };
+  // 2. work list: interfaces whose WENO7 flux a flagged interface's
+  //    compactification reads (i-1, i, i+1 of every flagged i); interface k
+  //    is periodic x_{k+1/2} (0..n-1) or non-periodic x_{k-1/2} (0..n)
+  int nw = 0;
+  for (int k0 = 0; k0 < nI; k0 += 64) {
+    const int k = k0 + lane;
+    const bool need = k < nI && (P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1));
+    const unsigned long long bal = __ballot(need);
+    if (need) wl[nw + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)k;
+    nw += __popcll(bal);
+  }
+  // 3. per component: interface fluxes, divided difference, compact solve, store
+  const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
+          c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
+  const T aL = ipow(rho, L);
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    wave_sync();  // previous component's LDS reads are done
+#pragma unroll
+    for (int j = 0; j < L; ++j) {  // selects, not a dynamic register index
+      qs[lane * L + j] = c == 0 ? q[0][j] : (c == 1 ? q[1][j] : (c == 2 ? q[2][j] : q[3][j]));
+      fs[lane * L + j] = c == 0 ? f[0][j] : (c == 1 ? f[1][j] : (c == 2 ? f[2][j] : f[3][j]));
+    }
+    wave_sync();
+    // WENO7 of the Lax-Friedrichs split on the listed interfaces
+    for (int t = lane; t < nw; t += 64) {
+      const int k = wl[t];
+      const int il = PER ? k : k - 1;
+      T fp[7], fm[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int ip = nidx(il - 3 + j);  // f+ stencil i-3 .. i+3
+        const int im = nidx(il + 4 - j);  // f- stencil i+4 .. i-2
+        fp[j] = T(0.5) * (fs[ip] + alpha * qs[ip]);
+        fm[j] = T(0.5) * (fs[im] - alpha * qs[im]);
+      }
+      ws[k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
+              weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
+    }
+    wave_sync();
+    // hybrid interface flux of interfaces kb .. kb+L (one beyond the lane's own)
+    auto iface = [&](int k) -> T {
+      if (P.weno_only || iflag(k)) {
+        const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
+        const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
+        return (K.alpha_c * ws[kr] + ws[k]) + K.alpha_c * ws[kl];
+      }
+      const int il = PER ? k : k - 1;
+      return (c3 * (fs[nidx(il + 3)] + fs[nidx(il - 2)]) + c2 * (fs[nidx(il + 2)] + fs[nidx(il - 1)])) +
+             c1 * (fs[nidx(il + 1)] + fs[nidx(il)]);
+    };
+    const int kb = PER ? lane * L - 1 : lane * L;
+    T x[L];
+    {
+      T Fprev = iface(PER ? wrap(kb, n) : kb);
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        const T Fn = iface(PER ? wrap(kb + j + 1, n) : kb + j + 1);
+        x[j] = (Fn - Fprev) / h;  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
+        Fprev = Fn;
+      }
+    }
+    // non-periodic: identity end rows, end contributions on the right-hand side
+    T r0 = T(0), rn1 = T(0);
+    if (!PER) {
+      r0 = __shfl(x[0], 0, 64);
+      rn1 = __shfl(x[L - 1], 63, 64);
+    }
+    T y[L];
+    // causal pass, local
+    T acc = T(0);
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+      T xv = x[j];
+      if (!PER) {
+        const int i = lane * L + j;
+        if (i == 0 || i == n - 1) {
+          xv = T(0);
+        } else {
+          if (i == 1) xv = xv - K.alpha_c * r0;
+          if (i == n - 2) xv = xv - K.alpha_c * rn1;
+        }
+      }
+      acc = xv + rho * acc;
+      y[j] = acc;
+    }
+    // carries: inclusive scan of C_{l+1} = e_l + a C_l, a = rho^L
+    T sc = acc, ad = aL;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const T o = shfl_up_rot(sc, d);
+      if (lane >= d) sc = sc + ad * o;
+      ad = ad * ad;
+    }
+    T carry = shfl_up_rot(sc, 1);  // exclusive: P_l
+    if (lane == 0) carry = T(0);
+    if (PER) {
+      // C_0 = (e_63 + a P_63) / (1 - rho^n) = S_63 / (1 - rho^n); C_l = P_l + a^l C_0
+      const T c0 = __shfl(sc, 63, 64) * K.inv1mrn;
+      carry = carry + ipow(aL, lane) * c0;
+    }
+    {
+      T pw = rho;
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        y[j] = y[j] + pw * carry;
+        pw = pw * rho;
+      }
+    }
+    T zn = T(0);
+    if (!PER) zn = __shfl((rho * y[L - 1]) / K.one_m_rho2, 63, 64);
+    // anti-causal pass, local
+    acc = T(0);
+#pragma unroll
+    for (int j = L - 1; j >= 0; --j) {
+      acc = y[j] + rho * acc;
+      y[j] = acc;
+    }
+    // carries from the right: D_{l-1} = g_l + a D_l
+    sc = acc;
+    ad = aL;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const T o = shfl_down_rot(sc, d);
+      if (lane + d < 64) sc = sc + ad * o;
+      ad = ad * ad;
+    }
+    carry = shfl_down_rot(sc, 1);  // exclusive: Q_l
+    if (lane == 63) carry = T(0);
+    {
+      // wrap (periodic) or the value beyond the end (non-periodic): a^(63-l) times
+      const T ar = ipow(aL, 63 - lane);
+      if (PER)
+        carry = carry + ar * (__shfl(sc, 0, 64) * K.inv1mrn);
+      else
+        carry = carry + ar * zn;
+    }
+    {
+      T pw = rho;
+#pragma unroll
+      for (int j = L - 1; j >= 0; --j) {
+        y[j] = K.c1p * (y[j] + pw * carry);
+        pw = pw * rho;
+      }
+    }
+    T* o = out + (size_t)c * NL + g0;
+    if (PER) {
+#pragma unroll
+      for (int j = 0; j < L; ++j) o[j] = y[j];
+    } else {
+      const T p0 = __shfl(y[0], 0, 64), pn = __shfl(y[L - 1], 63, 64);
+      const T det = T(1) - K.rnm1 * K.rnm1;
+      const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
+      T pl = ipow(rho, lane * L), pr = ipow(rho, n - 1 - lane * L);  // rho^i, rho^(n-1-i)
+      const T irho = T(1) / rho;
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        const int i = lane * L + j;
+        T v;
+        if (i == 0)
+          v = r0;
+        else if (i == n - 1)
+          v = rn1;
+        else
+          v = (y[j] + a * pl) + bb * pr;
+        o[j] = v;
+        pl = pl * rho;
+        pr = pr * irho;
+      }
+    }
+  }
+}
+
+// Dispatch on L = n / 64 (the caller checks n % 64 == 0 and n <= 512).
+template <class T, bool PER>
+__device__ __forceinline__ void line_wave_any(const Params<T>& P, const LineK<T>& K, int axis, int enforced_input,
+                                              const T* __restrict__ ql, const uint8_t* __restrict__ ml,
+                                              const double* __restrict__ scal, T* __restrict__ out, int line,
+                                              int nl, unsigned char* slice) {
+  switch (K.n >> 6) {
+    case 1: line_wave<T, PER, 1>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    case 2: line_wave<T, PER, 2>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    case 3: line_wave<T, PER, 3>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    case 4: line_wave<T, PER, 4>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    case 5: line_wave<T, PER, 5>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    case 6: line_wave<T, PER, 6>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    case 7: line_wave<T, PER, 7>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+    default: line_wave<T, PER, 8>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+  }
+}
+
+constexpr int kLineWaves = 2;  // lines (wavefronts) per block
+
+template <class T, bool PER>
+__global__ void __launch_bounds__(64 * kLineWaves) k_line_wave(Params<T> P, LineK<T> K, int axis,
+                                                              int enforced_input, const T* __restrict__ ql,
+                                                              const uint8_t* __restrict__ ml,
+                                                              const double* __restrict__ scal, T* __restrict__ out,
+                                                              int nl, int slice_bytes) {
+  extern __shared__ __align__(16) unsigned char wsm[];
+  const int w = threadIdx.x >> 6;
+  const int line = blockIdx.x * kLineWaves + w;
+  if (line >= nl) return;
+  line_wave_any<T, PER>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, wsm + (size_t)w * slice_bytes);
+}
+
+// The block form handles lines that are not a multiple of 64 nodes or longer
+// than 512.
+inline bool line_wave_ok(int n) { return n % 64 == 0 && n >= 64 && n <= 64 * kLineWaveMaxL; }
+
+}  // namespace hitx

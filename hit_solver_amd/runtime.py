@@ -205,6 +205,11 @@ class Plan:
         check(lib().hitx_viscous_rhs(self.handle, _p(q), _p(out), _stream()))
         return out
 
+    def set_line_form(self, form: int):
+        """0: one wavefront per line where the line length allows (multiple of
+        64, at most 512), else the block form; 1: the block form everywhere."""
+        check(lib().hitx_set_line_form(self.handle, int(form)))
+
     def timing(self, on: bool):
         check(lib().hitx_timing_enable(self.handle, int(on)))
 

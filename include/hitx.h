@@ -175,6 +175,13 @@ int hitx_viscous_rhs(hitx_plan* plan, const void* q, void* out, void* stream);
 size_t hitx_normals_workspace_bytes(int64_t n);
 int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* ws, size_t ws_bytes, void* stream);
 
+/* Launch-shape support (no reference counterpart): which form of the line
+ * sweep the plan launches.  1: one block of 512 threads per line; 2: one
+ * wavefront per line where the line length is a multiple of 64 up to 512
+ * (the block form otherwise); 0 (default): the wave form when the sweep has
+ * at least 2048 such lines.  Both compute the same derivative to rounding. */
+int hitx_set_line_form(hitx_plan* plan, int32_t form);
+
 /* Measurement support (no reference counterpart): when enabled, every
  * line-sweep kernel launch of this plan is bracketed by HIP events on its
  * stream.  hitx_timing_read synchronises those events, returns the summed
@@ -214,12 +221,14 @@ int hitx_batch_destroy(hitx_batch* batch);
 size_t hitx_batch_workspace_bytes(const hitx_batch* batch);
 /* Carve the instances' buffers from ws and upload the instance table (blocking). */
 int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
-/* Tile heights of the column-walk kernels for every instance (no reference
- * counterpart; the batch autotuner's knob): rb rows per wave of the stage
- * reductions (2..64), rbp rows per wave of the forcing projection (8..256).
- * Changes the summation order of those reductions only.  Blocking when bound
- * (re-uploads the instance table); no work may be in flight on the batch. */
-int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp);
+/* Launch shapes of every instance (no reference counterpart; the batch
+ * autotuner's knobs): rb rows per wave of the stage reductions (2..64), rbp
+ * rows per wave of the forcing projection (8..256), and the line-sweep form
+ * (0 auto, 1 block, 2 one wavefront per line where the line allows it).
+ * Changes the summation order of the reductions and the rounding of the line
+ * solve only.  Blocking when bound (re-uploads the instance table); no work
+ * may be in flight on the batch. */
+int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_form);
 /* Device pointer of one of instance i's buffers inside the bound workspace. */
 int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
 /* Fixed dt of every instance from its current state (Riemann loop). */

@@ -127,7 +127,8 @@ def test_hit_batch_tuned_tiles_agree():
     cfgs = [HIT2DConfig(nx=64, ny=48, target_mach=0.25, forcing_seed=3, **kw),
             HIT2DConfig(nx=40, ny=72, target_mach=0.4, forcing_seed=4, **kw)]
     a, b = BatchRunner(cfgs), BatchRunner(cfgs)
-    b.tune(8, 32)
+    a.tune(2, 8, 1)
+    b.tune(8, 32, 2)
     a.run()
     b.run()
     for i in range(2):

@@ -246,7 +246,7 @@ def test_riemann_runs_golden(golden, tag, cls, kw):
 
 
 # ---------------------------------------------------------------- vs oracle, other sizes
-@pytest.mark.parametrize("nx,ny", [(64, 48), (36, 42), (37, 40), (128, 96)])
+@pytest.mark.parametrize("nx,ny", [(64, 48), (36, 42), (37, 40), (128, 96), (192, 128)])
 def test_periodic_rhs_vs_oracle_sizes(nx, ny):
     p = OH.HITParams(nx=nx, ny=ny, target_mach=0.3, initial_kmin=3, initial_kmax=5)
     q = OH.initial_state(p, np.random.default_rng(nx * 1000 + ny))
@@ -261,7 +261,8 @@ def test_periodic_rhs_vs_oracle_sizes(nx, ny):
     assert_rhs(host(op.rhs(qd)), OP.rhs(q, Gas(), p.dx, p.dy, sp), what=f"{nx}x{ny}")
 
 
-@pytest.mark.parametrize("nx,ny,scheme", [(64, 64, "hybrid"), (50, 36, "hybrid"), (33, 40, "weno")])
+@pytest.mark.parametrize("nx,ny,scheme", [(64, 64, "hybrid"), (50, 36, "hybrid"), (33, 40, "weno"),
+                                          (128, 192, "hybrid"), (64, 128, "weno")])
 def test_nonperiodic_rhs_vs_oracle_sizes(nx, ny, scheme):
     p = OR.config3(nx=nx, ny=ny, shear_threshold=None, scheme=scheme)
     run = OR.RiemannRun(p)
@@ -271,6 +272,39 @@ def test_nonperiodic_rhs_vs_oracle_sizes(nx, ny, scheme):
     sp = p.sensor_params()
     np.testing.assert_array_equal(host(s.detect(dev(q))), ON.sensor(q, Gas(), p.dx, p.dy, sp))
     assert_rhs(host(op.rhs(dev(q))), ON.rhs(q, Gas(), p.dx, p.dy, sp, scheme), what=f"{nx}x{ny}")
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 2e-6)])
+@pytest.mark.parametrize("case", ["periodic", "nonperiodic"])
+def test_line_forms_agree(case, dtype, tol):
+    """The one-wavefront-per-line sweep and the block sweep give the same
+    derivative (rounding only: the wave form's solve is an exact scan, the
+    block form's truncates |rho|^52) on lines of 64*L nodes, L = 1..8."""
+    if case == "periodic":
+        nx, ny = 512, 320
+        p = OH.HITParams(nx=nx, ny=ny, target_mach=0.3, initial_kmin=3, initial_kmax=5)
+        q = OH.initial_state(p, np.random.default_rng(7))
+        x = np.arange(nx) * p.dx
+        q[0] *= 1.0 + 0.6 * (np.sin(x)[None, :] > 0.3)
+        q = o_enforce(q, Gas())
+        _, _, _, op, _ = _hit_objs(nx, ny)
+    else:
+        nx, ny = 448, 64
+        p = OR.config3(nx=nx, ny=ny, shear_threshold=None)
+        run = OR.RiemannRun(p)
+        run.run(max_steps=2)
+        q = run.q
+        _, _, _, op, _ = _rie_objs(nx, ny)
+    qd = dev(q, dtype)
+    plan = op.plan(dtype)
+    plan.set_line_form(1)
+    a = host(op.rhs(qd)).astype(np.float64)
+    plan.set_line_form(2)
+    b = host(op.rhs(qd)).astype(np.float64)
+    plan.set_line_form(0)
+    for c in range(4):
+        scale = np.abs(a[c]).max()
+        assert np.abs(b[c] - a[c]).max() <= tol * scale, (case, c, np.abs(b[c] - a[c]).max() / scale)
 
 
 def test_riemann_oracle_multistep_128():
