@@ -209,20 +209,21 @@ class BatchRunner:
         for _ in range(n):
             self._graph.replay()
 
-    def tune(self, rb: int, rbp: int, line_form: int = 0):
+    def tune(self, rb: int, rbp: int, line_form: int = 0, sensor_form: int = 0):
         """Launch shapes of every instance: rows per wave of the stage
-        reductions and of the forcing projection, and the line-sweep form
-        (0 auto, 1 block per line, 2 wavefront per line)."""
+        reductions and of the forcing projection, the line-sweep form (0 auto,
+        1 block per line, 2 wavefront per line) and the sensor form (0 auto,
+        1 fused tile, 2 split)."""
         torch.cuda.synchronize()
-        check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp), int(line_form)))
+        check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp), int(line_form), int(sensor_form)))
         self._graph = None  # grid sizes changed
 
     def autotune(self, candidates=None, steps: int = 3):
-        """Time a few steps per candidate (rb, rbp, line_form) on a copy of
-        the batch's device state, restore the state and keep the fastest
-        candidate.  Returns (best, {candidate: ms per step})."""
+        """Time a few steps per candidate (rb, rbp, line_form, sensor_form)
+        on a copy of the batch's device state, restore the state and keep the
+        fastest candidate.  Returns (best, {candidate: ms per step})."""
         if candidates is None:
-            candidates = [(rb, rbp, form) for form in (1, 2) for rb in (4, 16, 32)
+            candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2) for rb in (4, 16, 32)
                           for rbp in ((16, 64) if self.hit else (16,))]
         torch.cuda.synchronize()
         backup = self.ws.clone()

@@ -204,7 +204,7 @@ size_t line_lds_bytes(int n, int nch, int esize) {
 
 size_t sensor_lds_bytes(int w, int esize) {
   const int H = w + 1, EX = kSTX + 2 * H, EY = kSTY + 2 * H;
-  return (size_t)4 * EX * EY * esize + (size_t)(kSTX + 2 * w) * (kSTY + 2 * w) + (size_t)(kSTY + 2 * w) * kSTX +
+  return (size_t)5 * EX * EY * esize + (size_t)(kSTX + 2 * w) * (kSTY + 2 * w) + (size_t)(kSTY + 2 * w) * kSTX +
          (size_t)kSTY * (kSTX + 1) + 64;
 }
 
@@ -766,7 +766,7 @@ struct hitx_batch {
   std::vector<double> kpar;  // n * K_COUNT
   struct Buf {
     size_t Q, Q1, Q2, S, qT, dfx, dfyT, fx, fy, mask, maskT, part, scal, ctl, kpar, phi, fhat, noise, fpart, kxy, rng,
-        rngws, drows, frows;
+        rngws, drows, frows, rawm;
   };
   std::vector<Buf> buf;
   size_t off_tab = 0, off_offs = 0, ws_bytes = 0;
@@ -775,6 +775,8 @@ struct hitx_batch {
   size_t line_lds = 0, sensor_lds = 0, proj_lds = 0, wave_slice = 16;
   int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave
   bool wave_x = false, wave_y = false;
+  int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split
+  bool split_sensor = false;
   void* ws = nullptr;
 };
 
@@ -808,6 +810,9 @@ void batch_offsets(hitx_batch* b) {
     if (line_wave_ok(pl->d.nx)) lx += pl->d.ny;
     if (line_wave_ok(pl->d.ny)) ly += pl->d.nx;
   }
+  long long cells = 0;
+  for (const hitx_plan* pl : b->plans) cells += pl->N;
+  b->split_sensor = b->sensor_form == 2 || (b->sensor_form == 0 && cells >= (1ll << 20));
   b->wave_x = b->line_form == 2 || (b->line_form == 0 && lx >= kWaveFormMinLines);
   b->wave_y = b->line_form == 2 || (b->line_form == 0 && ly >= kWaveFormMinLines);
   if (b->line_form == 1) b->wave_x = b->wave_y = false;
@@ -834,7 +839,7 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   auto d = [&](size_t off) { return reinterpret_cast<double*>(base + off); };
   I.Q = t(o.Q); I.Q1 = t(o.Q1); I.Q2 = t(o.Q2); I.S = t(o.S);
   I.qT = t(o.qT); I.dfx = t(o.dfx); I.dfyT = t(o.dfyT); I.fx = t(o.fx); I.fy = t(o.fy);
-  I.mask = base + o.mask; I.maskT = base + o.maskT;
+  I.mask = base + o.mask; I.maskT = base + o.maskT; I.rawm = base + o.rawm;
   I.part = d(o.part); I.scal = d(o.scal); I.ctl = d(o.ctl); I.kpar = d(o.kpar); I.phi = d(o.phi);
   I.fhat = d(o.fhat); I.noise = d(o.noise); I.fpart = d(o.fpart); I.kxy = d(o.kxy);
   I.rng = reinterpret_cast<uint64_t*>(base + o.rng);
@@ -889,7 +894,12 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   b_stage_reduce<T, DRAG><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
   b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st);
   if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
-  b_sensor<T><<<b->total[OFF_SENS], kBlock, b->sensor_lds, s>>>(tab, offs, n, st);
+  if (b->split_sensor) {
+    b_rawmask<T><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
+    b_dilate<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+  } else {
+    b_sensor<T><<<b->total[OFF_SENS], kBlock, b->sensor_lds, s>>>(tab, offs, n, st);
+  }
   if (st == 3) {
     LAUNCHCHK();
     return HITX_OK;
@@ -1024,7 +1034,7 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     u.Q = take(4 * N * E); u.Q1 = take(4 * N * E); u.Q2 = take(4 * N * E); u.S = take(4 * N * E);
     u.qT = take(4 * N * E); u.dfx = take(4 * N * E); u.dfyT = take(4 * N * E);
     u.fx = take(N * E); u.fy = take(N * E);
-    u.mask = take(N); u.maskT = take(N);
+    u.mask = take(N); u.maskT = take(N); u.rawm = take(N);
     // partials sized for the smallest tile heights hitx_batch_tune accepts
     const int sr_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbMin, 1);
     const int pj_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbpMin, 0);
@@ -1087,12 +1097,14 @@ int hitx_batch_buffer(const hitx_batch* b, int32_t i, int32_t which, void** ptr)
   return HITX_OK;
 }
 
-int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form) {
+int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, int32_t sensor_form) {
   if (!b) return fail(HITX_EINVAL, "null batch");
   if (rb < kTuneRbMin || rb > kTuneRbMax || rbp < kTuneRbpMin || rbp > kTuneRbpMax)
     return fail(HITX_EINVAL, "tile heights out of range (rb 2..64, rbp 8..256)");
   if (line_form < 0 || line_form > 2) return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block) or 2 (wave)");
+  if (sensor_form < 0 || sensor_form > 2) return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused) or 2 (split)");
   b->line_form = line_form;
+  b->sensor_form = sensor_form;
   for (hitx_plan* pl : b->plans) {
     pl->rb = rb;
     pl->rbp = rbp;

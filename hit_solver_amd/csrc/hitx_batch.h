@@ -34,7 +34,7 @@ struct Inst {
   LineK<T> Kx, Ky;
   T *Q, *Q1, *Q2, *S;  // state, stage 1 and 2 results, stage 3 result (before hyperviscosity)
   T *qT, *dfx, *dfyT, *fx, *fy;
-  uint8_t *mask, *maskT;
+  uint8_t *mask, *maskT, *rawm;  // dilated mask, its transpose, undilated mask (split sensor)
   double *part, *scal, *ctl, *kpar, *phi, *fhat, *noise, *fpart, *kxy;
   uint64_t* rng;  // PCG64 state: state hi, state lo, increment hi, increment lo
   RngJob R;       // forcing noise request: nx*ny normals of the rng stream into noise
@@ -112,6 +112,25 @@ __global__ void __launch_bounds__(kBlock) b_sensor(const Inst<T>* __restrict__ t
   sensor_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.mask, I.maskT, st < 3 ? I.qT : nullptr, bx, by);
 }
 
+// Split sensor: undilated mask by column walk, then dilation + transposes.
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_rawmask(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                    int ninst, int st) {
+  HITX_BATCH_INST(OFF_SR)
+  if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  rawmask_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, lb);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) b_dilate(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                   int ninst, int st) {
+  HITX_BATCH_INST(OFF_SENS)
+  if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  const int ntx = (I.P.nx + kSTX - 1) / kSTX;
+  const int by = lb / ntx, bx = lb - by * ntx;
+  dilate_body<T>(I.P, stage_in(I, st), st < 3, I.rawm, I.mask, I.maskT, st < 3 ? I.qT : nullptr, bx, by);
+}
+
 template <class T, bool PER>
 __global__ void __launch_bounds__(kLineBlock, 4) b_line(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                      int ninst, int axis, int st) {
@@ -124,9 +143,9 @@ __global__ void __launch_bounds__(kLineBlock, 4) b_line(const Inst<T>* __restric
 
 // Wave form of the sweep: kLineWaves lines per block, one per wavefront.
 template <class T, bool PER>
-__global__ void __launch_bounds__(64 * kLineWaves) b_line_wave(const Inst<T>* __restrict__ tab,
-                                                              const int* __restrict__ offs, int ninst, int axis, int st,
-                                                              int slice_bytes) {
+__global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
+    b_line_wave(const Inst<T>* __restrict__ tab, const int* __restrict__ offs, int ninst, int axis, int st,
+                int slice_bytes) {
   extern __shared__ __align__(16) unsigned char wsm[];
   HITX_BATCH_INST(axis == 0 ? OFF_LXW : OFF_LYW)
   const int w = threadIdx.x >> 6;
@@ -146,11 +165,15 @@ __global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ t
   HITX_BATCH_INST(OFF_T16)
   const int ntx = (I.P.nx + 15) / 16;
   const int by = lb / ntx, bx = lb - by * ntx;
+  // the last stage writes Q in place when no hyperviscosity pass follows and
+  // there is no outflow guard (each node then reads Q only at itself)
+  T* out = (st == 2 && I.ctl[C_HVNOW] == 0.0 && I.P.guard == 0) ? I.Q : stage_out(I, st);
   update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
-                 stage_out(I, st), bx, by);
+                 out, bx, by);
 }
 
-// After the stages: hyperviscosity S -> Q with the fresh mask, or S -> Q copy.
+// After the stages: hyperviscosity S -> Q with the fresh mask, or the S -> Q
+// copy when the last stage could not write Q in place.
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_finish(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst) {
@@ -161,6 +184,7 @@ __global__ void __launch_bounds__(kBlock) b_finish(const Inst<T>* __restrict__ t
     hv_body<T>(I.P, I.S, I.mask, 1, 1, I.Q, bx, by);
     return;
   }
+  if (I.P.guard == 0) return;
   const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
   if (x >= I.P.nx || y >= I.P.ny) return;
   const int N = I.P.nx * I.P.ny, o = y * I.P.nx + x;

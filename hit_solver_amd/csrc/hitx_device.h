@@ -70,6 +70,19 @@ enum {
 enum { R_MAXR = 0, R_MAXP, R_MAXE, R_AX, R_AY, R_DIV2, R_VORT2, R_PSUM, R_DRAG };
 
 // ---------------------------------------------------------------- algebra
+// Division in the pointwise algebra: IEEE for double (the FP64 mode
+// reproduces the reference's quotients bit for bit); for float, the hardware
+// reciprocal times the numerator (<= 2 ulp; the FP32 mode is held to its
+// error budget, not to bit parity).
+template <class T>
+__device__ __forceinline__ T dv(T a, T b) {
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float dv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+
 // np.maximum: NaN-propagating maximum.
 template <class T>
 __device__ __forceinline__ T npmax(T a, T b) {
@@ -86,8 +99,8 @@ template <class T>
 __device__ __forceinline__ Prim<T> prim(const Params<T>& P, T q0, T q1, T q2, T q3) {
   Prim<T> r;
   r.rho = npmax(q0, P.rho_floor);
-  r.u = q1 / r.rho;
-  r.v = q2 / r.rho;
+  r.u = dv(q1, r.rho);
+  r.v = dv(q2, r.rho);
   T ke = (T(0.5) * r.rho) * (r.u * r.u + r.v * r.v);
   T p = P.gm1 * (q3 - ke);
   r.p = npmax(p, P.p_floor);
@@ -98,8 +111,8 @@ __device__ __forceinline__ Prim<T> prim(const Params<T>& P, T q0, T q1, T q2, T 
 template <class T>
 __device__ __forceinline__ void enforce(const Params<T>& P, T& q0, T& q1, T& q2, T& q3) {
   T rho = npmax(q0, P.rho_floor);
-  T u = q1 / rho;
-  T v = q2 / rho;
+  T u = dv(q1, rho);
+  T v = dv(q2, rho);
   T ke = (T(0.5) * rho) * (u * u + v * v);
   T p = P.gm1 * (q3 - ke);
   q0 = rho;
@@ -152,12 +165,12 @@ __device__ __forceinline__ T weno7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
           + v6 * (T(7043) * v6 - T(3882) * v7))
          + T(547) * (v7 * v7);
   T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2, e3 = eps + b3;
-  T a0 = T(1.0 / 35.0) / (e0 * e0);
-  T a1 = T(12.0 / 35.0) / (e1 * e1);
-  T a2 = T(18.0 / 35.0) / (e2 * e2);
-  T a3 = T(4.0 / 35.0) / (e3 * e3);
+  T a0 = dv(T(1.0 / 35.0), e0 * e0);
+  T a1 = dv(T(12.0 / 35.0), e1 * e1);
+  T a2 = dv(T(18.0 / 35.0), e2 * e2);
+  T a3 = dv(T(4.0 / 35.0), e3 * e3);
   T asum = ((a0 + a1) + a2) + a3;
-  return (((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3) / asum;
+  return dv((((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3), asum);
 }
 
 // ---------------------------------------------------------------- reductions

@@ -106,25 +106,25 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       const T vl = lane_from(cu.v, lane - 1), vr = lane_from(cu.v, lane + 1);
       if (W.own) {
         const int x = W.x;
-        const T cs = sqrt((P.gamma * cu.p) / cu.rho);
+        const T cs = sqrt(dv(P.gamma * cu.p, cu.rho));
         v[R_MAXR] = fmax(v[R_MAXR], (double)absT(cu.rho));
         v[R_MAXP] = fmax(v[R_MAXP], (double)absT(cu.p));
-        const T e = cu.p / (cu.rho * P.gm1);
+        const T e = dv(cu.p, cu.rho * P.gm1);
         v[R_MAXE] = fmax(v[R_MAXE], (double)absT(e));
         v[R_AX] = fmax(v[R_AX], (double)(absT(cu.u) + cs));
         v[R_AY] = fmax(v[R_AY], (double)(absT(cu.v) + cs));
         T ux, uy, vx, vy;
         if (P.periodic) {
-          ux = (ur - ul) / P.two_dx;
-          vx = (vr - vl) / P.two_dx;
-          uy = (nn.u - pv.u) / P.two_dy;
-          vy = (nn.v - pv.v) / P.two_dy;
+          ux = dv(ur - ul, P.two_dx);
+          vx = dv(vr - vl, P.two_dx);
+          uy = dv(nn.u - pv.u, P.two_dy);
+          vy = dv(nn.v - pv.v, P.two_dy);
         } else {  // OOP/problems/riemann_config3.py:268-275 (clamped neighbours)
           const bool xin = (x > 0 && x < nx - 1), yin = (y > 0 && y < ny - 1);
-          ux = xin ? (ur - ul) / P.two_dx : (ur - ul) / P.dx;
-          vx = xin ? (vr - vl) / P.two_dx : (vr - vl) / P.dx;
-          uy = yin ? (nn.u - pv.u) / P.two_dy : (nn.u - pv.u) / P.dy;
-          vy = yin ? (nn.v - pv.v) / P.two_dy : (nn.v - pv.v) / P.dy;
+          ux = dv(ur - ul, xin ? P.two_dx : P.dx);
+          vx = dv(vr - vl, xin ? P.two_dx : P.dx);
+          uy = dv(nn.u - pv.u, yin ? P.two_dy : P.dy);
+          vy = dv(nn.v - pv.v, yin ? P.two_dy : P.dy);
         }
         const T div = ux + vy;
         v[R_DIV2] += (double)div * (double)div;
@@ -352,15 +352,15 @@ __global__ void __launch_bounds__(kBlock) k_step_reduce(Params<T> P, const T* __
 // non-periodic case), at_lo/at_hi flag the domain ends.
 template <class T>
 __device__ __forceinline__ T rel_jump(T fl, T f, T fr, T eps, bool periodic, bool at_lo, bool at_hi) {
-  const T cv = absT((fr - T(2.0) * f) + fl) / (((absT(fr) + T(2.0) * absT(f)) + absT(fl)) + eps);
+  const T cv = dv(absT((fr - T(2.0) * f) + fl), (((absT(fr) + T(2.0) * absT(f)) + absT(fl)) + eps));
   if (periodic) {
-    const T jr = absT(fr - f) / ((absT(fr) + absT(f)) + eps);
-    const T jl = absT(f - fl) / ((absT(f) + absT(fl)) + eps);
+    const T jr = dv(absT(fr - f), ((absT(fr) + absT(f)) + eps));
+    const T jl = dv(absT(f - fl), ((absT(f) + absT(fl)) + eps));
     return npmax(npmax(jl, jr), cv);
   }
   T j = T(0);
-  if (!at_hi) j = npmax(j, absT(fr - f) / ((absT(fr) + absT(f)) + eps));
-  if (!at_lo) j = npmax(j, absT(f - fl) / ((absT(f) + absT(fl)) + eps));
+  if (!at_hi) j = npmax(j, dv(absT(fr - f), ((absT(fr) + absT(f)) + eps)));
+  if (!at_lo) j = npmax(j, dv(absT(f - fl), ((absT(f) + absT(fl)) + eps)));
   return npmax(j, cv);
 }
 
@@ -385,7 +385,8 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
   T* Lu = Lr + EX * EY;
   T* Lv = Lu + EX * EY;
   T* Lp = Lv + EX * EY;
-  uint8_t* raw = reinterpret_cast<uint8_t*>(Lp + EX * EY);
+  T* Le = Lp + EX * EY;  // internal energy p/(rho (gamma-1))
+  uint8_t* raw = reinterpret_cast<uint8_t*>(Le + EX * EY);
   uint8_t* mid = raw + RX * RY;
   uint8_t* outt = mid + RY * kSTX;  // [kSTY][kSTX + 1]
   for (int t = threadIdx.x; t < EX * EY; t += kBlock) {
@@ -403,6 +404,7 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
     Lu[t] = pr.u;
     Lv[t] = pr.v;
     Lp[t] = pr.p;
+    Le[t] = dv(pr.p, pr.rho * P.gm1);
   }
   __syncthreads();
   const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
@@ -416,10 +418,10 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
       const int ic = (r + 1) * EX + (c + 1);
       const int ixl = ic - 1, ixr = ic + 1, iyl = ic - EX, iyr = ic + EX;
       const bool xin = per || (gx > 0 && gx < nx - 1), yin = per || (gy > 0 && gy < ny - 1);
-      const T ux = xin ? (Lu[ixr] - Lu[ixl]) / P.two_dx : (Lu[ixr] - Lu[ixl]) / P.dx;
-      const T vx = xin ? (Lv[ixr] - Lv[ixl]) / P.two_dx : (Lv[ixr] - Lv[ixl]) / P.dx;
-      const T uy = yin ? (Lu[iyr] - Lu[iyl]) / P.two_dy : (Lu[iyr] - Lu[iyl]) / P.dy;
-      const T vy = yin ? (Lv[iyr] - Lv[iyl]) / P.two_dy : (Lv[iyr] - Lv[iyl]) / P.dy;
+      const T ux = dv(Lu[ixr] - Lu[ixl], xin ? P.two_dx : P.dx);
+      const T vx = dv(Lv[ixr] - Lv[ixl], xin ? P.two_dx : P.dx);
+      const T uy = dv(Lu[iyr] - Lu[iyl], yin ? P.two_dy : P.dy);
+      const T vy = dv(Lv[iyr] - Lv[iyl], yin ? P.two_dy : P.dy);
       const T div = ux + vy;
       if (comp_on && (double)div < div_thr) m = true;
       if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
@@ -429,9 +431,8 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
                          rel_jump(Lr[iyl], Lr[ic], Lr[iyr], er, per, ylo, yhi));
       const T jp = npmax(rel_jump(Lp[ixl], Lp[ic], Lp[ixr], ep, per, xlo, xhi),
                          rel_jump(Lp[iyl], Lp[ic], Lp[iyr], ep, per, ylo, yhi));
-      auto e = [&](int i) { return Lp[i] / (Lr[i] * P.gm1); };
-      const T je = npmax(rel_jump(e(ixl), e(ic), e(ixr), ee, per, xlo, xhi),
-                         rel_jump(e(iyl), e(ic), e(iyr), ee, per, ylo, yhi));
+      const T je = npmax(rel_jump(Le[ixl], Le[ic], Le[ixr], ee, per, xlo, xhi),
+                         rel_jump(Le[iyl], Le[ic], Le[iyr], ee, per, ylo, yhi));
       if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
       if (!per && P.bguard > 0) {
         const int g = P.bguard;
@@ -488,6 +489,144 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
       const size_t g = (size_t)gx * ny + gy;
 #pragma unroll
       for (int k = 0; k < 4; ++k) qT[k * (size_t)N + g] = st[k * kSTY * (kSTX + 1) + o];
+    }
+  }
+}
+
+// Split form of the sensor for large batches (the fused tile recomputes
+// primitives over a 2.1x halo and the undilated mask over a 1.9x halo).
+// 1. Column walk (halo 1, P.rb rows per wave): the undilated mask once per
+//    node into raw[ny][nx] (same per-node arithmetic as the fused tile).
+template <class T>
+__device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
+                                             const double* __restrict__ scal, uint8_t* __restrict__ raw, int bid) {
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const Walk W = walk_setup(nx, ny, P.periodic, P.rb, 1, bid);
+  if (!W.live) return;
+  const int lane = threadIdx.x & 63;
+  const bool per = P.periodic;
+  const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
+  const bool comp_on = scal[S_COMP_ON] != 0.0, shear_on = scal[S_SHEAR_ON] != 0.0;
+  const double div_thr = scal[S_DIV_THR], shear_thr = scal[S_SHEAR_THR], thr = P.jump_thr;
+  auto row_of = [&](int y) { return per ? (y < 0 ? y + ny : (y >= ny ? y - ny : y)) : clampi(y, 0, ny - 1); };
+  struct Node {
+    T rho, u, v, p, e;
+  };
+  auto node = [&](T a, T b, T c, T d) {
+    if (enforce_first) enforce(P, a, b, c, d);
+    const Prim<T> pr = prim(P, a, b, c, d);
+    return Node{pr.rho, pr.u, pr.v, pr.p, dv(pr.p, pr.rho * P.gm1)};
+  };
+  auto load_node = [&](int y) {
+    T a, b, c, d;
+    load4(q, N, row_of(y) * nx + W.xl, a, b, c, d);
+    return node(a, b, c, d);
+  };
+  Node pv = load_node(W.ya - 1), cu = load_node(W.ya), nn = load_node(W.ya + 1);
+  const int x = W.x;
+  const bool xin = per || (x > 0 && x < nx - 1);
+  const bool xlo = !per && x == 0, xhi = !per && x == nx - 1;
+  for (int y = W.ya; y < W.yb; ++y) {
+    T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0);
+    if (y + 1 < W.yb) load4(q, N, row_of(y + 2) * nx + W.xl, n0, n1, n2, n3);
+    const T ul = lane_from(cu.u, lane - 1), ur = lane_from(cu.u, lane + 1);
+    const T vl = lane_from(cu.v, lane - 1), vr = lane_from(cu.v, lane + 1);
+    const T rl = lane_from(cu.rho, lane - 1), rr = lane_from(cu.rho, lane + 1);
+    const T pl = lane_from(cu.p, lane - 1), pr = lane_from(cu.p, lane + 1);
+    const T el = lane_from(cu.e, lane - 1), erg = lane_from(cu.e, lane + 1);
+    if (W.own) {
+      bool m = false;
+      const bool yin = per || (y > 0 && y < ny - 1);
+      const T ux = dv(ur - ul, xin ? P.two_dx : P.dx);
+      const T vx = dv(vr - vl, xin ? P.two_dx : P.dx);
+      const T uy = dv(nn.u - pv.u, yin ? P.two_dy : P.dy);
+      const T vy = dv(nn.v - pv.v, yin ? P.two_dy : P.dy);
+      const T div = ux + vy;
+      if (comp_on && (double)div < div_thr) m = true;
+      if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
+      const bool ylo = !per && y == 0, yhi = !per && y == ny - 1;
+      const T jr = npmax(rel_jump(rl, cu.rho, rr, er, per, xlo, xhi), rel_jump(pv.rho, cu.rho, nn.rho, er, per, ylo, yhi));
+      const T jp = npmax(rel_jump(pl, cu.p, pr, ep, per, xlo, xhi), rel_jump(pv.p, cu.p, nn.p, ep, per, ylo, yhi));
+      const T je = npmax(rel_jump(el, cu.e, erg, ee, per, xlo, xhi), rel_jump(pv.e, cu.e, nn.e, ee, per, ylo, yhi));
+      if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+      if (!per && P.bguard > 0) {
+        const int g = P.bguard;
+        if (y < g || y >= ny - g || x < g || x >= nx - g) m = true;
+      }
+      raw[y * nx + x] = m ? 1 : 0;
+    }
+    pv = cu;
+    cu = nn;
+    if (y + 1 < W.yb) nn = node(n0, n1, n2, n3);
+  }
+}
+
+// 2. 32x16 tile: the (2w+1)^2 OR dilation of raw (out-of-domain nodes of the
+//    non-periodic grid false), mask and maskT stores, and (qT not null) the
+//    enforced state written transposed for the y sweep.
+template <class T>
+__device__ __forceinline__ void dilate_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
+                                            const uint8_t* __restrict__ rawg, uint8_t* __restrict__ mask,
+                                            uint8_t* __restrict__ maskT, T* __restrict__ qT, int bx, int by) {
+  __shared__ uint8_t raw[(kSTX + 16) * (kSTY + 16)];
+  __shared__ uint8_t mid[(kSTY + 16) * kSTX];
+  __shared__ uint8_t outt[kSTY * (kSTX + 1)];
+  __shared__ T st[4 * kSTY * (kSTX + 1)];
+  const int nx = P.nx, ny = P.ny, N = nx * ny, w = P.width;
+  const int RX = kSTX + 2 * w, RY = kSTY + 2 * w;
+  const int x0 = bx * kSTX, y0 = by * kSTY;
+  const bool per = P.periodic;
+  for (int t = threadIdx.x; t < RX * RY; t += kBlock) {
+    const int r = t / RX, c = t - r * RX;
+    int gy = y0 - w + r, gx = x0 - w + c;
+    uint8_t m = 0;
+    if (per) {
+      gy = ((gy % ny) + ny) % ny;
+      gx = ((gx % nx) + nx) % nx;
+      m = rawg[gy * nx + gx];
+    } else if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+      m = rawg[gy * nx + gx];
+    }
+    raw[t] = m;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < RY * kSTX; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    uint8_t o = 0;
+    for (int d = 0; d <= 2 * w; ++d) o |= raw[r * RX + c + d];
+    mid[t] = o;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSTY * kSTX; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    uint8_t o = 0;
+    for (int d = 0; d <= 2 * w; ++d) o |= mid[(r + d) * kSTX + c];
+    outt[r * (kSTX + 1) + c] = o;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) mask[gy * nx + gx] = o;
+    if (qT != nullptr && gy < ny && gx < nx) {
+      T a, b, cc, d;
+      load4(q, N, gy * nx + gx, a, b, cc, d);
+      if (enforce_first) enforce(P, a, b, cc, d);
+      const int oo = r * (kSTX + 1) + c;
+      st[oo] = a;
+      st[kSTY * (kSTX + 1) + oo] = b;
+      st[2 * kSTY * (kSTX + 1) + oo] = cc;
+      st[3 * kSTY * (kSTX + 1) + oo] = d;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSTY * kSTX; t += kBlock) {
+    const int c = t / kSTY, r = t - c * kSTY;  // consecutive threads -> consecutive y
+    const int gy = y0 + r, gx = x0 + c;
+    if (gy < ny && gx < nx) {
+      const int oo = r * (kSTX + 1) + c;
+      maskT[(size_t)gx * ny + gy] = outt[oo];
+      if (qT != nullptr) {
+        const size_t g = (size_t)gx * ny + gy;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qT[k * (size_t)N + g] = st[k * kSTY * (kSTX + 1) + oo];
+      }
     }
   }
 }
@@ -796,25 +935,25 @@ template <class T>
 __device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, const T* V, const T* Tt, int ly, int lx,
                                             T out[4]) {
   auto at = [&](const T* a, int dy, int dx) { return a[(ly + dy) * kVS + lx + dx]; };
-  auto ux = [&](int a, int b) { return (at(U, a, b + 1) - at(U, a, b - 1)) / P.two_dx; };
-  auto uy = [&](int a, int b) { return (at(U, a + 1, b) - at(U, a - 1, b)) / P.two_dy; };
-  auto vx = [&](int a, int b) { return (at(V, a, b + 1) - at(V, a, b - 1)) / P.two_dx; };
-  auto vy = [&](int a, int b) { return (at(V, a + 1, b) - at(V, a - 1, b)) / P.two_dy; };
+  auto ux = [&](int a, int b) { return dv(at(U, a, b + 1) - at(U, a, b - 1), P.two_dx); };
+  auto uy = [&](int a, int b) { return dv(at(U, a + 1, b) - at(U, a - 1, b), P.two_dy); };
+  auto vx = [&](int a, int b) { return dv(at(V, a, b + 1) - at(V, a, b - 1), P.two_dx); };
+  auto vy = [&](int a, int b) { return dv(at(V, a + 1, b) - at(V, a - 1, b), P.two_dy); };
   auto txx = [&](int a, int b) { return P.two_mu * ux(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
   auto tyy = [&](int a, int b) { return P.two_mu * vy(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
   auto txy = [&](int a, int b) { return P.mu * (uy(a, b) + vx(a, b)); };
   auto ex = [&](int a, int b) {
-    const T tx = (at(Tt, a, b + 1) - at(Tt, a, b - 1)) / P.two_dx;
+    const T tx = dv(at(Tt, a, b + 1) - at(Tt, a, b - 1), P.two_dx);
     return (at(U, a, b) * txx(a, b) + at(V, a, b) * txy(a, b)) + P.kcond * tx;
   };
   auto ey = [&](int a, int b) {
-    const T ty = (at(Tt, a + 1, b) - at(Tt, a - 1, b)) / P.two_dy;
+    const T ty = dv(at(Tt, a + 1, b) - at(Tt, a - 1, b), P.two_dy);
     return (at(U, a, b) * txy(a, b) + at(V, a, b) * tyy(a, b)) + P.kcond * ty;
   };
   out[0] = T(0);
-  out[1] = (txx(0, 1) - txx(0, -1)) / P.two_dx + (txy(1, 0) - txy(-1, 0)) / P.two_dy;
-  out[2] = (txy(0, 1) - txy(0, -1)) / P.two_dx + (tyy(1, 0) - tyy(-1, 0)) / P.two_dy;
-  out[3] = (ex(0, 1) - ex(0, -1)) / P.two_dx + (ey(1, 0) - ey(-1, 0)) / P.two_dy;
+  out[1] = dv(txx(0, 1) - txx(0, -1), P.two_dx) + dv(txy(1, 0) - txy(-1, 0), P.two_dy);
+  out[2] = dv(txy(0, 1) - txy(0, -1), P.two_dx) + dv(tyy(1, 0) - tyy(-1, 0), P.two_dy);
+  out[3] = dv(ex(0, 1) - ex(0, -1), P.two_dx) + dv(ey(1, 0) - ey(-1, 0), P.two_dy);
 }
 
 // Primitives u, v, p/(rho R) of the 20x20 tile around a 16x16 block
@@ -829,7 +968,7 @@ __device__ __forceinline__ void viscous_tile(const Params<T>& P, const T* __rest
     const Prim<T> pr = prim_at(P, q, N, gy * nx + gx, 0);
     U[ly * kVS + lx] = pr.u;
     V[ly * kVS + lx] = pr.v;
-    Tt[ly * kVS + lx] = pr.p / (pr.rho * P.rgas);
+    Tt[ly * kVS + lx] = dv(pr.p, pr.rho * P.rgas);
   }
 }
 
@@ -964,41 +1103,98 @@ __device__ __forceinline__ T lap_at(const Params<T>& P, const T* __restrict__ a,
     yl = clampi(y - 1, 0, ny - 1); yr = clampi(y + 1, 0, ny - 1);
   }
   const T c = a[y * nx + x];
-  const T lx = ((a[y * nx + xr] - T(2.0) * c) + a[y * nx + xl]) / P.dx2;
-  const T ly = ((a[yr * nx + x] - T(2.0) * c) + a[yl * nx + x]) / P.dy2;
+  const T lx = dv((a[y * nx + xr] - T(2.0) * c) + a[y * nx + xl], P.dx2);
+  const T ly = dv((a[yr * nx + x] - T(2.0) * c) + a[yl * nx + x], P.dy2);
   return lx + ly;
 }
 
+// 16x16 output tile per block; per component the state on the tile plus a
+// 2-node halo and its Laplacian on the tile plus a 1-node halo go through
+// LDS.  Edge padding (non-periodic) means the Laplacian at an out-of-grid
+// neighbour is the Laplacian at the clamped node, so every halo position
+// evaluates at its wrapped or clamped node.  Guard nodes whose source lies
+// outside the tile take the direct path.
 template <class T>
 __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict__ q, const uint8_t* __restrict__ active,
-                                               int guard, int invert, T* __restrict__ out, int bx, int by) {
+                                        int guard, int invert, T* __restrict__ out, int bx, int by) {
+  __shared__ T sq[20][21];
+  __shared__ T sl[18][19];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
-  if (x >= nx || y >= ny) return;
+  const int x0 = bx * 16, y0 = by * 16;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool per = P.periodic;
+  auto gmap = [&](int g, int n) { return per ? ((g % n) + n) % n : clampi(g, 0, n - 1); };
   int cx = x, cy = y;
   if (guard && P.guard > 0) {
     cx = clampi(x, P.guard, nx - 1 - P.guard);
     cy = clampi(y, P.guard, ny - 1 - P.guard);
   }
-  int xl, xr, yl, yr;
-  if (P.periodic) {
-    xl = wrap(cx - 1, nx); xr = wrap(cx + 1, nx); yl = wrap(cy - 1, ny); yr = wrap(cy + 1, ny);
-  } else {
-    xl = clampi(cx - 1, 0, nx - 1); xr = clampi(cx + 1, 0, nx - 1);
-    yl = clampi(cy - 1, 0, ny - 1); yr = clampi(cy + 1, 0, ny - 1);
-  }
-  const bool act = (active[cy * nx + cx] != 0) != (invert != 0);
+  const bool in = x < nx && y < ny;
+  const bool tiled = cx >= x0 && cx < x0 + 16 && cy >= y0 && cy < y0 + 16;
+  const bool act = in && ((active[gmap(cy, ny) * nx + gmap(cx, nx)] != 0) != (invert != 0));
+  // tile coordinate of the node a tile position evaluates at (wrapped: the
+  // position itself; clamped: the clamped node)
+  auto tcx = [&](int g) { return per ? g - (x0 - 2) : clampi(g, 0, nx - 1) - (x0 - 2); };
+  auto tcy = [&](int g) { return per ? g - (y0 - 2) : clampi(g, 0, ny - 1) - (y0 - 2); };
   T o[4];
-#pragma unroll
+#pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     const T* a = q + (size_t)c * N;
-    const T Lc = lap_at(P, a, cy, cx);
-    const T lx = ((lap_at(P, a, cy, xr) - T(2.0) * Lc) + lap_at(P, a, cy, xl)) / P.dx2;
-    const T ly = ((lap_at(P, a, yr, cx) - T(2.0) * Lc) + lap_at(P, a, yl, cx)) / P.dy2;
-    const T bih = lx + ly;
-    const T coef = act ? P.hv_coef[c] : T(0);
-    o[c] = a[cy * nx + cx] - (coef * bih);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 400; i += kBlock) {
+      const int ly = i / 20, lx = i - ly * 20;
+      sq[ly][lx] = a[gmap(y0 - 2 + ly, ny) * nx + gmap(x0 - 2 + lx, nx)];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 324; i += kBlock) {
+      const int ly = i / 18, lx = i - ly * 18;
+      const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
+      const int ey = tcy(gy), ex = tcx(gx);  // node evaluated, tile coordinates
+      const int exl = tcx(per ? gx - 1 : clampi(gx, 0, nx - 1) - 1), exr = tcx(per ? gx + 1 : clampi(gx, 0, nx - 1) + 1);
+      const int eyl = tcy(per ? gy - 1 : clampi(gy, 0, ny - 1) - 1), eyr = tcy(per ? gy + 1 : clampi(gy, 0, ny - 1) + 1);
+      const T cc = sq[ey][ex];
+      const T lx_ = dv((sq[ey][exr] - T(2.0) * cc) + sq[ey][exl], P.dx2);
+      const T ly_ = dv((sq[eyr][ex] - T(2.0) * cc) + sq[eyl][ex], P.dy2);
+      sl[ly][lx] = lx_ + ly_;
+    }
+    __syncthreads();
+    if (in) {
+      T bih, ac;
+      if (tiled) {
+        const int lcx = cx - x0 + 1, lcy = cy - y0 + 1;  // Laplacian tile coordinates
+        // neighbours of the (clamped) node: positions whose evaluation node is the neighbour
+        int lxl, lxr, lyl, lyr;
+        if (per) {
+          lxl = lcx - 1; lxr = lcx + 1; lyl = lcy - 1; lyr = lcy + 1;
+        } else {
+          lxl = clampi(cx - 1, 0, nx - 1) - x0 + 1; lxr = clampi(cx + 1, 0, nx - 1) - x0 + 1;
+          lyl = clampi(cy - 1, 0, ny - 1) - y0 + 1; lyr = clampi(cy + 1, 0, ny - 1) - y0 + 1;
+        }
+        const T Lc = sl[lcy][lcx];
+        const T lx_ = dv((sl[lcy][lxr] - T(2.0) * Lc) + sl[lcy][lxl], P.dx2);
+        const T ly_ = dv((sl[lyr][lcx] - T(2.0) * Lc) + sl[lyl][lcx], P.dy2);
+        bih = lx_ + ly_;
+        ac = sq[cy - y0 + 2][cx - x0 + 2];
+      } else {
+        int xl, xr, yl, yr;
+        if (per) {
+          xl = wrap(cx - 1, nx); xr = wrap(cx + 1, nx); yl = wrap(cy - 1, ny); yr = wrap(cy + 1, ny);
+        } else {
+          xl = clampi(cx - 1, 0, nx - 1); xr = clampi(cx + 1, 0, nx - 1);
+          yl = clampi(cy - 1, 0, ny - 1); yr = clampi(cy + 1, 0, ny - 1);
+        }
+        const T Lc = lap_at(P, a, cy, cx);
+        const T lx_ = dv((lap_at(P, a, cy, xr) - T(2.0) * Lc) + lap_at(P, a, cy, xl), P.dx2);
+        const T ly_ = dv((lap_at(P, a, yr, cx) - T(2.0) * Lc) + lap_at(P, a, yl, cx), P.dy2);
+        bih = lx_ + ly_;
+        ac = a[cy * nx + cx];
+      }
+      const T coef = act ? P.hv_coef[c] : T(0);
+      o[c] = ac - (coef * bih);
+    }
   }
+  if (!in) return;
   enforce(P, o[0], o[1], o[2], o[3]);
 #pragma unroll
   for (int c = 0; c < 4; ++c) out[(size_t)c * N + y * nx + x] = o[c];

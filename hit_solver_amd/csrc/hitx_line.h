@@ -30,9 +30,12 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS bytes of one wave's slice for a line of n nodes.
+// LDS bytes of one wave's slice for a line of n nodes: q, f and the WENO
+// fluxes with kLineHalo halo entries on each side (wrapped or clamped copies,
+// so stencils index without wrap or clamp), the work list, the mask (halo 2).
+constexpr int kLineHalo = 4;
 __host__ __device__ inline size_t line_wave_lds(int n, int esize) {
-  return (size_t)(3 * (n + 1)) * esize + (size_t)(n + 1) * 2 + (size_t)n + 16;
+  return (size_t)(3 * (n + 1 + 2 * kLineHalo)) * esize + (size_t)(n + 1) * 2 + (size_t)(n + 4) + 16;
 }
 
 // b^e for 0 <= e < 2^10 by squaring (uniform or per-lane e).
@@ -61,13 +64,14 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
                                           const T* __restrict__ ql, const uint8_t* __restrict__ ml,
                                           const double* __restrict__ scal, T* __restrict__ out, int line, int nl,
                                           unsigned char* slice) {
-  const int n = 64 * L, S = n + 1;
+  const int n = 64 * L, S = n + 1 + 2 * kLineHalo;
   const int lane = threadIdx.x & 63;
-  T* qs = reinterpret_cast<T*>(slice);
+  // halo-padded rows: entry i of the line at [i + kLineHalo]
+  T* qs = reinterpret_cast<T*>(slice) + kLineHalo;
   T* fs = qs + S;
   T* ws = fs + S;
-  uint16_t* wl = reinterpret_cast<uint16_t*>(ws + S);
-  uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + S);
+  uint16_t* wl = reinterpret_cast<uint16_t*>(ws - kLineHalo + S);
+  uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + n + 1) + 2;  // mask, halo 2
   const size_t NL = (size_t)nl * n;
   const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
   const T h = axis == 0 ? P.dx : P.dy;
@@ -98,23 +102,26 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     Mk[lane * L + j] = ml[g0 + j];
   }
   wave_sync();
+  if (lane < 2) {  // mask halo: wrapped (periodic) or clamped copies
+    Mk[-1 - lane] = PER ? Mk[n - 1 - lane] : Mk[0];
+    Mk[n + lane] = PER ? Mk[lane] : Mk[n - 1];
+  }
+  wave_sync();
   const int nI = PER ? n : n + 1;
-  auto iflag = [&](int k) -> bool {
-    if (PER) {
-      k = wrap(k, n);
-      return (Mk[k] | Mk[wrap(k + 1, n)]) != 0;
-    }
-    k = clampi(k, 0, n);
-    return (Mk[clampi(k - 1, 0, n - 1)] | Mk[clampi(k, 0, n - 1)]) != 0;
-  };
-  auto nidx = [&](int j) -> int { return PER ? wrap(j, n) : clampi(j, 0, n - 1); };
+  // interface flag: periodic k = x_{k+1/2} (nodes k, k+1); non-periodic
+  // k = x_{k-1/2} (nodes k-1, k); k in [-1, nI] reads the mask halo
+  auto iflag = [&](int k) -> bool { return PER ? (Mk[k] | Mk[k + 1]) != 0 : (Mk[k - 1] | Mk[k]) != 0; };
   // 2. work list: interfaces whose WENO7 flux a flagged interface's
-  //    compactification reads (i-1, i, i+1 of every flagged i); interface k
-  //    is periodic x_{k+1/2} (0..n-1) or non-periodic x_{k-1/2} (0..n)
+  //    compactification reads (i-1, i, i+1 of every flagged i)
   int nw = 0;
   for (int k0 = 0; k0 < nI; k0 += 64) {
     const int k = k0 + lane;
-    const bool need = k < nI && (P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1));
+    bool need = false;
+    if (k < nI) {
+      // non-periodic ends: the neighbours of interface 0 and n are clamped
+      const int km = PER ? k - 1 : (k > 0 ? k - 1 : 0), kp = PER ? k + 1 : (k < n ? k + 1 : n);
+      need = P.weno_only || iflag(km) || iflag(k) || iflag(kp);
+    }
     const unsigned long long bal = __ballot(need);
     if (need) wl[nw + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)k;
     nw += __popcll(bal);
@@ -132,6 +139,14 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       fs[lane * L + j] = c == 0 ? f[0][j] : (c == 1 ? f[1][j] : (c == 2 ? f[2][j] : f[3][j]));
     }
     wave_sync();
+    if (lane < kLineHalo) {  // halos: wrapped or clamped copies of the line ends
+      const int e = lane + 1;
+      qs[-e] = PER ? qs[n - e] : qs[0];
+      fs[-e] = PER ? fs[n - e] : fs[0];
+      qs[n - 1 + e] = PER ? qs[e - 1] : qs[n - 1];
+      fs[n - 1 + e] = PER ? fs[e - 1] : fs[n - 1];
+    }
+    wave_sync();
     // WENO7 of the Lax-Friedrichs split on the listed interfaces
     for (int t = lane; t < nw; t += 64) {
       const int k = wl[t];
@@ -139,8 +154,8 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       T fp[7], fm[7];
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        const int ip = nidx(il - 3 + j);  // f+ stencil i-3 .. i+3
-        const int im = nidx(il + 4 - j);  // f- stencil i+4 .. i-2
+        const int ip = il - 3 + j;  // f+ stencil i-3 .. i+3
+        const int im = il + 4 - j;  // f- stencil i+4 .. i-2
         fp[j] = T(0.5) * (fs[ip] + alpha * qs[ip]);
         fm[j] = T(0.5) * (fs[im] - alpha * qs[im]);
       }
@@ -148,25 +163,25 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
               weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
     }
     wave_sync();
+    if (lane == 0) {  // WENO flux halo: wrapped (periodic) or clamped (interfaces 0..n)
+      ws[-1] = PER ? ws[n - 1] : ws[0];
+      ws[nI] = PER ? ws[0] : ws[n];
+    }
+    wave_sync();
     // hybrid interface flux of interfaces kb .. kb+L (one beyond the lane's own)
     auto iface = [&](int k) -> T {
-      if (P.weno_only || iflag(k)) {
-        const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
-        const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
-        return (K.alpha_c * ws[kr] + ws[k]) + K.alpha_c * ws[kl];
-      }
+      if (P.weno_only || iflag(k)) return (K.alpha_c * ws[k + 1] + ws[k]) + K.alpha_c * ws[k - 1];
       const int il = PER ? k : k - 1;
-      return (c3 * (fs[nidx(il + 3)] + fs[nidx(il - 2)]) + c2 * (fs[nidx(il + 2)] + fs[nidx(il - 1)])) +
-             c1 * (fs[nidx(il + 1)] + fs[nidx(il)]);
+      return (c3 * (fs[il + 3] + fs[il - 2]) + c2 * (fs[il + 2] + fs[il - 1])) + c1 * (fs[il + 1] + fs[il]);
     };
     const int kb = PER ? lane * L - 1 : lane * L;
     T x[L];
     {
-      T Fprev = iface(PER ? wrap(kb, n) : kb);
+      T Fprev = iface(PER && kb < 0 ? n - 1 : kb);
 #pragma unroll
       for (int j = 0; j < L; ++j) {
-        const T Fn = iface(PER ? wrap(kb + j + 1, n) : kb + j + 1);
-        x[j] = (Fn - Fprev) / h;  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
+        const T Fn = iface(kb + j + 1);
+        x[j] = dv(Fn - Fprev, h);  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
         Fprev = Fn;
       }
     }
@@ -301,12 +316,24 @@ __device__ __forceinline__ void line_wave_any(const Params<T>& P, const LineK<T>
 
 constexpr int kLineWaves = 2;  // lines (wavefronts) per block
 
+// Periodic FP32 lines fit 4 wavefronts per SIMD (<= 128 VGPRs) without
+// spilling; the others keep their registers (a forced bound would spill).
 template <class T, bool PER>
-__global__ void __launch_bounds__(64 * kLineWaves) k_line_wave(Params<T> P, LineK<T> K, int axis,
-                                                              int enforced_input, const T* __restrict__ ql,
-                                                              const uint8_t* __restrict__ ml,
-                                                              const double* __restrict__ scal, T* __restrict__ out,
-                                                              int nl, int slice_bytes) {
+struct LineWaveOcc {
+  static constexpr int waves = 1;
+};
+template <>
+struct LineWaveOcc<float, true> {
+  static constexpr int waves = 4;
+};
+
+#define HITX_LINE_WAVE_ATTR(T, PER) __attribute__((amdgpu_waves_per_eu(LineWaveOcc<T, PER>::waves)))
+
+template <class T, bool PER>
+__global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
+    k_line_wave(Params<T> P, LineK<T> K, int axis, int enforced_input, const T* __restrict__ ql,
+                const uint8_t* __restrict__ ml, const double* __restrict__ scal, T* __restrict__ out, int nl,
+                int slice_bytes) {
   extern __shared__ __align__(16) unsigned char wsm[];
   const int w = threadIdx.x >> 6;
   const int line = blockIdx.x * kLineWaves + w;

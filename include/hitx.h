@@ -222,13 +222,14 @@ size_t hitx_batch_workspace_bytes(const hitx_batch* batch);
 /* Carve the instances' buffers from ws and upload the instance table (blocking). */
 int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
 /* Launch shapes of every instance (no reference counterpart; the batch
- * autotuner's knobs): rb rows per wave of the stage reductions (2..64), rbp
- * rows per wave of the forcing projection (8..256), and the line-sweep form
- * (0 auto, 1 block, 2 one wavefront per line where the line allows it).
- * Changes the summation order of the reductions and the rounding of the line
- * solve only.  Blocking when bound (re-uploads the instance table); no work
- * may be in flight on the batch. */
-int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_form);
+ * autotuner's knobs): rb rows per wave of the stage reductions and of the
+ * split sensor (2..64), rbp rows per wave of the forcing projection (8..256),
+ * the line-sweep form (0 auto, 1 block, 2 one wavefront per line where the
+ * line allows it) and the sensor form (0 auto, 1 fused tile, 2 split: the
+ * undilated mask by column walk, then the dilation).  Changes the summation
+ * order of the reductions and the rounding of the line solve only.  Blocking
+ * when bound (re-uploads the instance table); no work may be in flight. */
+int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_form, int32_t sensor_form);
 /* Device pointer of one of instance i's buffers inside the bound workspace. */
 int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
 /* Fixed dt of every instance from its current state (Riemann loop). */

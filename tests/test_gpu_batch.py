@@ -46,6 +46,24 @@ def test_riemann_batch_matches_single_instances():
         np.testing.assert_array_equal(host(b.q[i]), host(s.q))
 
 
+def test_riemann_batch_split_sensor_bitexact():
+    """The split sensor (undilated mask by column walk, then the dilation)
+    gives the fused tile's mask: with the single plan's reduction tiles the
+    batch still reproduces single instances bit for bit."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannConfig6, RiemannSolver
+
+    cfgs = [RiemannConfig3(nx=64, ny=48, shear_threshold=None, progress_every=10**9),
+            RiemannConfig6(nx=40, ny=72, shear_threshold=None, progress_every=10**9)]
+    b = BatchRunner(cfgs, max_steps=5)
+    b.tune(2, 8, 0, 2)
+    b.run()
+    for i, c in enumerate(cfgs):
+        s = RiemannSolver(c)
+        s.run(max_steps=5)
+        np.testing.assert_array_equal(host(b.q[i]), host(s.q))
+
+
 def test_hit_batch_matches_single_instances():
     from hit_solver_amd.batch import BatchRunner
     from hit_solver_amd.hit2d import HIT2DConfig, HITSolver
@@ -127,8 +145,8 @@ def test_hit_batch_tuned_tiles_agree():
     cfgs = [HIT2DConfig(nx=64, ny=48, target_mach=0.25, forcing_seed=3, **kw),
             HIT2DConfig(nx=40, ny=72, target_mach=0.4, forcing_seed=4, **kw)]
     a, b = BatchRunner(cfgs), BatchRunner(cfgs)
-    a.tune(2, 8, 1)
-    b.tune(8, 32, 2)
+    a.tune(2, 8, 1, 1)
+    b.tune(8, 32, 2, 2)
     a.run()
     b.run()
     for i in range(2):
