@@ -56,6 +56,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hit", action="store_true", help="skip the forced-HIT measurements")
     ap.add_argument("--hit-steps", type=int, default=60)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--serving-requests", type=int, default=24)
+    ap.add_argument("--serving-rate", type=float, default=4.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--serving-steps", type=int, nargs=2, default=(200, 1000), metavar=("MIN", "MAX"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run 3 plain device copies of this many MiB first (scale for rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -165,6 +169,28 @@ def hit_measurements(steps):
     return res
 
 
+def serving_measurement(count, rate, steps):
+    """Poisson stream of mixed requests (Riemann C3, C3 with the split at 0.8,
+    C6, forced HIT; grids 128^2..1024^2; FP32/FP64) through the serving loop."""
+    from hit_solver_amd.serving import ServingEngine, poisson_stream
+
+    reqs = poisson_stream(count, rate, seed=2024, steps=tuple(steps))
+    eng = ServingEngine(round_steps=10)
+    res, summary = eng.run(reqs)
+    cells = sum(r.n * r.n * r.steps for r in reqs)
+    summary.update({
+        "unit_latency": "s", "arrival_rate_per_s": rate, "seed": 2024,
+        "stream": f"{count} requests, kinds uniform over c3/c3_08/c6/hit, n uniform in 128..1024 step 64, "
+                  f"fp32/fp64 uniform, {steps[0]}-{steps[1]} steps each (bounded sample of the 200-5000 step "
+                  f"stream), {cells / 1e9:.2f} G cell-updates in total",
+        "form": "continuous batching: per-class batches on their own streams, rounds of 10 steps as HIP graphs, "
+                "HBM-budget admission (80% of free device memory)"})
+    for k in ("p50_s", "p99_s", "mcell_updates_per_s"):
+        if summary.get(k) is not None:
+            summary[k] = round(summary[k], 4)
+    return summary
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -248,6 +274,9 @@ def main():
         del s
         torch.cuda.empty_cache()
         out["hit"] = hit_measurements(args.hit_steps)
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        out["serving"] = serving_measurement(args.serving_requests, args.serving_rate, args.serving_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nx, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -11,8 +11,9 @@ step needs no host work and no synchronisation.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
-import math
+import dataclasses
 
 import numpy as np
 import torch
@@ -103,30 +104,40 @@ def _lib_kpar():
 
 class BatchRunner:
     """All instances share periodicity, dtype, scheme, sensor width and the drag
-    on/off setting (one kernel instantiation per batch)."""
+    on/off setting (one kernel instantiation per batch).
 
-    def __init__(self, configs, max_steps=None):
+    max_steps: None, one cap for every instance, or a list (one per instance).
+    init: True, or a list of booleans: instances left uninitialised are to be
+    filled by copy_from (moving a running instance between batches).
+    stream: the torch stream every launch of this batch goes to (default: the
+    current stream at each call)."""
+
+    def __init__(self, configs, max_steps=None, init=True, stream=None):
         if not configs:
             raise ValueError("empty batch")
         self.configs = list(configs)
+        n = len(self.configs)
+        self.stream = stream
         self.hit = isinstance(self.configs[0], HIT2DConfig)
-        specs, kpars, raw = [], [], []
-        self.rngs = []
-        for c in self.configs:
+        caps = list(max_steps) if isinstance(max_steps, (list, tuple)) else [max_steps] * n
+        todo = list(init) if isinstance(init, (list, tuple)) else [bool(init)] * n
+        if len(caps) != n or len(todo) != n:
+            raise ValueError("max_steps and init lists need one entry per instance")
+        specs, kpars, self.domains = [], [], []
+        for c, cap in zip(self.configs, caps):
             if self.hit != isinstance(c, HIT2DConfig):
                 raise ValueError("a batch holds either HIT or Riemann instances")
             if self.hit:
-                spec, k, domain, eq = hit_spec_kpar(c)
-                rng = np.random.default_rng(c.forcing_seed)
-                raw.append(initialize_hit_2d(c, domain, eq, rng))
-                self.rngs.append(pcg64_words(np.random.default_rng(c.forcing_seed + 1).bit_generator))
+                if cap is not None:
+                    c = dataclasses.replace(c, max_steps=int(cap))
+                spec, k, domain, _ = hit_spec_kpar(c)
+                self.domains.append(domain)
             else:
-                spec, k = riemann_spec_kpar(c, max_steps)
-                raw.append(None)
+                spec, k = riemann_spec_kpar(c, cap)
+                self.domains.append(c.domain)
             specs.append(spec)
             kpars.append(k)
-        self.specs = specs
-        n = len(specs)
+        self.specs, self.kpars = specs, kpars
         descs = (_lib.HitxDesc * n)()
         keep = []
         for i, sp in enumerate(specs):
@@ -139,30 +150,70 @@ class BatchRunner:
         self.handle = h
         self.n = n
         nbytes = int(lib().hitx_batch_workspace_bytes(h))
-        self.ws = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+        with self._ctx():
+            self.ws = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+        self._sync()
         check(lib().hitx_batch_bind(h, ctypes.c_void_p(self.ws.data_ptr()), nbytes))
         self.q = [self._view(i, BUF_STATE, sp.dtype, (4, sp.ny, sp.nx)) for i, sp in enumerate(specs)]
         self.ctl_views = [self._view(i, BUF_CTL, torch.float64, (32,)) for i in range(n)]
         self.noise = [self._view(i, BUF_NOISE, torch.float64, (sp.ny, sp.nx)) for i, sp in enumerate(specs)] \
             if self.hit else None
         self.rng_state = [self._view(i, BUF_RNG, torch.int64, (4,)) for i in range(n)] if self.hit else None
-        # initial states: enforce (+ outflow guard), as the single-instance drivers
-        for i, (c, sp) in enumerate(zip(self.configs, specs)):
-            q0 = torch.from_numpy(raw[i]).to("cuda", sp.dtype) if self.hit else c.initial_state()
-            get_plan(sp).enforce(q0, apply_guard=not self.hit, out=self.q[i])
-            ctl = torch.zeros(32, dtype=torch.float64)
-            ctl[C_ACTIVE] = 1.0
-            ctl[C_TFINAL] = c.tfinal
-            ctl[C_MAXSTEPS] = kpars[i][19]
-            ctl[C_POWER] = kpars[i][11]
-            ctl[C_ALPHA] = 1.0
-            self.ctl_views[i].copy_(ctl)
+        self.phi = [self._view(i, BUF_PHI, torch.float64, (max(1, len(sp.force_modes) // 2), 2)) for i, sp in
+                    enumerate(specs)] if self.hit else None
+        with self._ctx():
+            for i in range(n):
+                if todo[i]:
+                    self._init_instance(i)
+            if not self.hit:
+                self.fixed_dt()
+        self._graph = None
+        self._sync()
+
+    def _ctx(self):
+        return torch.cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
+
+    def _sync(self):
+        if self.stream is not None:
+            self.stream.synchronize()
+        else:
+            torch.cuda.synchronize()
+
+    def _init_instance(self, i):
+        """Initial state (enforce, + outflow guard), control scalars and noise
+        stream of instance i, as the single-instance drivers set them up."""
+        c, sp, k = self.configs[i], self.specs[i], self.kpars[i]
+        if self.hit:
+            domain = self.domains[i]
+            raw = initialize_hit_2d(c, domain, _make_equation(c), np.random.default_rng(c.forcing_seed))
+            q0 = torch.from_numpy(raw).to("cuda", sp.dtype)
+        else:
+            q0 = c.initial_state()
+        get_plan(sp).enforce(q0, apply_guard=not self.hit, out=self.q[i])
+        ctl = torch.zeros(32, dtype=torch.float64)
+        ctl[C_ACTIVE] = 1.0
+        ctl[C_TFINAL] = c.tfinal
+        ctl[C_MAXSTEPS] = k[19]
+        ctl[C_POWER] = k[11]
+        ctl[C_ALPHA] = 1.0
+        self.ctl_views[i].copy_(ctl)
+        if self.hit:
+            w = pcg64_words(np.random.default_rng(c.forcing_seed + 1).bit_generator)
+            self.rng_state[i].copy_(torch.from_numpy(w.view(np.int64).copy()))
+
+    def copy_from(self, i, other: "BatchRunner", j):
+        """Continue other's instance j as this batch's instance i (same
+        configuration): state, control scalars, OU coefficients, noise stream."""
+        with self._ctx():
+            self.q[i].copy_(other.q[j])
+            self.ctl_views[i].copy_(other.ctl_views[j])
             if self.hit:
-                self.rng_state[i].copy_(torch.from_numpy(self.rngs[i].view(np.int64).copy()))
-        if not self.hit:
-            check(lib().hitx_batch_fixed_dt(h, self._stream()))
-        self._pinned = None
-        torch.cuda.synchronize()
+                self.phi[i].copy_(other.phi[j])
+                self.rng_state[i].copy_(other.rng_state[j])
+
+    def fixed_dt(self):
+        """Fixed dt of the Riemann instances that have none yet."""
+        check(lib().hitx_batch_fixed_dt(self.handle, self._stream()))
 
     def __del__(self):
         try:
@@ -172,9 +223,9 @@ class BatchRunner:
         except Exception:
             pass
 
-    @staticmethod
-    def _stream():
-        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    def _stream(self):
+        st = self.stream if self.stream is not None else torch.cuda.current_stream()
+        return ctypes.c_void_p(st.cuda_stream)
 
     def _view(self, i, which, dtype, shape):
         p = ctypes.c_void_p()
@@ -195,19 +246,25 @@ class BatchRunner:
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=s):
-            for _ in range(steps):
-                self.step()
+        keep, self.stream = self.stream, None
+        try:
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(steps):
+                    self.step()
+        finally:
+            self.stream = keep
         torch.cuda.synchronize()
         self._graph, self._graph_steps = g, steps
         return g
 
     def replay(self, n: int = 1):
-        """Replay the captured graph n times (n * captured steps)."""
+        """Replay the captured graph n times (n * captured steps), on this
+        batch's stream."""
         if getattr(self, "_graph", None) is None:
             raise RuntimeError("capture() first")
-        for _ in range(n):
-            self._graph.replay()
+        with self._ctx():
+            for _ in range(n):
+                self._graph.replay()
 
     def tune(self, rb: int, rbp: int, line_form: int = 0, sensor_form: int = 0):
         """Launch shapes of every instance: rows per wave of the stage
@@ -247,7 +304,9 @@ class BatchRunner:
 
     def control(self):
         """Control scalars of all instances (host numpy, [n, 32]); synchronises."""
-        return torch.stack(self.ctl_views).cpu().numpy()
+        with self._ctx():
+            out = torch.stack(self.ctl_views).cpu().numpy()
+        return out
 
     def active(self):
         return self.control()[:, C_ACTIVE] != 0.0

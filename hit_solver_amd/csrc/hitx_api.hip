@@ -40,6 +40,9 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // bounds of the column-walk tile heights a batch may be tuned to
 constexpr int kTuneRbMin = 2, kTuneRbMax = 64, kTuneRbpMin = 8, kTuneRbpMax = 256;
+// a plan's own tile heights: enough waves for the grid alone
+int default_rb(int nx, int ny) { return std::max(2, std::min(32, ((nx + 61) / 62) * ny / 2048)); }
+int default_rbp(int nx, int ny) { return std::max(8, std::min(64, ((nx + 63) / 64) * ny / 1024)); }
 
 // Workspace of one normal-variate request (see hitx_rng.h).
 struct RngLayout {
@@ -484,9 +487,8 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->nblk_red = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 1024);
   pl->nblk_stage = (int)std::min<size_t>((N + kBlock - 1) / kBlock, 512);  // grid-stride reductions
   // column walks: rows per wave so that a plan alone still gives ~2048 waves
-  const int nseg1 = (desc->nx + 61) / 62, nseg0 = (desc->nx + 63) / 64;
-  pl->rb = std::max(2, std::min(32, nseg1 * desc->ny / 2048));
-  pl->rbp = std::max(8, std::min(64, nseg0 * desc->ny / 1024));
+  pl->rb = default_rb(desc->nx, desc->ny);
+  pl->rbp = default_rbp(desc->nx, desc->ny);
   pl->nblk_sr = walk_blocks(desc->nx, desc->ny, pl->rb, 1);
   pl->nblk_proj = walk_blocks(desc->nx, desc->ny, pl->rbp, 0);
   size_t o = 0;
@@ -1099,17 +1101,17 @@ int hitx_batch_buffer(const hitx_batch* b, int32_t i, int32_t which, void** ptr)
 
 int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, int32_t sensor_form) {
   if (!b) return fail(HITX_EINVAL, "null batch");
-  if (rb < kTuneRbMin || rb > kTuneRbMax || rbp < kTuneRbpMin || rbp > kTuneRbpMax)
-    return fail(HITX_EINVAL, "tile heights out of range (rb 2..64, rbp 8..256)");
+  if ((rb != 0 && (rb < kTuneRbMin || rb > kTuneRbMax)) || (rbp != 0 && (rbp < kTuneRbpMin || rbp > kTuneRbpMax)))
+    return fail(HITX_EINVAL, "tile heights out of range (rb 0 or 2..64, rbp 0 or 8..256)");
   if (line_form < 0 || line_form > 2) return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block) or 2 (wave)");
   if (sensor_form < 0 || sensor_form > 2) return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused) or 2 (split)");
   b->line_form = line_form;
   b->sensor_form = sensor_form;
   for (hitx_plan* pl : b->plans) {
-    pl->rb = rb;
-    pl->rbp = rbp;
-    pl->nblk_sr = walk_blocks(pl->d.nx, pl->d.ny, rb, 1);
-    pl->nblk_proj = walk_blocks(pl->d.nx, pl->d.ny, rbp, 0);
+    pl->rb = rb > 0 ? rb : default_rb(pl->d.nx, pl->d.ny);
+    pl->rbp = rbp > 0 ? rbp : default_rbp(pl->d.nx, pl->d.ny);
+    pl->nblk_sr = walk_blocks(pl->d.nx, pl->d.ny, pl->rb, 1);
+    pl->nblk_proj = walk_blocks(pl->d.nx, pl->d.ny, pl->rbp, 0);
   }
   batch_offsets(b);
   if (b->ws) return b->esize == 8 ? batch_bind_t<double>(b, b->ws, false) : batch_bind_t<float>(b, b->ws, false);

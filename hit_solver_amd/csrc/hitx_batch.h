@@ -250,10 +250,12 @@ __global__ void __launch_bounds__(kBlock) b_ctl_begin(const Inst<T>* __restrict_
 }
 
 // Fixed dt of the Riemann loop from the initial state: n = ceil(T/dt0),
-// dt = T/n (OOP/problems/riemann_config3.py:371-378).  One block per instance.
+// dt = T/n (OOP/problems/riemann_config3.py:371-378).  One block per
+// instance; instances that already hold their step count are left alone.
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_fixed_dt(const Inst<T>* __restrict__ tab) {
   const Inst<T>& I = tab[blockIdx.x];
+  if (I.kpar[K_FIXED_DT] == 0.0 || I.ctl[C_NSTEPS] > 0.0 || I.ctl[C_ACTIVE] == 0.0) return;
   __shared__ double r[3];
   reduce_partials<3, 1>(I.part, I.nblk_stage, r);
   __syncthreads();

@@ -223,7 +223,8 @@ size_t hitx_batch_workspace_bytes(const hitx_batch* batch);
 int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
 /* Launch shapes of every instance (no reference counterpart; the batch
  * autotuner's knobs): rb rows per wave of the stage reductions and of the
- * split sensor (2..64), rbp rows per wave of the forcing projection (8..256),
+ * split sensor (2..64), rbp rows per wave of the forcing projection (8..256)
+ * (0 for either: each instance's own default, set by its grid alone),
  * the line-sweep form (0 auto, 1 block, 2 one wavefront per line where the
  * line allows it) and the sensor form (0 auto, 1 fused tile, 2 split: the
  * undilated mask by column walk, then the dilation).  Changes the summation
