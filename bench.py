@@ -42,6 +42,9 @@ HBM_PEAK_GBS = 8000.0
 STEP_BYTES = {"fp64": 614.0, "fp32": 310.0}
 # forced HIT (forcing, drag, cooling, viscous terms; hyperviscosity every 5 steps)
 HIT_STEP_BYTES = {"fp64": 635.0, "fp32": 320.0}
+# HBM bytes per line-sweep launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 512 MiB copy
+# kernel's readings; profiles/k_line_riemann_c3_512_fp64.md), for the workloads that were profiled
+LINE_TRAFFIC = {(512, "fp64"): 17_445_171}
 
 
 def parse():
@@ -57,6 +60,9 @@ def parse():
     ap.add_argument("--no-hit", action="store_true", help="skip the forced-HIT measurements")
     ap.add_argument("--hit-steps", type=int, default=60)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--no-fp32-error", action="store_true", help="skip the FP32-vs-FP64 1000-step error")
+    ap.add_argument("--fp32-steps", type=int, default=1000)
+    ap.add_argument("--save-states", default=None, help="directory for the final FP32/FP64 HIT 512^2 states")
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--serving-rate", type=float, default=4.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--serving-steps", type=int, nargs=2, default=(200, 1000), metavar=("MIN", "MAX"))
@@ -169,6 +175,47 @@ def hit_measurements(steps):
     return res
 
 
+def fp32_error(steps, save_dir=None):
+    """Production forced HIT 512x512 (seed 1234) run `steps` steps in the FP32
+    batched form (FP32 state, FP64 reductions, the batch benchmark's launch
+    shapes) and in FP64; relative L2 of each conserved field."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+
+    c64 = HIT2DConfig.production_512(dtype="fp64", tfinal=1e9, max_steps=steps)
+    c32 = dataclasses.replace(c64, dtype="fp32")
+    out = {"workload": f"forced HIT 512x512 production settings, seed 1234, {steps} steps from the same initial "
+                       "state; FP32 batched form vs FP64 form on the device", "steps": steps, "checkpoints": {}}
+    a, b = BatchRunner([c64]), BatchRunner([c32])
+    b.tune(32, 64, 2, 2)
+    a.capture(10)
+    b.capture(10)
+    names = ("rho", "rho_u", "rho_v", "E")
+    done = 0
+    for ck in sorted({min(steps, x) for x in (10, 100, 500, steps)}):
+        reps = (ck - done) // 10
+        a.replay(reps)
+        b.replay(reps)
+        done += reps * 10
+        torch.cuda.synchronize()
+        q64, q32 = a.q[0].double(), b.q[0].double()
+        err = {nm: float(torch.linalg.vector_norm(q32[k] - q64[k]) / torch.linalg.vector_norm(q64[k]))
+               for k, nm in enumerate(names)}
+        out["checkpoints"][str(done)] = {k: float(f"{v:.3e}") for k, v in err.items()}
+    out["rel_l2_final"] = out["checkpoints"][str(done)]
+    out["max_rel_l2"] = max(out["rel_l2_final"].values())
+    if save_dir:
+        os.makedirs(save_dir, exist_ok=True)
+        np.save(os.path.join(save_dir, f"hit512_fp32_step{done}.npy"), b.q[0].cpu().numpy())
+        np.save(os.path.join(save_dir, f"hit512_fp64_step{done}.npy"), a.q[0].cpu().numpy())
+    return out
+
+
 def serving_measurement(count, rate, steps):
     """Poisson stream of mixed requests (Riemann C3, C3 with the split at 0.8,
     C6, forced HIT; grids 128^2..1024^2; FP32/FP64) through the serving loop."""
@@ -261,7 +308,7 @@ def main():
                    "steps_full_solve": s.n_steps, "dt": s.dt},
         "roofline": {"bound": "hbm", "kernel": "k_line (x and y sweep)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "launches": line_n, "avg_launch_us": round(line_avg_s * 1e6, 2),
+                     "traffic": LINE_TRAFFIC.get((nx, args.dtype)), "launches": line_n, "avg_launch_us": round(line_avg_s * 1e6, 2),
                      "alg_bytes_per_launch": line_bytes,
                      "timing": f"HIP events around each sweep launch on its stream, {args.kernel_steps} eager "
                                "steps before the timed graph replays"},
@@ -274,6 +321,9 @@ def main():
         del s
         torch.cuda.empty_cache()
         out["hit"] = hit_measurements(args.hit_steps)
+    if not args.no_fp32_error:
+        torch.cuda.empty_cache()
+        out["fp32_error"] = fp32_error(args.fp32_steps, args.save_states)
     if not args.no_serving:
         torch.cuda.empty_cache()
         out["serving"] = serving_measurement(args.serving_requests, args.serving_rate, args.serving_steps)

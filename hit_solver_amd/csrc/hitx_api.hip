@@ -724,7 +724,7 @@ int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* w
   const int nb = (J.nch + kRngAttemptThreads - 1) / kRngAttemptThreads;
   k_rng_attempts<<<nb, kRngAttemptThreads, 0, s>>>(J);
   k_rng_scan<<<1, kRngScan, 0, s>>>(J, (flags & HITX_NORMALS_SERIAL) ? 1 : 0);
-  k_rng_scatter<<<nb, kRngAttemptThreads, 0, s>>>(J);
+  k_rng_scatter<<<(J.nch + kRngScatterThreads - 1) / kRngScatterThreads, kRngScatterThreads, 0, s>>>(J);
   k_rng_finish<<<1, 64, 0, s>>>(J);
   LAUNCHCHK();
   return HITX_OK;
@@ -801,6 +801,8 @@ int batch_blocks(const hitx_batch* b, const hitx_plan* pl, int kind) {
     case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
     case OFF_RNG:
       return pl->d.n_force_modes > 0 ? (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads : 0;
+    case OFF_RNGS:
+      return pl->d.n_force_modes > 0 ? (rng_chunks(pl->N) + kRngScatterThreads - 1) / kRngScatterThreads : 0;
     default: return 0;
   }
 }
@@ -937,15 +939,13 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
   if (b->periodic && b->total[OFF_PROJ] > 0) {
     b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
     b_rng_scan<T><<<n, kRngScan, 0, s>>>(tab);
-    b_rng_scatter<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
+    b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, s>>>(tab, offs, n);
     b_rng_finish<T><<<n, 64, 0, s>>>(tab);
     b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, s>>>(tab, offs, n);
     b_fou<T><<<n, kBlock, 0, s>>>(tab);
     b_frows<T><<<b->total[OFF_FROWS], kBlock, 0, s>>>(tab, offs, n);
     b_fsynth<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
     b_falpha<T><<<n, kBlock, 0, s>>>(tab);
-    b_fscale<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
-    b_fstats<T><<<n, kBlock, 0, s>>>(tab);
   }
   LAUNCHCHK();
   for (int st = 0; st < 4; ++st) {

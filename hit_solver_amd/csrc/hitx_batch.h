@@ -45,7 +45,7 @@ struct Inst {
 // offsets kinds (rows of the offsets table, each ninst+1 ints)
 enum {
   OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_LXW, OFF_LYW,
-  OFF_KINDS
+  OFF_RNGS, OFF_KINDS
 };
 
 __device__ __forceinline__ int find_inst(const int* __restrict__ o, int n, int b) {
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ t
   // there is no outflow guard (each node then reads Q only at itself)
   T* out = (st == 2 && I.ctl[C_HVNOW] == 0.0 && I.P.guard == 0) ? I.Q : stage_out(I, st);
   update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
-                 out, bx, by);
+                 out, bx, by, (T)I.ctl[C_SCALE], (T)I.ctl[C_ALPHA]);
 }
 
 // After the stages: hyperviscosity S -> Q with the fresh mask, or the S -> Q
@@ -316,11 +316,12 @@ __global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict
 }
 
 template <class T>
-__global__ void __launch_bounds__(kRngAttemptThreads) b_rng_scatter(const Inst<T>* __restrict__ tab,
+__global__ void __launch_bounds__(kRngScatterThreads) b_rng_scatter(const Inst<T>* __restrict__ tab,
                                                                     const int* __restrict__ offs, int ninst) {
-  HITX_BATCH_INST(OFF_RNG)
+  __shared__ double lds[kRngScatterThreads / 64][64 * kRngChunk];
+  HITX_BATCH_INST(OFF_RNGS)
   if (I.kpar[K_FORCE] == 0.0) return;
-  rng_scatter_body(I.R, lb * kRngAttemptThreads + threadIdx.x);
+  rng_scatter_body(I.R, lb * kRngScatterThreads + threadIdx.x, lds[threadIdx.x >> 6]);
 }
 
 template <class T>
@@ -364,7 +365,10 @@ __global__ void __launch_bounds__(kBlock) b_frows(const Inst<T>* __restrict__ ta
   frows_body<T>(I.P, I.fhat, I.frows, lb * kBlock + threadIdx.x);
 }
 
-// fx, fy from the shell coefficients, with the primitives of Q for the power.
+// Raw fx, fy from the shell coefficients (the rescale by scale*alpha is
+// applied where the stage update reads them), with the sums the rescale and
+// the statistics need: sum(fx^2+fy^2), sum(rho*(fx*u+fy*v)) at Q, sum(fx^2),
+// sum(fy^2), sum(fx*fy).
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst) {
@@ -373,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
   const Params<T>& P = I.P;
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const double invN = 1.0 / ((double)nx * (double)ny);
-  double acc[2] = {0.0, 0.0};
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int idx = lb * kBlock + threadIdx.x; idx < N; idx += I.nblk_stage * kBlock) {
     const int y = idx / nx, x = idx - y * nx;
     double ax, ay;
@@ -384,18 +388,23 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
     const Prim<T> pr = prim_at(P, I.Q, N, idx, 0);
     acc[0] += (double)(fxv * fxv + fyv * fyv);
     acc[1] += (double)(pr.rho * (fxv * pr.u + fyv * pr.v));
+    acc[2] += (double)(fxv * fxv);
+    acc[3] += (double)(fyv * fyv);
+    acc[4] += (double)(fxv * fyv);
   }
-  block_reduce_write<2, 0>(acc, I.part + (size_t)lb * 2);
+  block_reduce_write<5, 0>(acc, I.part + (size_t)lb * 5);
 }
 
 // One block per instance: rms normalisation and the power rescale with the
-// alpha memory (OOP/forcing.py:87-112).
+// alpha memory (OOP/forcing.py:87-112), and the statistics of the rescaled
+// forcing (OOP/forcing.py:113-118) from the raw sums: the rescaled field is
+// scale*alpha times the raw one.
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_falpha(const Inst<T>* __restrict__ tab) {
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
-  __shared__ double r[2];
-  reduce_partials<2, 0>(I.part, I.nblk_stage, r);
+  __shared__ double r[5];
+  reduce_partials<5, 0>(I.part, I.nblk_stage, r);
   __syncthreads();
   if (threadIdx.x != 0) return;
   double* c = I.ctl;
@@ -426,43 +435,11 @@ __global__ void __launch_bounds__(kBlock) b_falpha(const Inst<T>* __restrict__ t
   }
   c[C_ALPHA] = a;
   c[C_ATGT] = atgt;
-}
-
-template <class T>
-__global__ void __launch_bounds__(kBlock) b_fscale(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
-                                                   int ninst) {
-  HITX_BATCH_INST(OFF_RED)
-  if (I.kpar[K_FORCE] == 0.0) return;
-  const Params<T>& P = I.P;
-  const int N = P.nx * P.ny;
-  const T scale = (T)I.ctl[C_SCALE], alpha = (T)I.ctl[C_ALPHA];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int idx = lb * kBlock + threadIdx.x; idx < N; idx += I.nblk_stage * kBlock) {
-    const T a = (I.fx[idx] * scale) * alpha, b = (I.fy[idx] * scale) * alpha;
-    I.fx[idx] = a;
-    I.fy[idx] = b;
-    const Prim<T> pr = prim_at(P, I.Q, N, idx, 0);
-    acc[0] += (double)(a * a);
-    acc[1] += (double)(b * b);
-    acc[2] += (double)(a * b);
-    acc[3] += (double)(pr.rho * (a * pr.u + b * pr.v));
-  }
-  block_reduce_write<4, 0>(acc, I.part + (size_t)lb * 4);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kBlock) b_fstats(const Inst<T>* __restrict__ tab) {
-  const Inst<T>& I = tab[blockIdx.x];
-  if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
-  __shared__ double r[4];
-  reduce_partials<4, 0>(I.part, I.nblk_stage, r);
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double N = (double)I.P.nx * (double)I.P.ny;
-  I.ctl[C_FXX] = r[0] / N;
-  I.ctl[C_FYY] = r[1] / N;
-  I.ctl[C_FXY] = r[2] / N;
-  I.ctl[C_PIN] = r[3] / N;
+  const double g = scale * a;
+  c[C_FXX] = g * g * r[2] / N;
+  c[C_FYY] = g * g * r[3] / N;
+  c[C_FXY] = g * g * r[4] / N;
+  c[C_PIN] = g * r[1] / N;
 }
 
 }  // namespace hitx

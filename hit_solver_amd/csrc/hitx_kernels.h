@@ -987,7 +987,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
                                             const T* __restrict__ dfyT, const T* __restrict__ fx,
                                             const T* __restrict__ fy, const double* __restrict__ scal,
                                             const double* __restrict__ drows, T dt, T* __restrict__ out, int bx,
-                                            int by) {
+                                            int by, T fsc = T(1), T fal = T(1)) {
   __shared__ T sU[20 * kVS], sV[20 * kVS], sT[20 * kVS];
   __shared__ T sD[4][16][17];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
@@ -1040,8 +1040,8 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   }
   if (P.force_on || P.drag_on) {
     const Prim<T> st = prim(P, s0, s1, s2, s3);
-    if (P.force_on) {  // OOP/hit2d.py:264-276
-      const T f1 = fx[idx], f2 = fy[idx];
+    if (P.force_on) {  // OOP/hit2d.py:264-276; f = (f_raw*scale)*alpha (OOP/forcing.py:91-112)
+      const T f1 = (fx[idx] * fsc) * fal, f2 = (fy[idx] * fsc) * fal;
       Lv[1] = Lv[1] + st.rho * f1;
       Lv[2] = Lv[2] + st.rho * f2;
       Lv[3] = Lv[3] + st.rho * (st.u * f1 + st.v * f2);

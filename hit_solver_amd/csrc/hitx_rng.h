@@ -251,21 +251,43 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
   }
 }
 
-// Step 4 for chunk c (one thread).
-__device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c) {
-  if (c >= J.nch || J.stat[0] != 0) return;
-  int p = J.entry[c];
-  long long k = J.base[c];
-  while (p < kRngChunk) {
-    const uint8_t code = J.att[(size_t)p * J.nch + c];
-    const int len = code & 0x7f;
-    if (code & 0x80) {
-      if (k < J.n) J.out[k] = J.vals[(size_t)p * J.nch + c];
-      if (k == J.n - 1) J.stat[1] = c * kRngChunk + p + len;  // words consumed
-      ++k;
+// Step 4, one wavefront per 64 consecutive chunks: their values occupy one
+// contiguous index range (at most 64*kRngChunk), staged in the wave's LDS
+// slice (lds: kRngChunk*64 doubles) and written out coalesced.  The lane
+// that writes value n-1 records how many words the n values consumed.
+constexpr int kRngScatterThreads = 128;
+__device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double* lds) {
+  if (J.stat[0] != 0) return;
+  const int lane = threadIdx.x & 63;
+  const int c0 = c - lane;  // chunk of lane 0
+  if (c0 >= J.nch) return;
+  const long long lo = J.base[c0];
+  long long k = lo;
+  if (c < J.nch) {
+    int p = J.entry[c];
+    k = J.base[c];
+    while (p < kRngChunk) {
+      const uint8_t code = J.att[(size_t)p * J.nch + c];
+      const int len = code & 0x7f;
+      if (code & 0x80) {
+        if (k < J.n) lds[k - lo] = J.vals[(size_t)p * J.nch + c];
+        if (k == J.n - 1) J.stat[1] = c * kRngChunk + p + len;  // words consumed
+        ++k;
+      }
+      p += len;
     }
-    p += len;
   }
+  long long hi = k;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(hi, off, 64);
+    hi = o > hi ? o : hi;
+  }
+  if (hi > J.n) hi = J.n;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (long long i = lo + lane; i < hi; i += 64) J.out[i] = lds[i - lo];
 }
 
 // Last step (one thread): advance the state, or regenerate serially.
@@ -293,8 +315,9 @@ __global__ void __launch_bounds__(kRngAttemptThreads) k_rng_attempts(RngJob J) {
   rng_attempt_body(J, blockIdx.x * kRngAttemptThreads + threadIdx.x, lds + threadIdx.x * kRngChunk);
 }
 __global__ void __launch_bounds__(kRngScan) k_rng_scan(RngJob J, int force_serial) { rng_scan_body(J, force_serial); }
-__global__ void __launch_bounds__(kRngAttemptThreads) k_rng_scatter(RngJob J) {
-  rng_scatter_body(J, blockIdx.x * kRngAttemptThreads + threadIdx.x);
+__global__ void __launch_bounds__(kRngScatterThreads) k_rng_scatter(RngJob J) {
+  __shared__ double lds[kRngScatterThreads / 64][64 * kRngChunk];
+  rng_scatter_body(J, blockIdx.x * kRngScatterThreads + threadIdx.x, lds[threadIdx.x >> 6]);
 }
 __global__ void k_rng_finish(RngJob J) {
   if (threadIdx.x == 0) rng_finish_body(J);
