@@ -105,25 +105,12 @@ def _timed(fn, k):
     return max(time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3)
 
 
-def hit_configs_batched(dtype="fp32"):
-    from hit_solver_amd.hit2d import HIT2DConfig
-
-    out, seed = [], 0
-    for n in (128, 256, 384, 512):
-        for _ in range(16):
-            mt = (0.1, 0.25, 0.4)[seed % 3]
-            out.append(HIT2DConfig.production_512(nx=n, ny=n, forcing_seed=seed, target_mach=mt,
-                                                  mach_control_target=mt, dtype=dtype, tfinal=1e9))
-            seed += 1
-    return out
-
-
 def hit_measurements(steps):
     """Forced HIT: plain single-instance FP64 (API path and device-control
     graph) and the batched FP32 form."""
     import torch
 
-    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.batch import BatchRunner, hit_batched_configs
     from hit_solver_amd.hit2d import HIT2DConfig, HITSolver
 
     res = {}
@@ -149,7 +136,7 @@ def hit_measurements(steps):
                                 "form": "one instance, device-side control, 10 steps per HIP graph"}
     del b
     torch.cuda.empty_cache()
-    cs = hit_configs_batched("fp32")
+    cs = hit_batched_configs("fp32")
     cells = sum(c.nx * c.ny for c in cs)
     b = BatchRunner(cs)
     for _ in range(2):

@@ -98,6 +98,20 @@ def riemann_spec_kpar(c: RiemannConfig3, max_steps=None):
     return spec, k
 
 
+def hit_batched_configs(dtype="fp32"):
+    """The batched forced-HIT workload: 64 instances, 16 each at 128^2, 256^2,
+    384^2 and 512^2, seeds 0-63, Mach (0.1, 0.25, 0.4)[seed % 3] (also the
+    Mach-control target), production settings otherwise."""
+    out, seed = [], 0
+    for n in (128, 256, 384, 512):
+        for _ in range(16):
+            mt = (0.1, 0.25, 0.4)[seed % 3]
+            out.append(HIT2DConfig.production_512(nx=n, ny=n, forcing_seed=seed, target_mach=mt,
+                                                  mach_control_target=mt, dtype=dtype, tfinal=1e9))
+            seed += 1
+    return out
+
+
 def _lib_kpar():
     return 24  # HITX_KPAR
 

@@ -69,12 +69,16 @@ __device__ __forceinline__ T* stage_out(const Inst<T>& I, int st) {
   return st == 0 ? I.Q1 : st == 1 ? I.Q2 : I.S;
 }
 
-#define HITX_BATCH_INST(KIND)                               \
+#define HITX_BATCH_INST_AT(KIND, GB)                        \
   const int* __restrict__ offs_ = offs + (KIND) * (ninst + 1); \
-  const int ii_ = find_inst(offs_, ninst, blockIdx.x);      \
+  const int gb_ = (GB);                                     \
+  const int ii_ = find_inst(offs_, ninst, gb_);             \
   const Inst<T>& I = tab[ii_];                              \
   if (I.ctl[C_ACTIVE] == 0.0) return;                       \
-  const int lb = blockIdx.x - offs_[ii_];
+  const int lb = gb_ - offs_[ii_];
+#define HITX_BATCH_INST(KIND) HITX_BATCH_INST_AT(KIND, (int)blockIdx.x)
+// tile kernels: XCD-contiguous block order
+#define HITX_BATCH_TILE(KIND) HITX_BATCH_INST_AT(KIND, xcd_remap((int)blockIdx.x, (int)gridDim.x))
 
 // ---------------------------------------------------------------- stage path
 // st = 0, 1, 2: RK stages on Q, Q1, Q2 (enforced input, transposed copy);
@@ -105,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) b_drows(const Inst<T>* __restrict__ ta
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_sensor(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst, int st) {
-  HITX_BATCH_INST(OFF_SENS)
+  HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
@@ -124,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) b_rawmask(const Inst<T>* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_dilate(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst, int st) {
-  HITX_BATCH_INST(OFF_SENS)
+  HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
@@ -162,7 +166,7 @@ __global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst, int st) {
-  HITX_BATCH_INST(OFF_T16)
+  HITX_BATCH_TILE(OFF_T16)
   const int ntx = (I.P.nx + 15) / 16;
   const int by = lb / ntx, bx = lb - by * ntx;
   // the last stage writes Q in place when no hyperviscosity pass follows and
@@ -177,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ t
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_finish(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst) {
-  HITX_BATCH_INST(OFF_T16)
+  HITX_BATCH_TILE(OFF_T16)
   const int ntx = (I.P.nx + 15) / 16;
   const int by = lb / ntx, bx = lb - by * ntx;
   if (I.ctl[C_HVNOW] != 0.0) {

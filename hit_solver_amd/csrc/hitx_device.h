@@ -69,6 +69,19 @@ enum {
 // Stage reduction slots (block partials).
 enum { R_MAXR = 0, R_MAXP, R_MAXE, R_AX, R_AY, R_DIV2, R_VORT2, R_PSUM, R_DRAG };
 
+// ---------------------------------------------------------------- launch mapping
+// Workgroups of a 1-D grid are dispatched round-robin over the 8 XCDs (each
+// with its own L2).  Tile kernels remap so that XCD j processes one contiguous
+// range of tile indices: neighbouring tiles then share halo and half-used
+// cache lines in one L2 instead of fetching them again from HBM.  A bijection
+// of [0, total).
+constexpr int kXCDs = 8;
+__device__ __forceinline__ int xcd_remap(int b, int total) {
+  const int q = total / kXCDs, r = total - q * kXCDs;
+  const int j = b % kXCDs, i = b / kXCDs;
+  return j * q + (j < r ? j : r) + i;
+}
+
 // ---------------------------------------------------------------- algebra
 // Division in the pointwise algebra: IEEE for double (the FP64 mode
 // reproduces the reference's quotients bit for bit); for float, the hardware

@@ -12,6 +12,16 @@
 namespace hitx {
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Tile coordinates of this block of a 2-D tile grid in XCD-contiguous order.
+struct Tile2 {
+  int bx, by;
+};
+__device__ __forceinline__ Tile2 tile_xcd() {
+  const int gx = gridDim.x, total = gx * gridDim.y;
+  const int b = xcd_remap((int)(blockIdx.y * gx + blockIdx.x), total);
+  return Tile2{b % gx, b / gx};
+}
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 template <class T>
@@ -635,7 +645,8 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                    const double* __restrict__ scal, uint8_t* __restrict__ mask,
                                                    uint8_t* __restrict__ maskT, T* __restrict__ qT) {
-  sensor_body<T>(P, q, enforce_first, scal, mask, maskT, qT, blockIdx.x, blockIdx.y);
+  const Tile2 t_ = tile_xcd();
+  sensor_body<T>(P, q, enforce_first, scal, mask, maskT, qT, t_.bx, t_.by);
 }
 
 // ================================================================ line sweep
@@ -1002,6 +1013,23 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   }
   const int idx = cy * nx + cx, oidx = y * nx + x;
   const bool visc = mode == MODE_VISC || (mode == MODE_STAGE && P.visc_on);
+  // the node's own operands are loaded before the tile phase so their
+  // latency overlaps it
+  T gx[4] = {T(0), T(0), T(0), T(0)}, a0[4] = {T(0), T(0), T(0), T(0)};
+  T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0), f1r = T(0), f2r = T(0);
+  if (in && mode != MODE_VISC) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gx[c] = dfx[c * N + idx];
+    if (mode == MODE_STAGE) {
+      load4(qs, N, idx, s0, s1, s2, s3);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a0[c] = q0[c * N + idx];
+      if (P.force_on) {
+        f1r = fx[idx];
+        f2r = fy[idx];
+      }
+    }
+  }
   if (visc) viscous_tile(P, qs, x0, y0, sU, sV, sT);
   if (mode != MODE_VISC) {
     // dfyT tile: thread (ty, tx) reads x-row x0+ty, y = y0+tx (unit stride in y)
@@ -1023,15 +1051,13 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const T gy = own ? sD[c][ty][tx] : dfyT[c * N + (size_t)cx * ny + cy];
-    Lv[c] = (-dfx[c * N + idx]) - gy;
+    Lv[c] = (-gx[c]) - gy;
   }
   if (mode == MODE_RHS) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c * N + oidx] = Lv[c];
     return;
   }
-  T s0, s1, s2, s3;
-  load4(qs, N, idx, s0, s1, s2, s3);
   if (P.visc_on) {
     T vv[4];
     viscous_lds(P, sU, sV, sT, ty + 2, tx + 2, vv);
@@ -1041,7 +1067,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   if (P.force_on || P.drag_on) {
     const Prim<T> st = prim(P, s0, s1, s2, s3);
     if (P.force_on) {  // OOP/hit2d.py:264-276; f = (f_raw*scale)*alpha (OOP/forcing.py:91-112)
-      const T f1 = (fx[idx] * fsc) * fal, f2 = (fy[idx] * fsc) * fal;
+      const T f1 = (f1r * fsc) * fal, f2 = (f2r * fsc) * fal;
       Lv[1] = Lv[1] + st.rho * f1;
       Lv[2] = Lv[2] + st.rho * f2;
       Lv[3] = Lv[3] + st.rho * (st.u * f1 + st.v * f2);
@@ -1066,13 +1092,12 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   const T qsv[4] = {s0, s1, s2, s3};
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const T a0 = q0[c * N + idx];
     if (stage == 0)
-      o[c] = a0 + dt * Lv[c];
+      o[c] = a0[c] + dt * Lv[c];
     else if (stage == 1)
-      o[c] = T(0.75) * a0 + T(0.25) * (qsv[c] + dt * Lv[c]);
+      o[c] = T(0.75) * a0[c] + T(0.25) * (qsv[c] + dt * Lv[c]);
     else
-      o[c] = T(1.0 / 3.0) * a0 + T(2.0 / 3.0) * (qsv[c] + dt * Lv[c]);
+      o[c] = T(1.0 / 3.0) * a0[c] + T(2.0 / 3.0) * (qsv[c] + dt * Lv[c]);
   }
   enforce(P, o[0], o[1], o[2], o[3]);
 #pragma unroll
@@ -1085,7 +1110,8 @@ __global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int st
                                                    const T* __restrict__ dfyT, const T* __restrict__ fx,
                                                    const T* __restrict__ fy, const double* __restrict__ scal,
                                                    const double* __restrict__ drows, T dt, T* __restrict__ out) {
-  update_body<T>(P, mode, stage, q0, qs, dfx, dfyT, fx, fy, scal, drows, dt, out, blockIdx.x, blockIdx.y);
+  const Tile2 t_ = tile_xcd();
+  update_body<T>(P, mode, stage, q0, qs, dfx, dfyT, fx, fy, scal, drows, dt, out, t_.bx, t_.by);
 }
 
 // ================================================================ hyperviscosity
@@ -1203,7 +1229,8 @@ __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict_
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_hv(Params<T> P, const T* __restrict__ q, const uint8_t* __restrict__ active,
                                                int guard, int invert, T* __restrict__ out) {
-  hv_body<T>(P, q, active, guard, invert, out, blockIdx.x, blockIdx.y);
+  const Tile2 t_ = tile_xcd();
+  hv_body<T>(P, q, active, guard, invert, out, t_.bx, t_.by);
 }
 
 // clean(): enforce_physical_state, then the outflow guard if requested.
