@@ -1316,6 +1316,7 @@ __device__ __forceinline__ void fproject_body(const Params<T>& P, const double* 
     const int ka = ky < 0 ? -ky : ky;
     double re = 0.0, im = 0.0;
     if (W.live) {
+#pragma unroll 8
       for (int l = 0; l < 64; ++l) {
         const int x = x0 + l;
         if (x >= nx) break;
@@ -1356,6 +1357,7 @@ __device__ __forceinline__ void frows_body(const Params<T>& P, const double* __r
   if (t >= P.ny * kx1) return;
   const int y = t / kx1, k = t - y * kx1;
   double hr[2] = {0.0, 0.0}, hs[2] = {0.0, 0.0};
+#pragma unroll 4
   for (int m = 0; m < nf; ++m) {
     const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
     if (kx != k && kx != -k) continue;

@@ -33,9 +33,17 @@ __device__ __forceinline__ void wave_sync() {
 // LDS bytes of one wave's slice for a line of n nodes: q, f and the WENO
 // fluxes with kLineHalo halo entries on each side (wrapped or clamped copies,
 // so stencils index without wrap or clamp), the work list, the mask (halo 2).
+// q and f are stored padded: node i at i + pad*((i + kLineHalo) / L), so the
+// lanes' chunks start an odd number of words apart (no LDS bank conflicts
+// for 4-byte elements when all lanes read the same chunk offset).
 constexpr int kLineHalo = 4;
+__host__ __device__ constexpr int line_pad(int L) { return L == 1 ? 0 : (L % 2 == 0 ? 1 : 2); }
+__host__ __device__ inline int line_row(int n) {  // padded row length of q and f
+  const int L = n / 64;
+  return n + 1 + 2 * kLineHalo + line_pad(L) * ((n + 1 + 2 * kLineHalo) / (L > 0 ? L : 1) + 1);
+}
 __host__ __device__ inline size_t line_wave_lds(int n, int esize) {
-  return (size_t)(3 * (n + 1 + 2 * kLineHalo)) * esize + (size_t)(n + 1) * 2 + (size_t)(n + 4) + 16;
+  return (size_t)(2 * line_row(n) + (n + 1 + 2 * kLineHalo)) * esize + (size_t)(n + 1) * 2 + (size_t)(n + 4) + 16;
 }
 
 // b^e for 0 <= e < 2^10 by squaring (uniform or per-lane e).
@@ -64,13 +72,15 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
                                           const T* __restrict__ ql, const uint8_t* __restrict__ ml,
                                           const double* __restrict__ scal, T* __restrict__ out, int line, int nl,
                                           unsigned char* slice) {
-  const int n = 64 * L, S = n + 1 + 2 * kLineHalo;
+  const int n = 64 * L, S = n + 1 + 2 * kLineHalo, R = line_row(n);
+  constexpr int PAD = line_pad(L);
   const int lane = threadIdx.x & 63;
-  // halo-padded rows: entry i of the line at [i + kLineHalo]
-  T* qs = reinterpret_cast<T*>(slice) + kLineHalo;
-  T* fs = qs + S;
-  T* ws = fs + S;
+  // halo-padded rows: node i of the line at [i + kLineHalo] (+ bank padding for q and f)
+  T* const qb = reinterpret_cast<T*>(slice) + kLineHalo;
+  T* const fb = qb + R;
+  T* ws = fb + R;
   uint16_t* wl = reinterpret_cast<uint16_t*>(ws - kLineHalo + S);
+  auto A = [&](int i) { return i + PAD * ((i + kLineHalo) / L); };
   uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + n + 1) + 2;  // mask, halo 2
   const size_t NL = (size_t)nl * n;
   const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
@@ -135,16 +145,16 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     wave_sync();  // previous component's LDS reads are done
 #pragma unroll
     for (int j = 0; j < L; ++j) {  // selects, not a dynamic register index
-      qs[lane * L + j] = c == 0 ? q[0][j] : (c == 1 ? q[1][j] : (c == 2 ? q[2][j] : q[3][j]));
-      fs[lane * L + j] = c == 0 ? f[0][j] : (c == 1 ? f[1][j] : (c == 2 ? f[2][j] : f[3][j]));
+      qb[A(lane * L + j)] = c == 0 ? q[0][j] : (c == 1 ? q[1][j] : (c == 2 ? q[2][j] : q[3][j]));
+      fb[A(lane * L + j)] = c == 0 ? f[0][j] : (c == 1 ? f[1][j] : (c == 2 ? f[2][j] : f[3][j]));
     }
     wave_sync();
     if (lane < kLineHalo) {  // halos: wrapped or clamped copies of the line ends
       const int e = lane + 1;
-      qs[-e] = PER ? qs[n - e] : qs[0];
-      fs[-e] = PER ? fs[n - e] : fs[0];
-      qs[n - 1 + e] = PER ? qs[e - 1] : qs[n - 1];
-      fs[n - 1 + e] = PER ? fs[e - 1] : fs[n - 1];
+      qb[A(-e)] = PER ? qb[A(n - e)] : qb[A(0)];
+      fb[A(-e)] = PER ? fb[A(n - e)] : fb[A(0)];
+      qb[A(n - 1 + e)] = PER ? qb[A(e - 1)] : qb[A(n - 1)];
+      fb[A(n - 1 + e)] = PER ? fb[A(e - 1)] : fb[A(n - 1)];
     }
     wave_sync();
     // WENO7 of the Lax-Friedrichs split on the listed interfaces
@@ -156,28 +166,35 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       for (int j = 0; j < 7; ++j) {
         const int ip = il - 3 + j;  // f+ stencil i-3 .. i+3
         const int im = il + 4 - j;  // f- stencil i+4 .. i-2
-        fp[j] = T(0.5) * (fs[ip] + alpha * qs[ip]);
-        fm[j] = T(0.5) * (fs[im] - alpha * qs[im]);
+        fp[j] = T(0.5) * (fb[A(ip)] + alpha * qb[A(ip)]);
+        fm[j] = T(0.5) * (fb[A(im)] - alpha * qb[A(im)]);
       }
       ws[k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
               weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
     }
     wave_sync();
-    if (lane == 0) {  // WENO flux halo: wrapped (periodic) or clamped (interfaces 0..n)
-      ws[-1] = PER ? ws[n - 1] : ws[0];
-      ws[nI] = PER ? ws[0] : ws[n];
+    if (lane < 2) {  // WENO flux halo: wrapped (periodic, 2 each side) or clamped (interfaces 0..n)
+      const int e = lane + 1;
+      if (PER) {
+        ws[-e] = ws[n - e];
+        ws[n - 1 + e] = ws[e - 1];
+      } else if (lane == 0) {
+        ws[-1] = ws[0];
+        ws[n + 1] = ws[n];
+      }
     }
     wave_sync();
     // hybrid interface flux of interfaces kb .. kb+L (one beyond the lane's own)
     auto iface = [&](int k) -> T {
       if (P.weno_only || iflag(k)) return (K.alpha_c * ws[k + 1] + ws[k]) + K.alpha_c * ws[k - 1];
       const int il = PER ? k : k - 1;
-      return (c3 * (fs[il + 3] + fs[il - 2]) + c2 * (fs[il + 2] + fs[il - 1])) + c1 * (fs[il + 1] + fs[il]);
+      return (c3 * (fb[A(il + 3)] + fb[A(il - 2)]) + c2 * (fb[A(il + 2)] + fb[A(il - 1)])) +
+             c1 * (fb[A(il + 1)] + fb[A(il)]);
     };
     const int kb = PER ? lane * L - 1 : lane * L;
     T x[L];
     {
-      T Fprev = iface(PER && kb < 0 ? n - 1 : kb);
+      T Fprev = iface(kb);  // periodic lane 0: interface -1, the halo copy of interface n-1
 #pragma unroll
       for (int j = 0; j < L; ++j) {
         const T Fn = iface(kb + j + 1);
