@@ -62,6 +62,20 @@ __host__ __device__ inline int walk_blocks(int nx, int ny, int rb, int halo) {
   return (waves + kWaves - 1) / kWaves;
 }
 
+// Value of lane l-1 (dir = -1) or l+1 (dir = +1) in a wavefront: DPP
+// wave_shr:1 / wave_shl:1 (one VALU op, no LDS); the end lanes receive 0.
+__device__ __forceinline__ int dpp_shift(int v, int dir) {
+  return dir < 0 ? __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false)
+                 : __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_step(float v, int dir) {
+  return __int_as_float(dpp_shift(__float_as_int(v), dir));
+}
+__device__ __forceinline__ double lane_step(double v, int dir) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_shift((int)(b & 0xffffffffll), dir), hi = dpp_shift((int)(b >> 32), dir);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <class T>
 __device__ __forceinline__ T lane_from(T v, int src) {
   return __shfl(v, src & 63, 64);
@@ -112,8 +126,8 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       // next row's loads in flight while this row is reduced
       T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0);
       if (y + 1 < W.yb) load4(q, N, row_of(y + 2) * nx + W.xl, n0, n1, n2, n3);
-      const T ul = lane_from(cu.u, lane - 1), ur = lane_from(cu.u, lane + 1);
-      const T vl = lane_from(cu.v, lane - 1), vr = lane_from(cu.v, lane + 1);
+      const T ul = lane_step(cu.u, -1), ur = lane_step(cu.u, 1);
+      const T vl = lane_step(cu.v, -1), vr = lane_step(cu.v, 1);
       if (W.own) {
         const int x = W.x;
         const T cs = sqrt(dv(P.gamma * cu.p, cu.rho));
@@ -539,11 +553,11 @@ __device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __rest
   for (int y = W.ya; y < W.yb; ++y) {
     T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0);
     if (y + 1 < W.yb) load4(q, N, row_of(y + 2) * nx + W.xl, n0, n1, n2, n3);
-    const T ul = lane_from(cu.u, lane - 1), ur = lane_from(cu.u, lane + 1);
-    const T vl = lane_from(cu.v, lane - 1), vr = lane_from(cu.v, lane + 1);
-    const T rl = lane_from(cu.rho, lane - 1), rr = lane_from(cu.rho, lane + 1);
-    const T pl = lane_from(cu.p, lane - 1), pr = lane_from(cu.p, lane + 1);
-    const T el = lane_from(cu.e, lane - 1), erg = lane_from(cu.e, lane + 1);
+    const T ul = lane_step(cu.u, -1), ur = lane_step(cu.u, 1);
+    const T vl = lane_step(cu.v, -1), vr = lane_step(cu.v, 1);
+    const T rl = lane_step(cu.rho, -1), rr = lane_step(cu.rho, 1);
+    const T pl = lane_step(cu.p, -1), pr = lane_step(cu.p, 1);
+    const T el = lane_step(cu.e, -1), erg = lane_step(cu.e, 1);
     if (W.own) {
       bool m = false;
       const bool yin = per || (y > 0 && y < ny - 1);

@@ -58,13 +58,62 @@ __device__ __forceinline__ T ipow(T b, int e) {
   return r;
 }
 
+// Cross-lane moves.  DPP (GFX9 controls): row_shr:d 0x110+d (lane l takes
+// l-d within its row of 16), row_shl:d 0x100+d (l+d), wave_shr:1 0x138 (l-1),
+// wave_shl:1 0x130 (l+1); lanes without a source receive 0.  readlane for a
+// uniform lane's value.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Affine scans over the wavefront with a fixed decay a (pow: a^1, a^2, a^4,
+// a^8, a^16; pf = a^((l&15)+1), pb = a^(16-(l&15)) for this lane):
+// forward  S_l = sum_{m=0..l}    a^m v_{l-m},
+// backward T_l = sum_{m=0..63-l} a^m v_{l+m}.
+// Kogge-Stone within rows of 16 by DPP row shifts, then the rows joined
+// through their end values (readlane).
 template <class T>
-__device__ __forceinline__ T shfl_up_rot(T v, int d) {  // value of lane (l - d) mod 64
-  return __shfl(v, (int)((threadIdx.x - d) & 63), 64);
+__device__ __forceinline__ T scan_fwd(T v, const T (&pw)[5], T pf, int lane) {
+  v = v + pw[0] * dpp<0x111>(v);
+  v = v + pw[1] * dpp<0x112>(v);
+  v = v + pw[2] * dpp<0x114>(v);
+  v = v + pw[3] * dpp<0x118>(v);
+  const T s0 = rdlane(v, 15);
+  const T s1 = rdlane(v, 31) + pw[4] * s0;
+  const T s2 = rdlane(v, 47) + pw[4] * s1;
+  const int r = lane >> 4;
+  const T prev = r == 0 ? T(0) : (r == 1 ? s0 : (r == 2 ? s1 : s2));
+  return v + pf * prev;
 }
 template <class T>
-__device__ __forceinline__ T shfl_down_rot(T v, int d) {  // value of lane (l + d) mod 64
-  return __shfl(v, (int)((threadIdx.x + d) & 63), 64);
+__device__ __forceinline__ T scan_bwd(T v, const T (&pw)[5], T pb, int lane) {
+  v = v + pw[0] * dpp<0x101>(v);
+  v = v + pw[1] * dpp<0x102>(v);
+  v = v + pw[2] * dpp<0x104>(v);
+  v = v + pw[3] * dpp<0x108>(v);
+  const T t3 = rdlane(v, 48);
+  const T t2 = rdlane(v, 32) + pw[4] * t3;
+  const T t1 = rdlane(v, 16) + pw[4] * t2;
+  const int r = lane >> 4;
+  const T next = r == 3 ? T(0) : (r == 2 ? t3 : (r == 1 ? t2 : t1));
+  return v + pb * next;
 }
 
 template <class T, bool PER, int L>
@@ -140,6 +189,13 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
   const T aL = ipow(rho, L);
+  // per-lane powers of the scans (fixed for the line)
+  T apw[5];
+  apw[0] = aL;
+#pragma unroll
+  for (int k = 1; k < 5; ++k) apw[k] = apw[k - 1] * apw[k - 1];
+  const T a_f = ipow(aL, (lane & 15) + 1), a_b = ipow(aL, 16 - (lane & 15));
+  const T a_l = PER ? ipow(aL, lane) : T(0), a_r = ipow(aL, 63 - lane);
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     wave_sync();  // previous component's LDS reads are done
@@ -205,8 +261,8 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     // non-periodic: identity end rows, end contributions on the right-hand side
     T r0 = T(0), rn1 = T(0);
     if (!PER) {
-      r0 = __shfl(x[0], 0, 64);
-      rn1 = __shfl(x[L - 1], 63, 64);
+      r0 = rdlane(x[0], 0);
+      rn1 = rdlane(x[L - 1], 63);
     }
     T y[L];
     // causal pass, local
@@ -227,19 +283,12 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       y[j] = acc;
     }
     // carries: inclusive scan of C_{l+1} = e_l + a C_l, a = rho^L
-    T sc = acc, ad = aL;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const T o = shfl_up_rot(sc, d);
-      if (lane >= d) sc = sc + ad * o;
-      ad = ad * ad;
-    }
-    T carry = shfl_up_rot(sc, 1);  // exclusive: P_l
-    if (lane == 0) carry = T(0);
+    T sc = scan_fwd(acc, apw, a_f, lane);
+    T carry = dpp<0x138>(sc);  // exclusive: P_l (lane 0: 0)
     if (PER) {
       // C_0 = (e_63 + a P_63) / (1 - rho^n) = S_63 / (1 - rho^n); C_l = P_l + a^l C_0
-      const T c0 = __shfl(sc, 63, 64) * K.inv1mrn;
-      carry = carry + ipow(aL, lane) * c0;
+      const T c0 = rdlane(sc, 63) * K.inv1mrn;
+      carry = carry + a_l * c0;
     }
     {
       T pw = rho;
@@ -250,7 +299,7 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       }
     }
     T zn = T(0);
-    if (!PER) zn = __shfl((rho * y[L - 1]) / K.one_m_rho2, 63, 64);
+    if (!PER) zn = rdlane((rho * y[L - 1]) / K.one_m_rho2, 63);
     // anti-causal pass, local
     acc = T(0);
 #pragma unroll
@@ -259,24 +308,13 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       y[j] = acc;
     }
     // carries from the right: D_{l-1} = g_l + a D_l
-    sc = acc;
-    ad = aL;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const T o = shfl_down_rot(sc, d);
-      if (lane + d < 64) sc = sc + ad * o;
-      ad = ad * ad;
-    }
-    carry = shfl_down_rot(sc, 1);  // exclusive: Q_l
-    if (lane == 63) carry = T(0);
-    {
-      // wrap (periodic) or the value beyond the end (non-periodic): a^(63-l) times
-      const T ar = ipow(aL, 63 - lane);
-      if (PER)
-        carry = carry + ar * (__shfl(sc, 0, 64) * K.inv1mrn);
-      else
-        carry = carry + ar * zn;
-    }
+    sc = scan_bwd(acc, apw, a_b, lane);
+    carry = dpp<0x130>(sc);  // exclusive: Q_l (lane 63: 0)
+    // wrap (periodic) or the value beyond the end (non-periodic): a^(63-l) times
+    if (PER)
+      carry = carry + a_r * (rdlane(sc, 0) * K.inv1mrn);
+    else
+      carry = carry + a_r * zn;
     {
       T pw = rho;
 #pragma unroll
@@ -290,7 +328,7 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
 #pragma unroll
       for (int j = 0; j < L; ++j) o[j] = y[j];
     } else {
-      const T p0 = __shfl(y[0], 0, 64), pn = __shfl(y[L - 1], 63, 64);
+      const T p0 = rdlane(y[0], 0), pn = rdlane(y[L - 1], 63);
       const T det = T(1) - K.rnm1 * K.rnm1;
       const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
       T pl = ipow(rho, lane * L), pr = ipow(rho, n - 1 - lane * L);  // rho^i, rho^(n-1-i)
