@@ -983,17 +983,40 @@ __device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, cons
 
 // Primitives u, v, p/(rho R) of the 20x20 tile around a 16x16 block
 // (periodic wrap) into LDS.
+// Split in two so that a caller can issue the loads, do other work, then
+// form the primitives: each thread holds at most two of the 400 tile nodes.
 template <class T>
-__device__ __forceinline__ void viscous_tile(const Params<T>& P, const T* __restrict__ q, int x0, int y0, T* U, T* V,
-                                             T* Tt) {
+struct VTileLoad {
+  T a[2][4];
+};
+template <class T>
+__device__ __forceinline__ VTileLoad<T> viscous_tile_load(const Params<T>& P, const T* __restrict__ q, int x0,
+                                                           int y0) {
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  for (int i = threadIdx.x; i < 400; i += kBlock) {
-    const int ly = i / 20, lx = i - ly * 20;
-    const int gy = wrap(y0 - 2 + ly, ny), gx = wrap(x0 - 2 + lx, nx);
-    const Prim<T> pr = prim_at(P, q, N, gy * nx + gx, 0);
-    U[ly * kVS + lx] = pr.u;
-    V[ly * kVS + lx] = pr.v;
-    Tt[ly * kVS + lx] = dv(pr.p, pr.rho * P.rgas);
+  VTileLoad<T> v;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = threadIdx.x + s * kBlock;
+    if (i < 400) {
+      const int ly = i / 20, lx = i - ly * 20;
+      const int gy = wrap(y0 - 2 + ly, ny), gx = wrap(x0 - 2 + lx, nx);
+      load4(q, N, gy * nx + gx, v.a[s][0], v.a[s][1], v.a[s][2], v.a[s][3]);
+    }
+  }
+  return v;
+}
+template <class T>
+__device__ __forceinline__ void viscous_tile_store(const Params<T>& P, const VTileLoad<T>& v, T* U, T* V, T* Tt) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = threadIdx.x + s * kBlock;
+    if (i < 400) {
+      const int ly = i / 20, lx = i - ly * 20;
+      const Prim<T> pr = prim(P, v.a[s][0], v.a[s][1], v.a[s][2], v.a[s][3]);
+      U[ly * kVS + lx] = pr.u;
+      V[ly * kVS + lx] = pr.v;
+      Tt[ly * kVS + lx] = dv(pr.p, pr.rho * P.rgas);
+    }
   }
 }
 
@@ -1044,12 +1067,44 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       }
     }
   }
-  if (visc) viscous_tile(P, qs, x0, y0, sU, sV, sT);
+  VTileLoad<T> vt;
+  if (visc) vt = viscous_tile_load(P, qs, x0, y0);
+  T dsh[4];
   if (mode != MODE_VISC) {
     // dfyT tile: thread (ty, tx) reads x-row x0+ty, y = y0+tx (unit stride in y)
     const int rx = x0 + ty, ry = y0 + tx;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sD[c][tx][ty] = (rx < nx && ry < ny) ? dfyT[c * N + (size_t)rx * ny + ry] : T(0);
+    for (int c = 0; c < 4; ++c) dsh[c] = (rx < nx && ry < ny) ? dfyT[c * N + (size_t)rx * ny + ry] : T(0);
+  }
+  // node-local source terms while the tile loads are in flight
+  T fsrc[4] = {T(0), T(0), T(0), T(0)}, dsrc[4] = {T(0), T(0), T(0), T(0)};
+  if (in && mode == MODE_STAGE && (P.force_on || P.drag_on)) {
+    const Prim<T> st = prim(P, s0, s1, s2, s3);
+    if (P.force_on) {  // OOP/hit2d.py:264-276; f = (f_raw*scale)*alpha (OOP/forcing.py:91-112)
+      const T f1 = (f1r * fsc) * fal, f2 = (f2r * fsc) * fal;
+      fsrc[1] = st.rho * f1;
+      fsrc[2] = st.rho * f2;
+      fsrc[3] = st.rho * (st.u * f1 + st.v * f2);
+    }
+    if (P.drag_on) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
+      const double* H = drows + (size_t)cy * 10;
+      double lx = H[0], ly = H[5];
+      for (int k = 1; k <= P.dkx; ++k) {
+        const double* tw = P.twx + ((size_t)(k + P.kmx) * nx + cx) * 2;
+        const double c = tw[0], sn = tw[1];
+        lx += H[2 * k - 1] * c - H[2 * k] * sn;
+        ly += H[5 + 2 * k - 1] * c - H[5 + 2 * k] * sn;
+      }
+      const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
+      dsrc[1] = d1;
+      dsrc[2] = d2;
+      dsrc[3] = st.u * d1 + st.v * d2;
+    }
+  }
+  if (visc) viscous_tile_store(P, vt, sU, sV, sT);
+  if (mode != MODE_VISC) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sD[c][tx][ty] = dsh[c];
   }
   __syncthreads();
   if (!in) return;
@@ -1078,28 +1133,14 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
 #pragma unroll
     for (int c = 0; c < 4; ++c) Lv[c] = Lv[c] + vv[c];
   }
-  if (P.force_on || P.drag_on) {
-    const Prim<T> st = prim(P, s0, s1, s2, s3);
-    if (P.force_on) {  // OOP/hit2d.py:264-276; f = (f_raw*scale)*alpha (OOP/forcing.py:91-112)
-      const T f1 = (f1r * fsc) * fal, f2 = (f2r * fsc) * fal;
-      Lv[1] = Lv[1] + st.rho * f1;
-      Lv[2] = Lv[2] + st.rho * f2;
-      Lv[3] = Lv[3] + st.rho * (st.u * f1 + st.v * f2);
-    }
-    if (P.drag_on) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
-      const double* H = drows + (size_t)cy * 10;
-      double lx = H[0], ly = H[5];
-      for (int k = 1; k <= P.dkx; ++k) {
-        const double* tw = P.twx + ((size_t)(k + P.kmx) * nx + cx) * 2;
-        const double c = tw[0], sn = tw[1];
-        lx += H[2 * k - 1] * c - H[2 * k] * sn;
-        ly += H[5 + 2 * k - 1] * c - H[5 + 2 * k] * sn;
-      }
-      const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
-      Lv[1] = Lv[1] + d1;
-      Lv[2] = Lv[2] + d2;
-      Lv[3] = Lv[3] + (st.u * d1 + st.v * d2);
-    }
+  // the reference's order: rhs + viscous, + forcing, + drag, + cooling
+  if (P.force_on) {
+#pragma unroll
+    for (int c = 1; c < 4; ++c) Lv[c] = Lv[c] + fsrc[c];
+  }
+  if (P.drag_on) {
+#pragma unroll
+    for (int c = 1; c < 4; ++c) Lv[c] = Lv[c] + dsrc[c];
   }
   if (P.cool_on) Lv[3] = Lv[3] + (T)scal[S_COOL];  // OOP/hit2d.py:402-426
   T o[4];
