@@ -18,6 +18,12 @@ HITX_F64, HITX_F32 = 0, 1
 HITX_HYBRID, HITX_WENO = 0, 1
 HITX_EINVAL, HITX_EDEVICE, HITX_ENOMEM = -1, -2, -3
 HITX_MAX_MODES = 128
+# hitx_diagnostics output slots (include/hitx.h HITX_DIAG_*)
+(DIAG_RHO, DIAG_U, DIAG_V, DIAG_P, DIAG_C, DIAG_U2V2, DIAG_RUU, DIAG_RVV, DIAG_RUV, DIAG_RKE, DIAG_DIV2,
+ DIAG_VORT2, DIAG_LOW, DIAG_DRAG_E, DIAG_MASK, DIAG_COUNT) = range(16)
+# hitx_riemann_diagnostics output slots (HITX_RDIAG_*)
+(RDIAG_RHO_MIN, RDIAG_RHO_MAX, RDIAG_P_MIN, RDIAG_P_MAX, RDIAG_VORT_MAX, RDIAG_NONFINITE_Q, RDIAG_NONFINITE_RP,
+ RDIAG_RHO_NONPOS, RDIAG_P_NONPOS, RDIAG_MASK, RDIAG_COUNT) = range(11)
 
 c_i32 = ctypes.c_int32
 c_dbl = ctypes.c_double
@@ -64,6 +70,9 @@ SIGNATURES = {
                                    ctypes.c_size_t, c_vp]),
     "hitx_primitives": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "hitx_viscous_rhs": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
+    "hitx_diag_workspace_bytes": (ctypes.c_size_t, [c_vp, c_i32]),
+    "hitx_diagnostics": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_riemann_diagnostics": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),
     "hitx_timing_enable": (c_i32, [c_vp, c_i32]),
     "hitx_set_line_form": (c_i32, [c_vp, c_i32]),
     "hitx_normals_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),

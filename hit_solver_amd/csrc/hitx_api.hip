@@ -11,6 +11,7 @@
 
 #include "../../include/hitx.h"
 #include "hitx_batch.h"
+#include "hitx_diag.h"
 #include "hitx_line.h"
 
 using namespace hitx;
@@ -119,6 +120,7 @@ struct hitx_plan {
   std::vector<int32_t> drag_modes, force_modes;
   double* d_tw = nullptr;   // twx then twy
   int* d_modes = nullptr;   // drag modes then force modes
+  double* d_dk = nullptr;   // spectral derivative tables, x then y (periodic plans)
   LineK<double> lx_d{}, ly_d{};
   LineK<float> lx_f{}, ly_f{};
   // workspace layout
@@ -209,6 +211,26 @@ size_t sensor_lds_bytes(int w, int esize) {
   const int H = w + 1, EX = kSTX + 2 * H, EY = kSTY + 2 * H;
   return (size_t)5 * EX * EY * esize + (size_t)(kSTX + 2 * w) * (kSTY + 2 * w) + (size_t)(kSTY + 2 * w) * kSTX +
          (size_t)kSTY * (kSTX + 1) + 64;
+}
+
+// Spectral derivative kernel of a periodic line of n nodes spaced h apart,
+// extended for the tiled convolution: t[i] = d((i - kDiagPad) mod n),
+// i < 2 n + 2 kDiagPad, with d(0) = 0 and, for m != 0,
+// d(m) = (2 pi / (n h)) (-1)^m / 2 * (cot(pi m / n) for even n, 1/sin(pi m / n) for odd n),
+// evaluated in long double.
+std::vector<double> spectral_derivative_table(int n, double h) {
+  const long double pi = 3.14159265358979323846264338327950288L;
+  std::vector<double> d(n, 0.0);
+  const long double scale = 2.0L * pi / ((long double)n * (long double)h);
+  for (int m = 1; m < n; ++m) {
+    const long double th = pi * (long double)m / (long double)n;
+    const long double sg = (m & 1) ? -0.5L : 0.5L;
+    const long double v = (n % 2 == 0) ? std::cos(th) / std::sin(th) : 1.0L / std::sin(th);
+    d[m] = (double)(scale * sg * v);
+  }
+  std::vector<double> t((size_t)2 * n + 2 * kDiagPad);
+  for (size_t i = 0; i < t.size(); ++i) t[i] = d[(((long)i - kDiagPad) % n + n) % n];
+  return t;
 }
 
 template <class T>
@@ -468,6 +490,16 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
       return fail(HITX_EDEVICE, "allocating plan tables failed");
     }
   }
+  if (desc->periodic) {
+    std::vector<double> dk = spectral_derivative_table(desc->nx, desc->dx);
+    const std::vector<double> dky = spectral_derivative_table(desc->ny, desc->dy);
+    dk.insert(dk.end(), dky.begin(), dky.end());
+    if (hipMalloc(&pl->d_dk, dk.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(pl->d_dk, dk.data(), dk.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      hitx_plan_destroy(pl);
+      return fail(HITX_EDEVICE, "allocating plan tables failed");
+    }
+  }
   pl->lx_d = make_linek<double>(desc->nx);
   pl->ly_d = make_linek<double>(desc->ny);
   pl->lx_f = make_linek<float>(desc->nx);
@@ -516,6 +548,7 @@ int hitx_plan_destroy(hitx_plan* pl) {
   for (hipEvent_t e : pl->ev) (void)hipEventDestroy(e);
   if (pl->d_tw) (void)hipFree(pl->d_tw);
   if (pl->d_modes) (void)hipFree(pl->d_modes);
+  if (pl->d_dk) (void)hipFree(pl->d_dk);
   delete pl;
   return HITX_OK;
 }
@@ -686,6 +719,90 @@ int hitx_primitives(hitx_plan* pl, const void* q, void* rho, void* u, void* v, v
     k_primitives<float><<<pl->nblk_red, kBlock, 0, s>>>(make_params<float>(pl), static_cast<const float*>(q),
                                                         static_cast<float*>(rho), static_cast<float*>(u),
                                                         static_cast<float*>(v), static_cast<float*>(p));
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+size_t hitx_diag_workspace_bytes(const hitx_plan* pl, int32_t with_fields) {
+  if (!pl) return 0;
+  const size_t N = pl->N, ny = pl->d.ny;
+  size_t o = with_fields ? 0 : align256(6 * N * sizeof(double));
+  o += align256(ny * kDiagRowW * sizeof(double));
+  o += align256(ny * kDiagMomW * sizeof(double));
+  o += align256((size_t)kDiagF * kDiagMaxModes * 2 * sizeof(double));
+  return o;
+}
+
+int hitx_diagnostics(hitx_plan* pl, const void* q, const uint8_t* mask, const int32_t* modes_host,
+                     int32_t n_modes, double drag_coef, double* fields, double* out, void* ws, size_t ws_bytes,
+                     void* stream) {
+  if (!pl) return fail(HITX_EINVAL, "null plan");
+  if (!q || !out) return fail(HITX_EINVAL, "null array");
+  if (!pl->d.periodic || !pl->d_dk) return fail(HITX_EINVAL, "diagnostics are defined on the periodic grid only");
+  if (n_modes < 0 || n_modes > kDiagMaxModes)
+    return fail(HITX_EINVAL, "n_lowpass must be in 0.." + std::to_string(kDiagMaxModes));
+  if (n_modes > 0 && !modes_host) return fail(HITX_EINVAL, "null low-pass mode list");
+  const size_t need = hitx_diag_workspace_bytes(pl, fields != nullptr);
+  if (!ws || ws_bytes < need) return fail(HITX_ENOMEM, "workspace too small: need " + std::to_string(need) + " bytes");
+  DiagModes M{};
+  M.n = n_modes;
+  for (int m = 0; m < n_modes; ++m) {
+    const int kx = modes_host[2 * m], ky = modes_host[2 * m + 1];
+    if (std::abs(kx) > kDiagMaxK || std::abs(ky) > kDiagMaxK)
+      return fail(HITX_EINVAL, "low-pass mode index beyond " + std::to_string(kDiagMaxK));
+    if (kx == 0 && ky == 0) return fail(HITX_EINVAL, "the low-pass band excludes k = 0");
+    M.kx[m] = (int8_t)kx;
+    M.ky[m] = (int8_t)ky;
+    M.kxmax = std::max(M.kxmax, std::abs(kx));
+    M.kymax = std::max(M.kymax, std::abs(ky));
+  }
+  const int nx = pl->d.nx, ny = pl->d.ny;
+  const size_t N = pl->N;
+  uint8_t* b = static_cast<uint8_t*>(ws);
+  double* F = fields;
+  if (!F) {
+    F = reinterpret_cast<double*>(b);
+    b += align256(6 * N * sizeof(double));
+  }
+  double* rowp = reinterpret_cast<double*>(b);
+  b += align256((size_t)ny * kDiagRowW * sizeof(double));
+  double* rowm = reinterpret_cast<double*>(b);
+  b += align256((size_t)ny * kDiagMomW * sizeof(double));
+  double* modes = reinterpret_cast<double*>(b);
+  double *rho = F, *u = F + N, *v = F + 2 * N, *p = F + 3 * N, *dvg = F + 4 * N, *vort = F + 5 * N;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Params<double> P = make_params<double>(pl);
+  if (pl->esize == 8)
+    k_diag_fields<double><<<ny, kBlock, 0, s>>>(P, static_cast<const double*>(q), rho, u, v, p, rowp, M.kxmax);
+  else
+    k_diag_fields<float><<<ny, kBlock, 0, s>>>(P, static_cast<const float*>(q), rho, u, v, p, rowp, M.kxmax);
+  k_finalize_sum<<<kDiagS, kBlock, 0, s>>>(rowp, ny, kDiagRowW, out);
+  if (M.n > 0) k_diag_modes<<<kDiagF * M.n, kBlock, 0, s>>>(rowp, ny, M, modes);
+  const double* dkx = pl->d_dk;
+  const double* dky = pl->d_dk + 2 * (size_t)nx + 2 * kDiagPad;
+  k_diag_deriv<0><<<dim3((ny + kDerLines - 1) / kDerLines, (nx + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
+      nx, ny, dkx, u, v, dvg, vort);
+  k_diag_deriv<1><<<dim3((nx + kDerLines - 1) / kDerLines, (ny + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
+      nx, ny, dky, v, u, dvg, vort);
+  k_diag_moments<<<ny, kBlock, 0, s>>>(nx, ny, rho, u, v, dvg, vort, mask, out, modes, M, -drag_coef, rowm);
+  k_finalize_sum<<<kDiagMomW, kBlock, 0, s>>>(rowm, ny, kDiagMomW, out + kDiagS);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+int hitx_riemann_diagnostics(hitx_plan* pl, const void* q, const uint8_t* mask, double* fields, double* out,
+                             void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_ws(pl, ws, ws_bytes);
+  if (rc) return rc;
+  if (!q || !out) return fail(HITX_EINVAL, "null array");
+  const WS w = carve(pl, ws);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Params<double> P = make_params<double>(pl);
+  if (pl->esize == 8)
+    k_riemann_diag<double><<<pl->nblk_red, kBlock, 0, s>>>(P, static_cast<const double*>(q), mask, fields, w.part);
+  else
+    k_riemann_diag<float><<<pl->nblk_red, kBlock, 0, s>>>(P, static_cast<const float*>(q), mask, fields, w.part);
+  k_riemann_diag_finalize<<<1, kBlock, 0, s>>>(w.part, pl->nblk_red, out);
   LAUNCHCHK();
   return HITX_OK;
 }

@@ -10,13 +10,16 @@ HIP kernels; the quadrant initial state is built on the host (setup).
 
 from __future__ import annotations
 
+import json
 import math
 from dataclasses import asdict, dataclass
+from pathlib import Path
 from typing import ClassVar
 
 import numpy as np
 import torch
 
+from . import _lib
 from .domain import Domain2D
 from .equations import EulerEquation2D
 from .runtime import PlanSpec, get_plan
@@ -254,6 +257,8 @@ def apply_outflow_guard(q, guard_cells: int = 2, equation: EulerEquation2D | Non
 
 
 def raw_density_pressure(q, equation):
+    """riemann_config3.py:432-439 (elementwise; the checks below run on the
+    device through hitx_riemann_diagnostics)."""
     rho = q[0]
     u = q[1] / rho
     v = q[2] / rho
@@ -261,16 +266,24 @@ def raw_density_pressure(q, equation):
     return rho, (equation.gamma - 1.0) * (q[3] - ke)
 
 
+def _check_plan(q, equation, domain: Domain2D | None = None):
+    ny, nx = (int(n) for n in q.shape[1:])
+    dx, dy = (domain.dx, domain.dy) if domain is not None else (1.0, 1.0)
+    return get_plan(PlanSpec(nx=nx, ny=ny, dx=dx, dy=dy, periodic=False, dtype=q.dtype, gamma=equation.gamma,
+                             rho_floor=equation.rho_floor, p_floor=equation.pressure_floor))
+
+
 def validate_raw_physical_state(q, equation, label: str = "state") -> None:
-    """riemann_config3.py:432-452 (raises FloatingPointError)."""
-    if bool(torch.any(~torch.isfinite(q))):
+    """riemann_config3.py:442-452 (raises FloatingPointError); one device
+    reduction."""
+    v = _check_plan(q, equation).riemann_diagnostics(q)
+    if v[_lib.RDIAG_NONFINITE_Q] > 0:
         raise FloatingPointError(f"{label} contains NaN or Inf values")
-    rho, p = raw_density_pressure(q, equation)
-    if bool(torch.any(~torch.isfinite(rho))) or bool(torch.any(~torch.isfinite(p))):
+    if v[_lib.RDIAG_NONFINITE_RP] > 0:
         raise FloatingPointError(f"{label} has non-finite density or pressure")
-    if bool(torch.any(rho <= 0.0)):
+    if v[_lib.RDIAG_RHO_NONPOS] > 0:
         raise FloatingPointError(f"{label} contains non-positive density")
-    if bool(torch.any(p <= 0.0)):
+    if v[_lib.RDIAG_P_NONPOS] > 0:
         raise FloatingPointError(f"{label} contains non-positive pressure")
 
 
@@ -279,23 +292,54 @@ def hyperviscosity_enabled_for_config(config: RiemannConfig3) -> bool:
     return config.scheme == "hybrid" and config.mn > 0.0 and config.hyperviscosity_interval > 0
 
 
+def primitive_fields(q, equation, domain: Domain2D | None = None):
+    """riemann_config3.py:455-456: rho, u, v, p (floored primitives) as
+    float64 device planes."""
+    _v, planes = _check_plan(q, equation, domain).riemann_diagnostics(q, fields=True)
+    return planes[0], planes[1], planes[2], planes[3]
+
+
 def vorticity_z(q, domain: Domain2D, equation):
-    """riemann_config3.py:459-463 with one-sided ends (diagnostics only)."""
-    rho, u, v, _p = equation.primitive_from_conservative(q)
+    """riemann_config3.py:459-463: dv/dx - du/dy of the floored primitives,
+    central inside and one-sided at the ends (:268-275); float64 plane."""
+    _v, planes = _check_plan(q, equation, domain).riemann_diagnostics(q, fields=True)
+    return planes[4]
 
-    def fd(a, h, dim):
-        d = torch.empty_like(a)
-        if dim == -1:
-            d[:, 1:-1] = (a[:, 2:] - a[:, :-2]) / (2.0 * h)
-            d[:, 0] = (a[:, 1] - a[:, 0]) / h
-            d[:, -1] = (a[:, -1] - a[:, -2]) / h
-        else:
-            d[1:-1] = (a[2:] - a[:-2]) / (2.0 * h)
-            d[0] = (a[1] - a[0]) / h
-            d[-1] = (a[-1] - a[-2]) / h
-        return d
 
-    return fd(v, domain.dx, -1) - fd(u, domain.dy, -2)
+def _diagnostics(q, config: RiemannConfig3, time: float, steps: int, hyperviscosity_applications: int, sensor):
+    """riemann_config3.py:629-650 from one sensor launch and one device
+    reduction."""
+    mask = sensor.detect_u8(q)
+    v = _check_plan(q, config.equation, config.domain).riemann_diagnostics(q, mask)
+    n = config.nx * config.ny
+    return RiemannDiagnostics(
+        backend=config.backend, scheme=config.scheme,
+        hyperviscosity_enabled=hyperviscosity_enabled_for_config(config),
+        hyperviscosity_applications=hyperviscosity_applications,
+        final_sensor_fraction=float(v[_lib.RDIAG_MASK] / n),
+        density_min=float(v[_lib.RDIAG_RHO_MIN]), density_max=float(v[_lib.RDIAG_RHO_MAX]),
+        pressure_min=float(v[_lib.RDIAG_P_MIN]), pressure_max=float(v[_lib.RDIAG_P_MAX]),
+        max_abs_vorticity=float(v[_lib.RDIAG_VORT_MAX]),
+        has_nonfinite=bool(v[_lib.RDIAG_NONFINITE_Q] > 0 or v[_lib.RDIAG_NONFINITE_RP] > 0),
+        density_positive=bool(v[_lib.RDIAG_RHO_NONPOS] == 0),
+        pressure_positive=bool(v[_lib.RDIAG_P_NONPOS] == 0),
+        time=time, steps=steps)
+
+
+def print_diagnostic_summary(diagnostics: RiemannDiagnostics) -> None:
+    """riemann_config3.py:653-664."""
+    d = diagnostics
+    print("Diagnostic summary:")
+    print(f"  backend              : {d.backend}")
+    print(f"  scheme               : {d.scheme}")
+    print(f"  density min/max      : {d.density_min:.8e} / {d.density_max:.8e}")
+    print(f"  pressure min/max     : {d.pressure_min:.8e} / {d.pressure_max:.8e}")
+    print(f"  max |vorticity|      : {d.max_abs_vorticity:.8e}")
+    print(f"  NaN/Inf present      : {d.has_nonfinite}")
+    print(f"  density positive     : {d.density_positive}")
+    print(f"  pressure positive    : {d.pressure_positive}")
+    print(f"  final sensor fraction: {d.final_sensor_fraction:.6f}")
+    print(f"  hyperviscosity       : {d.hyperviscosity_enabled} ({d.hyperviscosity_applications} applications)")
 
 
 class RiemannSolver:
@@ -399,27 +443,83 @@ class RiemannSolver:
         return self.q
 
     def diagnostics(self) -> RiemannDiagnostics:
-        q, c, eq = self.q, self.config, self.equation
-        rho, p = raw_density_pressure(q, eq)
-        w = vorticity_z(q, c.domain, eq)
-        nonfinite = bool(torch.any(~torch.isfinite(q)) | torch.any(~torch.isfinite(rho)) | torch.any(~torch.isfinite(p)))
-        return RiemannDiagnostics(
-            backend=c.backend, scheme=c.scheme, hyperviscosity_enabled=self.hv_enabled,
-            hyperviscosity_applications=self.hv_applications,
-            final_sensor_fraction=float(self.sensor.detect(q).double().mean()),
-            density_min=float(rho.min()), density_max=float(rho.max()), pressure_min=float(p.min()),
-            pressure_max=float(p.max()), max_abs_vorticity=float(w.abs().max()), has_nonfinite=nonfinite,
-            density_positive=bool(torch.all(rho > 0.0)), pressure_positive=bool(torch.all(p > 0.0)),
-            time=self.time, steps=self.steps)
+        return _diagnostics(self.q, self.config, self.time, self.steps, self.hv_applications, self.sensor)
 
 
 def run_riemann(config: RiemannConfig3):
-    """riemann_config3.py:681-749: returns (q, RiemannDiagnostics)."""
+    """riemann_config3.py:681-749: returns (q, RiemannDiagnostics), with the
+    reference's progress lines and summary."""
+    print("Using array backend: hip")
     s = RiemannSolver(config)
-    q = s.run()
+    label = getattr(config, "configuration_label", f"Config {config.configuration_number}")
+    print(f"Riemann {label}: nx={config.nx}, ny={config.ny}, dt={s.dt:.5e}, steps={s.n_steps}, "
+          f"tfinal={config.tfinal}")
+    note = "enabled on compact-FD nodes only" if s.hv_enabled else "disabled"
+    if config.scheme == "weno" and config.mn > 0.0:
+        note += " (WENO-only scheme)"
+    print(f"scheme={config.scheme}; local hyperviscosity {note} (mn={config.mn})")
+    while s.steps < s.n_steps:
+        s.step()
+        if s.steps % config.progress_every == 0 or s.steps == s.n_steps:
+            print(f"step={s.steps:6d}, time={s.time:.6f}")
     d = s.diagnostics()
-    validate_raw_physical_state(q, config.equation, "final state")
-    return q, d
+    validate_raw_physical_state(s.q, config.equation, "final state")
+    print_diagnostic_summary(d)
+    return s.q, d
+
+
+def run_riemann_config3(config: RiemannConfig3):
+    return run_riemann(config)
+
+
+def make_run_id(prefix: str) -> str:
+    """Timestamped run identifier (OOP/run_utils.py:14-18)."""
+    import uuid
+    from datetime import datetime
+
+    return f"{prefix}_{datetime.now().strftime('%Y%m%d_%H%M%S_%f')}_{uuid.uuid4().hex[:8]}"
+
+
+def save_riemann_npz(path: Path, q, config: RiemannConfig3, diagnostics: RiemannDiagnostics) -> Path:
+    """riemann_config3.py:497-519: grid, state, primitive planes, omega_z,
+    time, steps and the configuration and diagnostics as JSON strings."""
+    d = config.domain
+    _v, planes = _check_plan(q, config.equation, d).riemann_diagnostics(q, fields=True)
+    f = planes.cpu().numpy()
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(
+        path, x=d.x, y=d.y, q=q.double().cpu().numpy(), rho=f[0], u=f[1], v=f[2], pressure=f[3], omega_z=f[4],
+        time=np.asarray(diagnostics.time), steps=np.asarray(diagnostics.steps),
+        configuration_number=np.asarray(config.configuration_number),
+        config_json=np.asarray(json.dumps(riemann_config_dict(config), default=str)),
+        diagnostics_json=np.asarray(json.dumps(asdict(diagnostics), default=str)),
+    )
+    return path
+
+
+def load_riemann_npz(path: Path):
+    """riemann_config3.py:522-532: (npz data, RiemannConfig3 of its config_json)."""
+    data = np.load(path, allow_pickle=False)
+    conf = json.loads(str(data["config_json"])) if "config_json" in data else {}
+    fields = RiemannConfig3.__dataclass_fields__
+    return data, RiemannConfig3(**{k: v for k, v in conf.items() if k != "configuration_number" and k in fields})
+
+
+def save_run_outputs(q, config: RiemannConfig3, diagnostics: RiemannDiagnostics, output_dir: Path,
+                     run_id: str | None = None, plot: bool = True, **plot_options) -> dict:
+    """riemann_config3.py:756-810: <output_dir>/<run_id>/config.json,
+    diagnostics.json and <run_id>_final.npz.  Figures are not drawn by this
+    backend: the npz is the input of the reference's plot_from_npz."""
+    run_id = run_id or make_run_id(f"riemann{config.configuration_number}")
+    run_dir = Path(output_dir) / run_id
+    run_dir.mkdir(parents=True, exist_ok=True)
+    for name, payload in (("config.json", riemann_config_dict(config)), ("diagnostics.json", asdict(diagnostics))):
+        (run_dir / name).write_text(json.dumps(payload, indent=2, sort_keys=True))
+    npz_path = save_riemann_npz(run_dir / f"{run_id}_final.npz", q, config, diagnostics)
+    if plot:
+        print(f"figure not drawn (plot_from_npz of the reference reads {npz_path.name})")
+    return {"npz": npz_path}
 
 
 def riemann_config_dict(config: RiemannConfig3) -> dict:

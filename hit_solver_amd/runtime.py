@@ -205,6 +205,40 @@ class Plan:
         check(lib().hitx_viscous_rhs(self.handle, _p(q), _p(out), _stream()))
         return out
 
+    def diagnostics(self, q, mask_u8=None, lowpass_modes=(), drag_coef: float = 0.0, fields: bool = False):
+        """Diagnostic sums of q in double (HITX_DIAG_* slots, synchronises) and,
+        when fields, the device planes [6, ny, nx] = rho, u, v, p, divergence,
+        vorticity (float64)."""
+        self._q(q)
+        if mask_u8 is not None:
+            require_device(mask_u8, "mask", torch.uint8, (self.spec.ny, self.spec.nx))
+        modes = np.ascontiguousarray(np.asarray(lowpass_modes, dtype=np.int32).reshape(-1, 2))
+        need = int(lib().hitx_diag_workspace_bytes(self.handle, int(fields)))
+        if getattr(self, "_dws", None) is None or self._dws.numel() < need:
+            self._dws = torch.empty(need, dtype=torch.uint8, device=q.device)
+        planes = torch.empty((6, self.spec.ny, self.spec.nx), dtype=torch.float64, device=q.device) if fields \
+            else None
+        out = torch.empty(_lib.DIAG_COUNT, dtype=torch.float64, device=q.device)
+        check(lib().hitx_diagnostics(self.handle, _p(q), _p(mask_u8), modes.ctypes.data_as(ctypes.c_void_p),
+                                     len(modes), float(drag_coef), _p(planes), _p(out), _p(self._dws),
+                                     self._dws.numel(), _stream()))
+        sums = out.cpu().numpy()
+        return (sums, planes) if fields else sums
+
+    def riemann_diagnostics(self, q, mask_u8=None, fields: bool = False):
+        """HITX_RDIAG_* values of q (synchronises) and, when fields, the device
+        planes [5, ny, nx] = rho, u, v, p, omega_z (float64)."""
+        self._q(q)
+        if mask_u8 is not None:
+            require_device(mask_u8, "mask", torch.uint8, (self.spec.ny, self.spec.nx))
+        planes = torch.empty((5, self.spec.ny, self.spec.nx), dtype=torch.float64, device=q.device) if fields \
+            else None
+        out = torch.empty(_lib.RDIAG_COUNT, dtype=torch.float64, device=q.device)
+        check(lib().hitx_riemann_diagnostics(self.handle, _p(q), _p(mask_u8), _p(planes), _p(out), _p(self.ws),
+                                             self.ws_bytes, _stream()))
+        vals = out.cpu().numpy()
+        return (vals, planes) if fields else vals
+
     def set_line_form(self, form: int):
         """0: one wavefront per line where the line length allows (multiple of
         64, at most 512), else the block form; 1: the block form everywhere."""

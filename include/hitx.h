@@ -163,6 +163,75 @@ int hitx_primitives(hitx_plan* plan, const void* q, void* rho, void* u, void* v,
  * adds when desc.viscosity > 0): out [4][ny][nx], out[0] = 0. */
 int hitx_viscous_rhs(hitx_plan* plan, const void* q, void* out, void* stream);
 
+/* HIT diagnostics of a periodic state, computed in double whatever the
+ * plan's dtype.  Replaces the host reductions of
+ *   OOP/hit2d.py:471-548   compute_diagnostics (means, moments, Mt, rms)
+ *   OOP/hit2d.py:459-468   divergence_and_vorticity (spectral derivatives)
+ *   OOP/hit2d.py:429-453   large_scale_kinetic_fraction
+ *   OOP/hit2d.py:370-399, 452-456  source_power(large_scale_drag_source)
+ *   OOP/hit2d.py:806, 882  mean(detect(q)) for the WENO fraction (mask count)
+ *   OOP/hit2d.py:571-596   the save_snapshot fields.
+ * out: device double[HITX_DIAG_COUNT], the sums below (means are sum / N,
+ *   N = nx*ny; u' = u - sum_u/N, v' likewise).
+ * lowpass_modes_host: n_lowpass (kx, ky) index pairs of the low-pass band
+ *   0 < |k| <= kmax (|kx|, |ky| <= 4, n_lowpass <= 64; 0 = no band: the
+ *   LOW and DRAG_E sums are 0).
+ * drag_coef: c of the drag source s = -c * lowpass(momentum - mean).
+ * mask: uint8 [ny][nx] WENO mask to count, or NULL (MASK_COUNT = 0).
+ * fields: device double [6][ny][nx] = rho, u, v, p, divergence, vorticity,
+ *   or NULL (then the workspace holds them).
+ * ws: hitx_diag_workspace_bytes(plan, fields != NULL) bytes. */
+enum {
+  HITX_DIAG_RHO = 0,      /* sum rho */
+  HITX_DIAG_U = 1,        /* sum u */
+  HITX_DIAG_V = 2,        /* sum v */
+  HITX_DIAG_P = 3,        /* sum p */
+  HITX_DIAG_C = 4,        /* sum sqrt(gamma p / rho) */
+  HITX_DIAG_U2V2 = 5,     /* sum u^2 + v^2 */
+  HITX_DIAG_RUU = 6,      /* sum rho u'^2 */
+  HITX_DIAG_RVV = 7,      /* sum rho v'^2 */
+  HITX_DIAG_RUV = 8,      /* sum rho u' v' */
+  HITX_DIAG_RKE = 9,      /* sum rho (u'^2 + v'^2) */
+  HITX_DIAG_DIV2 = 10,    /* sum divergence^2 */
+  HITX_DIAG_VORT2 = 11,   /* sum vorticity^2 */
+  HITX_DIAG_LOW = 12,     /* sum rho (u_low^2 + v_low^2), low-pass of u', v' */
+  HITX_DIAG_DRAG_E = 13,  /* sum u s_x + v s_y (drag energy source) */
+  HITX_DIAG_MASK = 14,    /* number of mask nodes */
+  HITX_DIAG_COUNT = 15
+};
+size_t hitx_diag_workspace_bytes(const hitx_plan* plan, int32_t with_fields);
+int hitx_diagnostics(hitx_plan* plan, const void* q, const uint8_t* mask, const int32_t* lowpass_modes_host,
+                     int32_t n_lowpass, double drag_coef, double* fields, double* out, void* ws, size_t ws_bytes,
+                     void* stream);
+
+/* Riemann run checks, diagnostics and output fields of q, in double whatever
+ * the plan's dtype.  Replaces
+ *   OOP/problems/riemann_config3.py:432-452  validate_raw_physical_state
+ *   OOP/problems/riemann_config3.py:629-650  _diagnostics (RiemannDiagnostics)
+ *   OOP/problems/riemann_config3.py:459-463  vorticity_z (finite differences of :268-275)
+ *   OOP/problems/riemann_config3.py:497-519  the save_riemann_npz planes.
+ * Raw density and pressure are q0 and (gamma-1)(q3 - rho/2 (u^2+v^2)) with
+ * u = q1/q0, v = q2/q0 (no floors); the vorticity uses the floored
+ * primitives.  Minima and maxima propagate NaN.
+ * out: device double[HITX_RDIAG_COUNT].  mask: uint8 WENO mask to count, or
+ * NULL.  fields: device double [5][ny][nx] = rho, u, v, p (floored
+ * primitives), omega_z; or NULL.  ws: hitx_workspace_bytes(plan). */
+enum {
+  HITX_RDIAG_RHO_MIN = 0,
+  HITX_RDIAG_RHO_MAX = 1,
+  HITX_RDIAG_P_MIN = 2,
+  HITX_RDIAG_P_MAX = 3,
+  HITX_RDIAG_VORT_MAX = 4,     /* max |omega_z| */
+  HITX_RDIAG_NONFINITE_Q = 5,  /* nodes with a non-finite component of q */
+  HITX_RDIAG_NONFINITE_RP = 6, /* nodes with non-finite raw rho or p */
+  HITX_RDIAG_RHO_NONPOS = 7,   /* nodes where raw rho > 0 fails */
+  HITX_RDIAG_P_NONPOS = 8,     /* nodes where raw p > 0 fails */
+  HITX_RDIAG_MASK = 9,         /* mask nodes */
+  HITX_RDIAG_COUNT = 10
+};
+int hitx_riemann_diagnostics(hitx_plan* plan, const void* q, const uint8_t* mask, double* fields, double* out,
+                             void* ws, size_t ws_bytes, void* stream);
+
 /* Forcing noise: the next n standard normals of a NumPy Generator(PCG64)
  * stream, bit for bit as numpy.random.default_rng(seed).standard_normal(n)
  * returns them; replaces the host draw self._rng.standard_normal((ny, nx))

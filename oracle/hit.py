@@ -1,8 +1,9 @@
 """Forced 2-D HIT: initial condition, sources, Mach control and step loop.
 
 Restates reference OOP/hit2d.py:139-204 (init), :264-276 (forcing source),
-:279-349 (Mach control), :352-426 (drag, cooling), :599-615 (equation
-choice) and :764-918 (the step loop, without file output).
+:279-349 (Mach control), :352-426 (drag, cooling), :429-548 (diagnostics),
+:571-596 (snapshot fields), :599-615 (equation choice) and :764-918 (the
+step loop, without file output).
 """
 
 from __future__ import annotations
@@ -171,6 +172,95 @@ def cooling_source(q, gas, tc, p_ref=None):
     return s
 
 
+def spectral_derivatives(a, nx, ny, dx, dy):
+    """:146-155: d/dx, d/dy by fft2 -> i k -> ifft2().real."""
+    kx, ky, _ = wavenumbers(nx, ny, dx, dy)
+    ah = np.fft.fft2(a)
+    return np.fft.ifft2(1j * kx * ah).real, np.fft.ifft2(1j * ky * ah).real
+
+
+def divergence_and_vorticity(u, v, nx, ny, dx, dy):
+    """:459-468."""
+    du_dx, du_dy = spectral_derivatives(u, nx, ny, dx, dy)
+    dv_dx, dv_dy = spectral_derivatives(v, nx, ny, dx, dy)
+    return du_dx + dv_dy, dv_dx - du_dy
+
+
+def large_scale_kinetic_fraction(q, gas, nx, ny, dx, dy, kmax):
+    """:429-449."""
+    if kmax <= 0.0:
+        return 0.0
+    rho, u, v, _ = primitives(q, gas)
+    uf = u - np.mean(u)
+    vf = v - np.mean(v)
+    total = float(np.mean(rho * (uf**2 + vf**2)))
+    if total <= np.finfo(float).eps:
+        return 0.0
+    ul = band_filter(uf, nx, ny, dx, dy, 0.0, kmax)
+    vl = band_filter(vf, nx, ny, dx, dy, 0.0, kmax)
+    return float(np.mean(rho * (ul**2 + vl**2))) / total
+
+
+def source_power(s):
+    """:452-456."""
+    return float(np.mean(s[3]))
+
+
+def compute_diagnostics(q, gas, nx, ny, dx, dy, length_x, length_y, initial_mass=None, forcing_info=None,
+                        drag_power=0.0, cooling_power=0.0, large_scale_fraction=0.0, weno_fraction=0.0):
+    """:471-548."""
+    rho, u, v, pr = primitives(q, gas)
+    div, vort = divergence_and_vorticity(u, v, nx, ny, dx, dy)
+    c = np.sqrt(gas.gamma * pr / rho)
+    mass = float(np.mean(rho) * length_x * length_y)
+    uf = u - np.mean(u)
+    vf = v - np.mean(v)
+    uu = float(np.mean(rho * uf**2))
+    vv = float(np.mean(rho * vf**2))
+    uv = float(np.mean(rho * uf * vf))
+    cs = uu + vv
+    cov = np.sqrt(max(uu * vv, 0.0))
+    d = {
+        "kinetic_energy": 0.5 * cs,
+        "Kx": 0.5 * uu,
+        "Ky": 0.5 * vv,
+        "A_K": abs(uu - vv) / cs if cs > 0.0 else 0.0,
+        "C_uv": uv / cov if cov > 0.0 else 0.0,
+        "rms_velocity": float(np.sqrt(np.mean(u**2 + v**2))),
+        "turbulent_mach": float(np.sqrt(np.mean(u**2 + v**2)) / np.mean(c)),
+        "mean_density": float(np.mean(rho)),
+        "mean_pressure": float(np.mean(pr)),
+        "divergence_rms": float(np.sqrt(np.mean(div**2))),
+        "vorticity_rms": float(np.sqrt(np.mean(vort**2))),
+        "mass": mass,
+        "mass_error": 0.0 if initial_mass is None else mass - initial_mass,
+        "large_scale_kinetic_fraction": large_scale_fraction,
+        "drag_power": drag_power,
+        "cooling_power": cooling_power,
+        "weno_fraction": weno_fraction,
+    }
+    d.update({"P_in": 0.0, "Fxx": 0.0, "Fyy": 0.0, "Fxy": 0.0, "A_F": 0.0, "forcing_alpha": 0.0,
+              "forcing_target_power": 0.0, "mach_control_mach": d["turbulent_mach"], "mach_control_target": 0.0,
+              "mach_control_power_desired": 0.0})
+    if forcing_info is not None:
+        d.update({
+            "P_in": forcing_info["injected_power"], "Fxx": forcing_info["Fxx"], "Fyy": forcing_info["Fyy"],
+            "Fxy": forcing_info["Fxy"], "A_F": forcing_info["A_F"], "forcing_alpha": forcing_info["alpha"],
+            "forcing_target_power": forcing_info.get("target_power", 0.0),
+            "mach_control_mach": forcing_info.get("mach_control_mach", d["turbulent_mach"]),
+            "mach_control_target": forcing_info.get("mach_control_target", 0.0),
+            "mach_control_power_desired": forcing_info.get("mach_control_power_desired", 0.0),
+        })
+    return d
+
+
+def snapshot_fields(q, gas, nx, ny, dx, dy):
+    """:571-596: rho, u, v, pressure, vorticity, divergence."""
+    rho, u, v, pr = primitives(q, gas)
+    div, vort = divergence_and_vorticity(u, v, nx, ny, dx, dy)
+    return {"rho": rho, "u": u, "v": v, "pressure": pr, "vorticity": vort, "divergence": div}
+
+
 def turbulent_mach(rho, u, v, pr, gamma):
     """:279-290."""
     c = np.sqrt(gamma * pr / rho)
@@ -278,6 +368,21 @@ class HITRun:
         self.q = q
         self.last_dt = dt
         return q
+
+    def diagnostics(self, initial=False, initial_mass=None):
+        """The loop's diagnostics of the current state: :806-818 (initial:
+        no forcing information, no source powers) and :881-905."""
+        p, gas, q = self.p, self.gas, self.q
+        weno = float(np.mean(P.sensor(q, gas, p.dx, p.dy, p.sensor_params())))
+        frac = large_scale_kinetic_fraction(q, gas, p.nx, p.ny, p.dx, p.dy, p.large_scale_drag_kmax)
+        args = (q, gas, p.nx, p.ny, p.dx, p.dy, p.length, p.length)
+        if initial:
+            return compute_diagnostics(*args, large_scale_fraction=frac, weno_fraction=weno)
+        drag = drag_source(q, gas, p.nx, p.ny, p.dx, p.dy, p.large_scale_drag, p.large_scale_drag_kmax)
+        cool = cooling_source(q, gas, p.cooling_time, p.cooling_target_pressure)
+        return compute_diagnostics(*args, initial_mass=initial_mass, forcing_info=self.info,
+                                   drag_power=source_power(drag), cooling_power=source_power(cool),
+                                   large_scale_fraction=frac, weno_fraction=weno)
 
     def run(self):
         while not self.done():

@@ -1,8 +1,9 @@
 """Lax-Liu Riemann configurations 3, 3 @ (0.8, 0.8) and 6, and the fixed-dt loop.
 
 Restates reference OOP/problems/riemann_config3.py:19-99 (config 3 and its
-initial state), :667-749 (run_riemann), OOP/problems/riemann_config3_08.py:9-19
-and OOP/problems/riemann_config6.py:40-87.
+initial state), :432-463 and :629-650 (raw density and pressure, vorticity,
+run diagnostics), :497-519 (the saved planes), :667-749 (run_riemann),
+OOP/problems/riemann_config3_08.py:9-19 and OOP/problems/riemann_config6.py:40-87.
 """
 
 from __future__ import annotations
@@ -12,7 +13,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import nonperiodic as N
-from .equations import Gas, conservative, enforce
+from .equations import Gas, conservative, enforce, primitives
 from .rk import ssprk3
 
 # (rho, u, v, p) per quadrant: I = top-right, II = top-left, III = bottom-left, IV = bottom-right
@@ -104,6 +105,43 @@ def initial_state(p: RiemannParams):
         for k in range(4):
             prim[k][sel] = st[k]
     return conservative(*prim, p.gas())
+
+
+def raw_density_pressure(q, gas: Gas):
+    """riemann_config3.py:432-439 (no floors)."""
+    rho = q[0]
+    u = q[1] / rho
+    v = q[2] / rho
+    return rho, (gas.gamma - 1.0) * (q[3] - 0.5 * rho * (u**2 + v**2))
+
+
+def vorticity_z(q, gas: Gas, dx, dy):
+    """riemann_config3.py:459-463."""
+    _rho, u, v, _p = primitives(q, gas)
+    return N.fd(v, dx, -1) - N.fd(u, dy, -2)
+
+
+def diagnostics(q, p: RiemannParams, time, steps, hv_applications):
+    """riemann_config3.py:629-650 as a dict of the RiemannDiagnostics fields."""
+    gas = p.gas()
+    rho, pr = raw_density_pressure(q, gas)
+    w = vorticity_z(q, gas, p.dx, p.dy)
+    return dict(
+        backend=None, scheme=p.scheme, hyperviscosity_enabled=hyperviscosity_enabled(p),
+        hyperviscosity_applications=hv_applications,
+        final_sensor_fraction=float(np.mean(N.sensor(q, gas, p.dx, p.dy, p.sensor_params()))),
+        density_min=float(np.min(rho)), density_max=float(np.max(rho)), pressure_min=float(np.min(pr)),
+        pressure_max=float(np.max(pr)), max_abs_vorticity=float(np.max(np.abs(w))),
+        has_nonfinite=bool(np.any(~np.isfinite(q)) | np.any(~np.isfinite(rho)) | np.any(~np.isfinite(pr))),
+        density_positive=bool(np.all(rho > 0.0)), pressure_positive=bool(np.all(pr > 0.0)), time=time,
+        steps=steps)
+
+
+def saved_fields(q, p: RiemannParams):
+    """riemann_config3.py:497-519: the planes save_riemann_npz writes."""
+    gas = p.gas()
+    rho, u, v, pr = primitives(q, gas)
+    return {"rho": rho, "u": u, "v": v, "pressure": pr, "omega_z": vorticity_z(q, gas, p.dx, p.dy)}
 
 
 def hyperviscosity_enabled(p: RiemannParams):

@@ -234,3 +234,62 @@ def test_riemann_initial_states_reference_tests():
     np.testing.assert_array_equal(u, [[-0.75] * 4, [-0.75] * 4, [0.75] * 4, [0.75] * 4])
     np.testing.assert_array_equal(v, [[0.5, 0.5, -0.5, -0.5]] * 4)
     np.testing.assert_array_equal(pr, np.ones((4, 4)))
+
+
+# ---------------------------------------------------------------- diagnostics
+@pytest.mark.parametrize("nx,ny", [(32, 24), (33, 25)])
+def test_spectral_derivatives_of_a_plane_wave(nx, ny):
+    """d/dx, d/dy of sin(3x + 2y) on the 2 pi box are 3 cos, 2 cos (even and odd n)."""
+    L = 2.0 * np.pi
+    dx, dy = L / nx, L / ny
+    X, Y = np.meshgrid(dx * (np.arange(nx) + 0.5), dy * (np.arange(ny) + 0.5), indexing="xy")
+    gx, gy = H.spectral_derivatives(np.sin(3 * X + 2 * Y), nx, ny, dx, dy)
+    assert np.abs(gx - 3 * np.cos(3 * X + 2 * Y)).max() < 1e-12
+    assert np.abs(gy - 2 * np.cos(3 * X + 2 * Y)).max() < 1e-12
+
+
+def _shear_state(n, A, B, C, p0, gamma=1.4):
+    L = 2.0 * np.pi
+    h = L / n
+    X, Y = np.meshgrid(h * (np.arange(n) + 0.5), h * (np.arange(n) + 0.5), indexing="xy")
+    u = A * np.sin(Y) + C * np.sin(3 * X)
+    v = B * np.sin(X)
+    return conservative(np.ones_like(u), u, v, np.full_like(u, p0), Gas(gamma=gamma)), h
+
+
+def test_diagnostics_closed_forms():
+    """rho = 1, u = A sin y + C sin 3x, v = B sin x, p = p0: every diagnostic
+    in closed form (moments, Mt, divergence, vorticity, low-k fraction with the
+    |k| = 3 mode outside the band, drag and cooling powers)."""
+    n, A, B, C, p0, g = 32, 0.3, 0.2, 0.1, 0.8, 1.4
+    q, h = _shear_state(n, A, B, C, p0, g)
+    gas = Gas(gamma=g)
+    L = 2.0 * np.pi
+    frac = H.large_scale_kinetic_fraction(q, gas, n, n, h, h, 2.0)
+    assert frac == pytest.approx((A**2 + B**2) / (A**2 + B**2 + C**2), rel=1e-12)
+    drag = H.source_power(H.drag_source(q, gas, n, n, h, h, 0.1, 2.0))
+    assert drag == pytest.approx(-0.1 * (A**2 + B**2) / 2, rel=1e-12)
+    cool = H.source_power(H.cooling_source(q, gas, 5.0))
+    assert cool == pytest.approx(-(p0 - 1 / g) / (g - 1) / 5.0, rel=1e-12)
+    d = H.compute_diagnostics(q, gas, n, n, h, h, L, L, initial_mass=L * L * 0.5)
+    uu, vv = (A**2 + C**2) / 2, B**2 / 2
+    assert d["kinetic_energy"] == pytest.approx((uu + vv) / 2, rel=1e-12)
+    assert d["Kx"] == pytest.approx(uu / 2, rel=1e-12)
+    assert d["Ky"] == pytest.approx(vv / 2, rel=1e-12)
+    assert d["A_K"] == pytest.approx(abs(uu - vv) / (uu + vv), rel=1e-12)
+    assert abs(d["C_uv"]) < 1e-14
+    rms = np.sqrt(uu + vv)
+    assert d["rms_velocity"] == pytest.approx(rms, rel=1e-12)
+    assert d["turbulent_mach"] == pytest.approx(rms / np.sqrt(g * p0), rel=1e-12)
+    assert d["mean_density"] == pytest.approx(1.0, rel=1e-14)
+    assert d["mean_pressure"] == pytest.approx(p0, rel=1e-12)
+    assert d["divergence_rms"] == pytest.approx(3 * C / np.sqrt(2), rel=1e-12)  # du/dx = 3C cos 3x
+    # vorticity = dv/dx - du/dy = B cos x - A cos y - 0
+    assert d["vorticity_rms"] == pytest.approx(np.sqrt((A**2 + B**2) / 2), rel=1e-12)
+    assert d["mass"] == pytest.approx(L * L, rel=1e-14)
+    assert d["mass_error"] == pytest.approx(L * L * 0.5, rel=1e-12)
+    assert d["mach_control_mach"] == d["turbulent_mach"] and d["P_in"] == 0.0
+    f = H.snapshot_fields(q, gas, n, n, h, h)
+    X, Y = np.meshgrid(h * (np.arange(n) + 0.5), h * (np.arange(n) + 0.5), indexing="xy")
+    assert np.abs(f["vorticity"] - (B * np.cos(X) - A * np.cos(Y))).max() < 1e-12
+    assert np.abs(f["divergence"] - 3 * C * np.cos(3 * X)).max() < 1e-12
