@@ -73,6 +73,8 @@ SIGNATURES = {
     "hitx_diag_workspace_bytes": (ctypes.c_size_t, [c_vp, c_i32]),
     "hitx_diagnostics": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),
     "hitx_riemann_diagnostics": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_hit_initial_state": (c_i32, [c_vp, c_vp, c_i32, c_dbl, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_riemann_initial_state": (c_i32, [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp]),
     "hitx_timing_enable": (c_i32, [c_vp, c_i32]),
     "hitx_set_line_form": (c_i32, [c_vp, c_i32]),
     "hitx_normals_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),

@@ -199,8 +199,8 @@ class BatchRunner:
         c, sp, k = self.configs[i], self.specs[i], self.kpars[i]
         if self.hit:
             domain = self.domains[i]
-            raw = initialize_hit_2d(c, domain, _make_equation(c), np.random.default_rng(c.forcing_seed))
-            q0 = torch.from_numpy(raw).to("cuda", sp.dtype)
+            q0 = initialize_hit_2d(c, domain, _make_equation(c), np.random.default_rng(c.forcing_seed),
+                                   dtype=sp.dtype)
         else:
             q0 = c.initial_state()
         get_plan(sp).enforce(q0, apply_guard=not self.hit, out=self.q[i])

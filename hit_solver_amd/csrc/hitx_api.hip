@@ -12,6 +12,7 @@
 #include "../../include/hitx.h"
 #include "hitx_batch.h"
 #include "hitx_diag.h"
+#include "hitx_init.h"
 #include "hitx_line.h"
 
 using namespace hitx;
@@ -780,12 +781,80 @@ int hitx_diagnostics(hitx_plan* pl, const void* q, const uint8_t* mask, const in
   if (M.n > 0) k_diag_modes<<<kDiagF * M.n, kBlock, 0, s>>>(rowp, ny, M, modes);
   const double* dkx = pl->d_dk;
   const double* dky = pl->d_dk + 2 * (size_t)nx + 2 * kDiagPad;
-  k_diag_deriv<0><<<dim3((ny + kDerLines - 1) / kDerLines, (nx + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
+  k_diag_deriv<0, false><<<dim3((ny + kDerLines - 1) / kDerLines, (nx + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
       nx, ny, dkx, u, v, dvg, vort);
-  k_diag_deriv<1><<<dim3((nx + kDerLines - 1) / kDerLines, (ny + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
+  k_diag_deriv<1, true><<<dim3((nx + kDerLines - 1) / kDerLines, (ny + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
       nx, ny, dky, v, u, dvg, vort);
   k_diag_moments<<<ny, kBlock, 0, s>>>(nx, ny, rho, u, v, dvg, vort, mask, out, modes, M, -drag_coef, rowm);
   k_finalize_sum<<<kDiagMomW, kBlock, 0, s>>>(rowm, ny, kDiagMomW, out + kDiagS);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+int hitx_hit_initial_state(hitx_plan* pl, const double* modes_host, int32_t n_modes, double target_mach,
+                           void* q_out, double* out_dev, void* ws, size_t ws_bytes, void* stream) {
+  if (!pl) return fail(HITX_EINVAL, "null plan");
+  if (!q_out || !out_dev) return fail(HITX_EINVAL, "null array");
+  if (!pl->d.periodic || !pl->d_dk) return fail(HITX_EINVAL, "the HIT initial state needs a periodic plan");
+  if (n_modes < 1 || !modes_host) return fail(HITX_EINVAL, "no streamfunction modes");
+  const size_t N = pl->N;
+  if ((size_t)n_modes * kInitModeW > N)
+    return fail(HITX_EINVAL, "more streamfunction modes than nx*ny/4");
+  const size_t need = hitx_diag_workspace_bytes(pl, 0);
+  if (!ws || ws_bytes < need) return fail(HITX_ENOMEM, "workspace too small: need " + std::to_string(need) + " bytes");
+  const int nx = pl->d.nx, ny = pl->d.ny;
+  double* F = static_cast<double*>(ws);
+  // planes: psi, d/dx, d/dy, discard, block partials + sums (3 nblk + 4 <= N
+  // doubles since nblk <= N/256 + 1 and N >= 64), mode table
+  double *psi = F, *dpx = F + N, *dpy = F + 2 * N, *dmy = F + 3 * N, *modes = F + 5 * N;
+  double* part = F + 4 * N;
+  double* sums = part + (size_t)3 * pl->nblk_red;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIPCHK(hipMemcpyAsync(modes, modes_host, (size_t)n_modes * kInitModeW * sizeof(double), hipMemcpyHostToDevice, s));
+  const hitx_desc& d = pl->d;
+  const double x0 = 0.0, y0 = 0.0;  // HIT domain origin (OOP/hit2d.py:114-119)
+  k_init_psi<<<pl->nblk_red, kBlock, 0, s>>>(nx, ny, d.dx, d.dy, x0, y0, modes, n_modes, psi);
+  const double* dkx = pl->d_dk;
+  const double* dky = pl->d_dk + 2 * (size_t)nx + 2 * kDiagPad;
+  k_diag_deriv<0, false><<<dim3((ny + kDerLines - 1) / kDerLines, (nx + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
+      nx, ny, dkx, psi, psi, dpx, dmy);
+  k_diag_deriv<1, false><<<dim3((nx + kDerLines - 1) / kDerLines, (ny + kDerPos - 1) / kDerPos), kBlock, 0, s>>>(
+      nx, ny, dky, psi, psi, dpy, dmy);
+  const double p0 = 1.0 / d.gamma;
+  const double c0 = std::sqrt(d.gamma * p0 / 1.0);
+  k_init_sums<<<pl->nblk_red, kBlock, 0, s>>>((int)N, dpx, dpy, c0, part);
+  k_finalize_sum<<<3, kBlock, 0, s>>>(part, pl->nblk_red, 3, sums);
+  k_init_rms<<<pl->nblk_red, kBlock, 0, s>>>((int)N, dpx, dpy, sums, part);
+  k_finalize_sum<<<1, kBlock, 0, s>>>(part, pl->nblk_red, 1, sums + 3);
+  k_init_scale<<<1, 64, 0, s>>>((int)N, sums, target_mach, out_dev);
+  const double rho0 = std::max(1.0, d.rho_floor), pf = std::max(p0, d.p_floor);
+  if (pl->esize == 8)
+    k_init_hit_write<double><<<pl->nblk_red, kBlock, 0, s>>>((int)N, dpx, dpy, sums, out_dev, d.gamma, rho0, pf,
+                                                             static_cast<double*>(q_out));
+  else
+    k_init_hit_write<float><<<pl->nblk_red, kBlock, 0, s>>>((int)N, dpx, dpy, sums, out_dev, d.gamma, rho0, pf,
+                                                            static_cast<float*>(q_out));
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+int hitx_riemann_initial_state(hitx_plan* pl, const double* quadrants_host, double x_min, double y_min,
+                               double x_split, double y_split, void* q_out, void* stream) {
+  if (!pl) return fail(HITX_EINVAL, "null plan");
+  if (!q_out || !quadrants_host) return fail(HITX_EINVAL, "null array");
+  Quadrants Qd;
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 4; ++c) Qd.s[k][c] = quadrants_host[4 * k + c];
+  const hitx_desc& d = pl->d;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (pl->esize == 8)
+    k_init_riemann<double><<<pl->nblk_red, kBlock, 0, s>>>(d.nx, d.ny, d.dx, d.dy, x_min, y_min, x_split, y_split, Qd,
+                                                           d.gamma, d.rho_floor, d.p_floor,
+                                                           static_cast<double*>(q_out));
+  else
+    k_init_riemann<float><<<pl->nblk_red, kBlock, 0, s>>>(d.nx, d.ny, d.dx, d.dy, x_min, y_min, x_split, y_split, Qd,
+                                                          d.gamma, d.rho_floor, d.p_floor,
+                                                          static_cast<float*>(q_out));
   LAUNCHCHK();
   return HITX_OK;
 }

@@ -104,14 +104,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Spectral derivative along AXIS (0: x, lines are rows; 1: y, lines are
-// columns) of two planes a and b.  AXIS 0 writes oa = da/dx, ob = db/dx;
-// AXIS 1 updates oa = oa + da/dy, ob = ob - db/dy (divergence du/dx + dv/dy
-// and vorticity dv/dx - du/dy with a = v, b = u).  dk is the extended table
-// dk[t] = d((t - kDiagPad) mod n), t < 2 n + 2 kDiagPad.
+// columns) of two planes a and b.  ACC = false writes oa = da, ob = db;
+// ACC = true (AXIS 1 only) updates oa = oa + da/dy, ob = ob - db/dy
+// (divergence du/dx + dv/dy and vorticity dv/dx - du/dy with a = v, b = u).
+// dk is the extended table dk[t] = d((t - kDiagPad) mod n), t < 2 n + 2 kDiagPad.
 constexpr int kDerLines = 64, kDerPos = 32, kDerPer = kDerPos / kWaves;
 constexpr int kDerLds = (kDerLines * (kDerPos + 1) > kDerPos * (kDerLines + 1)) ? kDerLines * (kDerPos + 1)
                                                                               : kDerPos * (kDerLines + 1);
-template <int AXIS>
+template <int AXIS, bool ACC>
 __global__ void __launch_bounds__(kBlock)
     k_diag_deriv(int nx, int ny, const double* __restrict__ dk, const double* __restrict__ a,
                  const double* __restrict__ b, double* __restrict__ oa, double* __restrict__ ob) {
@@ -160,8 +160,13 @@ __global__ void __launch_bounds__(kBlock)
         const int p = pg + i;
         if (p < n) {
           const size_t idx = (size_t)p * nx + gl;
-          oa[idx] = oa[idx] + ca[i];
-          ob[idx] = ob[idx] - cb[i];
+          if (ACC) {
+            oa[idx] = oa[idx] + ca[i];
+            ob[idx] = ob[idx] - cb[i];
+          } else {
+            oa[idx] = ca[i];
+            ob[idx] = cb[i];
+          }
         }
       }
     }

@@ -7,8 +7,8 @@ clip, Mach control, one forcing update, one fused SSP-RK3 step, and the
 compact-node hyperviscosity pass every ``hyperviscosity_interval`` steps,
 with the reference's diagnostics and files (config.json, hit2d_stepNNNNNNN.npz
 snapshots, diagnostic_history.npz; OOP/hit2d.py:429-761) from one device
-reduction call per sample.  The initial condition is built on the host
-(setup, once).
+reduction call per sample.  The initial condition is built on the device
+from the phases the reference's generator draws on the host.
 """
 
 from __future__ import annotations
@@ -109,13 +109,12 @@ def create_grid(config: HIT2DConfig):
     return domain, X, Y
 
 
-def initialize_hit_2d(config: HIT2DConfig, domain: Domain2D, equation, rng: np.random.Generator):
-    """OOP/hit2d.py:158-204 (host, setup): random-phase streamfunction on the
-    half-plane of the shell, amplitude 1/k^2, spectral curl, mean removal,
-    rho = 1, p = 1/gamma, velocity scaled to the target turbulent Mach number.
-    Returns the conservative state as float64 numpy [4, ny, nx]."""
-    X, Y = domain.mesh()
-    psi = np.zeros_like(X)
+def streamfunction_modes(config: HIT2DConfig, rng: np.random.Generator) -> np.ndarray:
+    """Rows (kx, ky, amplitude, phase) of the initial streamfunction in the
+    reference's loop order, phases drawn from rng as the reference draws them
+    (OOP/hit2d.py:180-191): one of each +/- pair, kmin <= |k| <= kmax,
+    amplitude 1/|k|^2."""
+    rows = []
     K = config.initial_kmax
     for kx in range(-K, K + 1):
         for ky in range(-K, K + 1):
@@ -124,33 +123,26 @@ def initialize_hit_2d(config: HIT2DConfig, domain: Domain2D, equation, rng: np.r
             mag = np.sqrt(kx**2 + ky**2)
             if mag < config.initial_kmin or mag > config.initial_kmax:
                 continue
-            amp = 1.0 / max(kx**2 + ky**2, 1)
-            phase = rng.uniform(0.0, 2.0 * np.pi)
-            psi += amp * np.cos(kx * X + ky * Y + phase)
-    kxv = 2.0 * np.pi * np.fft.fftfreq(domain.nx, d=domain.dx)
-    kyv = 2.0 * np.pi * np.fft.fftfreq(domain.ny, d=domain.dy)
-    KX, KY = np.meshgrid(kxv, kyv, indexing="xy")
-    ph = np.fft.fft2(psi)
-    u = np.fft.ifft2(1j * KY * ph).real
-    v = -np.fft.ifft2(1j * KX * ph).real
-    u -= np.mean(u)
-    v -= np.mean(v)
-    rho = np.ones_like(u)
-    p = np.full_like(u, 1.0 / config.gamma)
-    c = np.sqrt(equation.gamma * p / rho)
-    vrms = np.sqrt(np.mean(u**2 + v**2))
-    if float(vrms) <= np.finfo(float).eps:
+            rows.append((kx, ky, 1.0 / max(kx**2 + ky**2, 1), rng.uniform(0.0, 2.0 * np.pi)))
+    return np.array(rows, dtype=np.float64).reshape(-1, 4)
+
+
+def initialize_hit_2d(config: HIT2DConfig, domain: Domain2D, equation, rng: np.random.Generator,
+                      dtype=torch.float64, out=None):
+    """OOP/hit2d.py:158-204 on the device (hitx_hit_initial_state): the
+    streamfunction of streamfunction_modes at the cell centres, u = dpsi/dy,
+    v = -dpsi/dx (spectral), mean removal, rho = 1, p = 1/gamma, velocity
+    scaled to the target turbulent Mach number.  Returns the conservative
+    state [4, ny, nx] of dtype on the device (formed in double)."""
+    modes = np.ascontiguousarray(streamfunction_modes(config, rng))
+    if len(modes) == 0:
         raise ValueError("initial streamfunction produced zero velocity")
-    scale = config.target_mach * float(np.mean(c)) / vrms
-    u *= scale
-    v *= scale
-    rho = np.maximum(rho, equation.rho_floor)
-    p = np.maximum(p, equation.pressure_floor)
-    q = np.zeros((4, domain.ny, domain.nx))
-    q[0] = rho
-    q[1] = rho * u
-    q[2] = rho * v
-    q[3] = p / (equation.gamma - 1.0) + 0.5 * rho * (u**2 + v**2)
+    plan = get_plan(PlanSpec(nx=domain.nx, ny=domain.ny, dx=domain.dx, dy=domain.dy, periodic=True, dtype=dtype,
+                             gamma=equation.gamma, rho_floor=equation.rho_floor, p_floor=equation.pressure_floor))
+    q = torch.empty((4, domain.ny, domain.nx), dtype=dtype, device="cuda") if out is None else out
+    rms = plan.hit_initial_state(modes, config.target_mach, q)
+    if rms <= np.finfo(float).eps:
+        raise ValueError("initial streamfunction produced zero velocity")
     return q
 
 
@@ -466,7 +458,7 @@ class HITSolver:
             max_rescale=c.max_forcing_rescale, alpha_memory=c.forcing_alpha_memory, seed=c.forcing_seed + 1,
             dtype=self.dtype)
         if q0 is None:
-            q0 = torch.from_numpy(initialize_hit_2d(c, d, self.equation, rng)).to("cuda", self.dtype)
+            q0 = initialize_hit_2d(c, d, self.equation, rng, dtype=self.dtype)
         eq = self.equation
         self.q = eq.enforce_physical_state(q0)
         dm = drag_modes(d, c.large_scale_drag_kmax) if c.large_scale_drag > 0.0 and c.large_scale_drag_kmax > 0.0 \

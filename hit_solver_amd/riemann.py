@@ -5,7 +5,7 @@ NonPeriodicEulerShockSensor2D, NonPeriodicHybridEuler2DOperator,
 LocalHyperviscosity2D, apply_outflow_guard, validate_raw_physical_state,
 hyperviscosity_enabled_for_config, run_riemann), riemann_config3_08.py and
 riemann_config6.py.  The operator, sensor, hyperviscosity and guard run as
-HIP kernels; the quadrant initial state is built on the host (setup).
+HIP kernels, and so does the quadrant initial state.
 """
 
 from __future__ import annotations
@@ -76,28 +76,14 @@ class RiemannConfig3:
     def torch_dtype(self):
         return torch.float32 if self.dtype == "fp32" else torch.float64
 
-    def initial_primitives(self):
-        """Quadrant states on the cell centres (riemann_config3.py:60-99)."""
-        d = self.domain
-        X, Y = np.meshgrid(d.x, d.y, indexing="xy")
-        top, right = Y >= self.y_split, X >= self.x_split
-        q1, q2, q3, q4 = self.quadrants
-        prim = [np.full((self.ny, self.nx), q3[k]) for k in range(4)]
-        for sel, st in ((top & right, q1), (top & ~right, q2), (~top & right, q4)):
-            for k in range(4):
-                prim[k][sel] = st[k]
-        return prim
-
     def initial_state(self):
-        rho, u, v, p = self.initial_primitives()
-        rho = np.maximum(rho, 1e-10)
-        p = np.maximum(p, 1e-10)
-        q = np.zeros((4, self.ny, self.nx))
-        q[0] = rho
-        q[1] = rho * u
-        q[2] = rho * v
-        q[3] = p / (self.gamma - 1.0) + 0.5 * rho * (u**2 + v**2)
-        return torch.from_numpy(q).to(device="cuda", dtype=self.torch_dtype)
+        """Conservative quadrant state on the device (hitx_riemann_initial_state),
+        formed in double with the floors of OOP/equations.py:147-163."""
+        d, eq = self.domain, self.equation
+        plan = get_plan(PlanSpec(nx=self.nx, ny=self.ny, dx=d.dx, dy=d.dy, periodic=False, dtype=self.torch_dtype,
+                                 gamma=self.gamma, rho_floor=eq.rho_floor, p_floor=eq.pressure_floor))
+        q = torch.empty((4, self.ny, self.nx), dtype=self.torch_dtype, device="cuda")
+        return plan.riemann_initial_state(self.quadrants, self.x_min, self.y_min, self.x_split, self.y_split, q)
 
 
 @dataclass(frozen=True)

@@ -225,6 +225,26 @@ class Plan:
         sums = out.cpu().numpy()
         return (sums, planes) if fields else sums
 
+    def hit_initial_state(self, modes, target_mach: float, out):
+        """HIT initial state into out from (kx, ky, amplitude, phase) rows;
+        returns the velocity rms before scaling (synchronises)."""
+        self._q(out, "out")
+        modes = np.ascontiguousarray(np.asarray(modes, dtype=np.float64).reshape(-1, 4))
+        need = int(lib().hitx_diag_workspace_bytes(self.handle, 0))
+        ws = torch.empty(need, dtype=torch.uint8, device=out.device)  # setup only: not kept
+        sc = torch.empty(2, dtype=torch.float64, device=out.device)
+        check(lib().hitx_hit_initial_state(self.handle, modes.ctypes.data_as(ctypes.c_void_p), len(modes),
+                                           float(target_mach), _p(out), _p(sc), _p(ws), need, _stream()))
+        return float(sc[0].cpu())
+
+    def riemann_initial_state(self, quadrants, x_min, y_min, x_split, y_split, out):
+        """Quadrant initial state into out (rows (rho, u, v, p) of I, II, III, IV)."""
+        self._q(out, "out")
+        qd = np.ascontiguousarray(np.asarray(quadrants, dtype=np.float64).reshape(4, 4))
+        check(lib().hitx_riemann_initial_state(self.handle, qd.ctypes.data_as(ctypes.c_void_p), float(x_min),
+                                               float(y_min), float(x_split), float(y_split), _p(out), _stream()))
+        return out
+
     def riemann_diagnostics(self, q, mask_u8=None, fields: bool = False):
         """HITX_RDIAG_* values of q (synchronises) and, when fields, the device
         planes [5, ny, nx] = rho, u, v, p, omega_z (float64)."""

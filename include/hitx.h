@@ -232,6 +232,29 @@ enum {
 int hitx_riemann_diagnostics(hitx_plan* plan, const void* q, const uint8_t* mask, double* fields, double* out,
                              void* ws, size_t ws_bytes, void* stream);
 
+/* Initial states on the device, formed in double and stored in the plan's
+ * dtype (the reference builds them with NumPy on the host).
+ * hitx_hit_initial_state replaces OOP/hit2d.py:158-204 initialize_hit_2d
+ *   (periodic plan, domain origin 0):
+ *   modes_host: n_modes rows (kx, ky, amplitude, phase) in the reference's
+ *     loop order, phases drawn on the host from its generator;
+ *   psi = sum a cos(kx x + ky y + phase) at the cell centres; u = dpsi/dy,
+ *   v = -dpsi/dx (spectral); mean removal; scale = target_mach * mean(c) /
+ *   rms(u, v) with rho = 1, p = 1/gamma; conservative variables with the
+ *   floors of OOP/equations.py:147-163.
+ *   out_dev: device double[2] = rms before scaling (the caller raises when it
+ *   is <= machine epsilon, as the reference does), scale.
+ *   n_modes * 4 <= nx * ny; ws: hitx_diag_workspace_bytes(plan, 0) bytes.
+ * hitx_riemann_initial_state replaces OOP/problems/riemann_config3.py:60-99
+ *   (riemann_config3_08.py:9-19, riemann_config6.py:40-87):
+ *   quadrants_host: 4 rows (rho, u, v, p) for I top-right, II top-left,
+ *   III bottom-left, IV bottom-right; top = y >= y_split, right = x >= x_split
+ *   at the cell centres x_min + dx (i + 1/2), y_min + dy (j + 1/2). */
+int hitx_hit_initial_state(hitx_plan* plan, const double* modes_host, int32_t n_modes, double target_mach,
+                           void* q_out, double* out_dev, void* ws, size_t ws_bytes, void* stream);
+int hitx_riemann_initial_state(hitx_plan* plan, const double* quadrants_host, double x_min, double y_min,
+                               double x_split, double y_split, void* q_out, void* stream);
+
 /* Forcing noise: the next n standard normals of a NumPy Generator(PCG64)
  * stream, bit for bit as numpy.random.default_rng(seed).standard_normal(n)
  * returns them; replaces the host draw self._rng.standard_normal((ny, nx))
