@@ -42,9 +42,10 @@ HBM_PEAK_GBS = 8000.0
 STEP_BYTES = {"fp64": 614.0, "fp32": 310.0}
 # forced HIT (forcing, drag, cooling, viscous terms; hyperviscosity every 5 steps)
 HIT_STEP_BYTES = {"fp64": 635.0, "fp32": 320.0}
-# HBM bytes per line-sweep launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 512 MiB copy
-# kernel's readings; profiles/k_line_riemann_c3_512_fp64.md), for the workloads that were profiled
-LINE_TRAFFIC = {(512, "fp64"): 17_445_171}
+# HBM bytes per line-sweep launch (both sweeps of a stage) measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (scaled by a 512 MiB copy kernel's readings; profiles/k_line_riemann_c3_512_fp64.md), for the workloads that
+# were profiled
+LINE_TRAFFIC: dict = {}
 
 
 def parse():
@@ -273,8 +274,9 @@ def main():
     T = max(wall, ev_s)
     value = nx * nx * args.steps / T / 1e6
     W = 8 if args.dtype == "fp64" else 4
-    # dominant kernel: one line sweep = read q (4W) + mask (1 B), write dF (4W) per cell
-    line_bytes = (8 * W + 1) * nx * nx
+    # dominant kernel: the line sweeps of a stage, x and y in one launch; per sweep
+    # read q (4W) + mask (1 B), write dF (4W) per cell
+    line_bytes = 2 * (8 * W + 1) * nx * nx
     line_avg_s = line_ms / 1e3 / max(line_n, 1)
     achieved = line_bytes / line_avg_s / 1e9
     out = {
@@ -293,12 +295,13 @@ def main():
         "config": {"workload": f"Riemann C3 {nx}x{nx} {args.dtype} hybrid compact/WENO7, CFL 0.4, "
                                f"mn 0.001 HV every step, guard 2", "nx": nx, "ny": nx, "scheme": "hybrid",
                    "steps_full_solve": s.n_steps, "dt": s.dt},
-        "roofline": {"bound": "hbm", "kernel": "k_line (x and y sweep)", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_line (x and y sweeps of a stage, one launch)",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": LINE_TRAFFIC.get((nx, args.dtype)), "launches": line_n, "avg_launch_us": round(line_avg_s * 1e6, 2),
                      "alg_bytes_per_launch": line_bytes,
-                     "timing": f"HIP events around each sweep launch on its stream, {args.kernel_steps} eager "
-                               "steps before the timed graph replays"},
+                     "timing": f"HIP events around each stage's sweep launch on its stream, {args.kernel_steps} "
+                               "eager steps before the timed graph replays"},
         "graph": "one step (3 fused stages + hyperviscosity) captured as a HIP graph, replayed",
         "step_roofline": {"alg_bytes_per_cell_update": STEP_BYTES[args.dtype],
                           "achieved_GBs": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9, 1),

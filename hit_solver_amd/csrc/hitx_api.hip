@@ -87,13 +87,24 @@ LineK<T> make_linek(int n) {
   const double a = kAlphaC;
   const double rho = (-1.0 + std::sqrt(1.0 - 4.0 * a * a)) / (2.0 * a);
   k.n = n;
-  // largest chunk length that still gives every thread of the line block a task
-  k.L = 1;
-  for (int L : {8, 4, 2, 1})
-    if (n % L == 0 && 4 * (n / L) >= kLineBlock) {
+  // one chunk of L <= 4 per thread of the line block when the line allows it
+  // (the smallest such L: most threads busy); otherwise the largest chunk
+  // length that still gives every thread a task (at most two)
+  k.L = 0;
+  for (int L : {1, 2, 4})
+    if (n % L == 0 && 4 * (n / L) <= kLineBlock) {
       k.L = L;
       break;
     }
+  k.small = k.L != 0;
+  if (!k.small) {
+    k.L = 1;
+    for (int L : {8, 4, 2, 1})
+      if (n % L == 0 && 4 * (n / L) >= kLineBlock) {
+        k.L = L;
+        break;
+      }
+  }
   k.nch = n / k.L;
   k.M = (52 + k.L - 1) / k.L;
   k.rho = (T)rho;
@@ -205,7 +216,40 @@ Params<T> make_params(const hitx_plan* pl) {
 size_t proj_lds_bytes(const hitx_plan* pl) { return (size_t)kWaves * 64 * (2 * pl->fky + 1) * sizeof(double); }
 
 size_t line_lds_bytes(int n, int nch, int esize) {
-  return (size_t)(12 * (n + 1) + 66 + 8 * nch + 20) * esize + 4 * (size_t)(n + 1) + n + 64;
+  return (size_t)(8 * (n + 1) + 66 + 8 * nch + 20) * esize + 4 * (size_t)(n + 1) + n + 64;
+}
+
+// Lines in one-chunk shape (every launched line small) go in one launch;
+// otherwise one launch per sweep.
+template <class T, bool PER>
+void launch_k_line(bool small, size_t lds, hipStream_t s, const Params<T>& P, const LineK<T>& KX, const LineK<T>& KY,
+                   const T* qx, const uint8_t* mx, const T* qy, const uint8_t* my, const double* scal, T* ox, T* oy,
+                   int nlx, int nly) {
+  if (small) {
+    k_line<T, PER, true, 2><<<nlx + nly, kLineBlock, lds, s>>>(P, KX, KY, qx, mx, qy, my, scal, ox, oy, nlx, nly);
+    return;
+  }
+  if (nlx > 0)
+    k_line<T, PER, false, 0><<<nlx, kLineBlock, lds, s>>>(P, KX, KY, qx, mx, qy, my, scal, ox, oy, nlx, nly);
+  if (nly > 0)
+    k_line<T, PER, false, 1><<<nly, kLineBlock, lds, s>>>(P, KX, KY, qx, mx, qy, my, scal, ox, oy, nlx, nly);
+}
+
+template <class T>
+hipError_t allow_line_lds(size_t bytes) {
+  hipError_t e = hipSuccess;
+  const int b = (int)bytes;
+  const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+  const void* fs[] = {
+      reinterpret_cast<const void*>(&k_line<T, true, true, 2>), reinterpret_cast<const void*>(&k_line<T, true, false, 0>),
+      reinterpret_cast<const void*>(&k_line<T, true, false, 1>), reinterpret_cast<const void*>(&k_line<T, false, true, 2>),
+      reinterpret_cast<const void*>(&k_line<T, false, false, 0>),
+      reinterpret_cast<const void*>(&k_line<T, false, false, 1>),
+      reinterpret_cast<const void*>(&b_line<T, true, true>), reinterpret_cast<const void*>(&b_line<T, true, false>),
+      reinterpret_cast<const void*>(&b_line<T, false, true>), reinterpret_cast<const void*>(&b_line<T, false, false>)};
+  for (const void* f : fs)
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, a, b);
+  return e;
 }
 
 size_t sensor_lds_bytes(int w, int esize) {
@@ -318,29 +362,42 @@ bool use_wave(const hitx_plan* pl, int n) {
   return use_wave_form(pl->line_form, n, nl);
 }
 
-// One line sweep along axis 0 (lines of nx, ny of them) or axis 1 (the
-// transposed state: lines of ny, nx of them).
+// The two line sweeps of a stage: x-lines of qs (enforced on load) and
+// y-lines of the enforced transposed state.  Sweeps in the block form go in
+// one launch (the x- and y-line blocks fill the device together); a sweep in
+// the wave form has its own launch.
 template <class T>
-void launch_line(const hitx_plan* pl, const Params<T>& P, int axis, int enforced, const T* in, const uint8_t* mask,
-                 const double* scal, T* out, hipStream_t s) {
-  const int n = axis == 0 ? pl->d.nx : pl->d.ny, nl = axis == 0 ? pl->d.ny : pl->d.nx;
-  const LineK<T>& K = linek<T>(pl, axis);
-  if (use_wave(pl, n)) {
-    const int sb = (int)wave_slice_bytes(n, pl->esize);
+void launch_sweeps(const hitx_plan* pl, const Params<T>& P, const T* qs, const WS& w, hipStream_t s) {
+  const int nx = pl->d.nx, ny = pl->d.ny;
+  const bool wx = use_wave(pl, nx), wy = use_wave(pl, ny);
+  const int sb = (int)wave_slice_bytes(std::max(nx, ny), pl->esize);
+  for (int axis = 0; axis < 2; ++axis) {
+    if (!(axis == 0 ? wx : wy)) continue;
+    const int nl = axis == 0 ? ny : nx;
     const int nb = (nl + kLineWaves - 1) / kLineWaves;
+    const T* in = axis == 0 ? qs : static_cast<const T*>(w.qT);
+    const uint8_t* mk = axis == 0 ? w.mask : w.maskT;
+    T* out = static_cast<T*>(axis == 0 ? w.dfx : w.dfyT);
     if (P.periodic)
-      k_line_wave<T, true><<<nb, 64 * kLineWaves, kLineWaves * sb, s>>>(P, K, axis, enforced, in, mask, scal, out,
-                                                                          nl, sb);
+      k_line_wave<T, true><<<nb, 64 * kLineWaves, kLineWaves * sb, s>>>(P, linek<T>(pl, axis), axis, axis, in, mk,
+                                                                          w.scal, out, nl, sb);
     else
-      k_line_wave<T, false><<<nb, 64 * kLineWaves, kLineWaves * sb, s>>>(P, K, axis, enforced, in, mask, scal, out,
-                                                                           nl, sb);
-    return;
+      k_line_wave<T, false><<<nb, 64 * kLineWaves, kLineWaves * sb, s>>>(P, linek<T>(pl, axis), axis, axis, in, mk,
+                                                                           w.scal, out, nl, sb);
   }
-  const size_t lds = axis == 0 ? pl->line_lds_x : pl->line_lds_y;
+  const int nlx = wx ? 0 : ny, nly = wy ? 0 : nx;
+  if (nlx + nly == 0) return;
+  const size_t lds = std::max(wx ? 0 : pl->line_lds_x, wy ? 0 : pl->line_lds_y);
+  const T* qT = static_cast<const T*>(w.qT);
+  T* fx = static_cast<T*>(w.dfx);
+  T* fyT = static_cast<T*>(w.dfyT);
+  const bool small = (wx || linek<T>(pl, 0).small) && (wy || linek<T>(pl, 1).small);
   if (P.periodic)
-    k_line<T, true><<<nl, kLineBlock, lds, s>>>(P, K, axis, enforced, in, mask, scal, out);
+    launch_k_line<T, true>(small, lds, s, P, linek<T>(pl, 0), linek<T>(pl, 1), qs, w.mask, qT,
+                           w.maskT, w.scal, fx, fyT, nlx, nly);
   else
-    k_line<T, false><<<nl, kLineBlock, lds, s>>>(P, K, axis, enforced, in, mask, scal, out);
+    launch_k_line<T, false>(small, lds, s, P, linek<T>(pl, 0), linek<T>(pl, 1), qs, w.mask, qT,
+                            w.maskT, w.scal, fx, fyT, nlx, nly);
 }
 
 template <class T>
@@ -349,10 +406,7 @@ int run_sweeps(hitx_plan* pl, const T* qs, const WS& w, hipStream_t s) {
   if (rc) return rc;
   const Params<T> P = make_params<T>(pl);
   if ((rc = tmark(pl, s))) return rc;
-  launch_line<T>(pl, P, 0, 0, qs, w.mask, w.scal, static_cast<T*>(w.dfx), s);
-  if ((rc = tmark(pl, s))) return rc;
-  if ((rc = tmark(pl, s))) return rc;
-  launch_line<T>(pl, P, 1, 1, static_cast<const T*>(w.qT), w.maskT, w.scal, static_cast<T*>(w.dfyT), s);
+  launch_sweeps<T>(pl, P, qs, w, s);
   if ((rc = tmark(pl, s))) return rc;
   LAUNCHCHK();
   return HITX_OK;
@@ -366,12 +420,7 @@ int set_line_lds(hitx_plan* pl) {
   if (mx > (size_t)maxsh)
     return fail(HITX_EINVAL, "grid line too long for LDS: need " + std::to_string(mx) + " bytes, device allows " +
                                  std::to_string(maxsh));
-  if (mx > 65536) {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line<T, true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx));
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_line<T, false>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx));
-  }
+  if (mx > 65536) HIPCHK(allow_line_lds<T>(mx));
   return HITX_OK;
 }
 
@@ -961,6 +1010,7 @@ struct hitx_batch {
   int total[OFF_KINDS] = {0};
   std::vector<int> offs;  // OFF_KINDS * (n + 1)
   size_t line_lds = 0, sensor_lds = 0, proj_lds = 0, wave_slice = 16;
+  bool line_small[2] = {true, true};  // every block-form line of the axis in one-chunk shape
   int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave
   bool wave_x = false, wave_y = false;
   int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split
@@ -1104,10 +1154,18 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
         b_line_wave<T, false><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
     }
     if (b->total[kb] > 0) {
-      if (b->periodic)
-        b_line<T, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
-      else
-        b_line<T, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+      const bool sm = b->line_small[axis];
+      if (b->periodic) {
+        if (sm)
+          b_line<T, true, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+        else
+          b_line<T, true, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+      } else {
+        if (sm)
+          b_line<T, false, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+        else
+          b_line<T, false, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+      }
     }
   }
   b_update<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n, st);
@@ -1158,12 +1216,7 @@ int batch_set_lds(hitx_batch* b) {
   int maxsh = 0;
   HIPCHK(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
   if (b->line_lds > (size_t)maxsh) return fail(HITX_EINVAL, "grid line too long for LDS");
-  if (b->line_lds > 65536) {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&b_line<T, true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->line_lds));
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&b_line<T, false>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->line_lds));
-  }
+  if (b->line_lds > 65536) HIPCHK(allow_line_lds<T>(b->line_lds));
   return HITX_OK;
 }
 
@@ -1197,6 +1250,8 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     }
     b->plans.push_back(pl);
     b->line_lds = std::max({b->line_lds, pl->line_lds_x, pl->line_lds_y});
+    b->line_small[0] = b->line_small[0] && pl->lx_d.small;
+    b->line_small[1] = b->line_small[1] && pl->ly_d.small;
     if (line_wave_ok(pl->d.nx)) b->wave_slice = std::max(b->wave_slice, wave_slice_bytes(pl->d.nx, pl->esize));
     if (line_wave_ok(pl->d.ny)) b->wave_slice = std::max(b->wave_slice, wave_slice_bytes(pl->d.ny, pl->esize));
     b->sensor_lds = std::max(b->sensor_lds, pl->sensor_lds);

@@ -135,14 +135,14 @@ __global__ void __launch_bounds__(kBlock) b_dilate(const Inst<T>* __restrict__ t
   dilate_body<T>(I.P, stage_in(I, st), st < 3, I.rawm, I.mask, I.maskT, st < 3 ? I.qT : nullptr, bx, by);
 }
 
-template <class T, bool PER>
+template <class T, bool PER, bool SMALL>
 __global__ void __launch_bounds__(kLineBlock, 4) b_line(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                      int ninst, int axis, int st) {
   HITX_BATCH_INST(axis == 0 ? OFF_LX : OFF_LY)
   if (axis == 0)
-    line_body<T, PER>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, lb, I.P.ny);
+    line_body<T, PER, SMALL>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, lb, I.P.ny);
   else
-    line_body<T, PER>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, lb, I.P.nx);
+    line_body<T, PER, SMALL>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, lb, I.P.nx);
 }
 
 // Wave form of the sweep: kLineWaves lines per block, one per wavefront.

@@ -54,6 +54,7 @@ struct Params {
 template <class T>
 struct LineK {
   int n, L, nch, M;
+  int small;                              // 4 n / L <= 512 with L <= 4: one chunk per thread
   T rho, c1p, inv1mrn, rnm1, one_m_rho2;  // rho, 1+rho^2, 1/(1-rho^n), rho^(n-1), 1-rho^2
   T alpha_c;                              // compact alpha 3/8
   T pw[66];                               // rho^k, k = 0..65 (0 beyond)

@@ -682,8 +682,10 @@ __device__ __forceinline__ T pwk(const T* PW, int k) { return k <= 65 ? PW[k] : 
 // periodic wrap is summed in closed form).  The non-periodic system is the
 // zero-Dirichlet interior problem plus the homogeneous solutions
 // a rho^i + b rho^(n-1-i) that restore the end rows.
-// Requires 4*n/L <= 2*blockDim.x (at most two chunks per thread).
-template <class T, bool PER>
+// Requires 4*n/L <= 2*blockDim.x (at most two chunks per thread); SMALL
+// (lines the plan cuts into chunks of L <= 4, at most one per thread) keeps
+// one chunk of four in registers, which holds the kernel to 64 VGPRs.
+template <class T, bool PER, bool SMALL>
 __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h, const LineK<T>& K,
                                             const T* __restrict__ PW, T* __restrict__ scratch, T* __restrict__ out,
                                             size_t NL, size_t g0) {
@@ -696,10 +698,11 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
     const T* f = F + c * S;
     return PER ? (f[i] - f[wrap(i - 1, n)]) / h : (f[i + 1] - f[i]) / h;
   };
-  T y[2][8];
+  constexpr int NT = SMALL ? 1 : 2, LM = SMALL ? 4 : 8;
+  T y[NT][LM];
   // causal pass, local
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < NT; ++u) {
     const int t = threadIdx.x + u * B;
     if (t < ntask) {
       const int line = t / nch, s0 = (t - line * nch) * L;
@@ -710,7 +713,7 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
       }
       T acc = T(0);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < LM; ++k) {
         if (k < L) {
           const int i = s0 + k;
           T xv = raw(line, i);
@@ -732,7 +735,7 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
   __syncthreads();
   // causal carries, then the anti-causal local pass
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < NT; ++u) {
     const int t = threadIdx.x + u * B;
     if (t < ntask) {
       const int line = t / nch, ch = t - line * nch;
@@ -751,12 +754,12 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
         for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch - m];
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
+      for (int k = 0; k < LM; ++k)
         if (k < L) y[u][k] = y[u][k] + pwk(PW, k + 1) * carry;
       if (!PER && ch == nch - 1) bnd[line * 3 + 0] = (rho * y[u][L - 1]) / K.one_m_rho2;
       T acc = T(0);
 #pragma unroll
-      for (int k = 7; k >= 0; --k) {
+      for (int k = LM - 1; k >= 0; --k) {
         if (k < L) {
           acc = y[u][k] + rho * acc;
           y[u][k] = acc;
@@ -768,7 +771,7 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
   __syncthreads();
   // anti-causal carries, scale by 1 + rho^2
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < NT; ++u) {
     const int t = threadIdx.x + u * B;
     if (t < ntask) {
       const int line = t / nch, ch = t - line * nch;
@@ -789,7 +792,7 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
         if (rem < M) carry = carry + pwk(PW, rem * L) * bnd[line * 3 + 0];
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
+      for (int k = 0; k < LM; ++k)
         if (k < L) y[u][k] = K.c1p * (y[u][k] + pwk(PW, L - k) * carry);
       if (!PER) {
         if (ch == 0) bnd[line * 3 + 1] = y[u][0];
@@ -800,21 +803,21 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
   if (!PER) __syncthreads();
   // (non-periodic: end-row corrections) and store
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < NT; ++u) {
     const int t = threadIdx.x + u * B;
     if (t < ntask) {
       const int line = t / nch, s0 = (t - line * nch) * L;
       T* o = out + line * NL + g0;
       if (PER) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < LM; ++k)
           if (k < L) o[s0 + k] = y[u][k];
       } else {
         const T p0 = bnd[line * 3 + 1], pn = bnd[line * 3 + 2];
         const T det = T(1) - K.rnm1 * K.rnm1;
         const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < LM; ++k) {
           if (k < L) {
             const int i = s0 + k;
             T v;
@@ -839,27 +842,31 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
 // axis 1: y-lines of qT [4][nx][ny], flux_y, alpha_y, dy.
 // OOP/spatial_operator.py:179-249, :321-342; non-periodic
 // OOP/problems/riemann_config3.py:165-265, :344-357.
+// LDS holds two 4 x (n+1) planes, the state and the point flux; per
+// component the WENO7 values are staged in the compact solve's scratch and
+// the hybrid fluxes overwrite that component's state plane, so four line
+// blocks fit a CU.  The WENO work list is built in interface order (wave
+// ballots), so neighbouring lanes read neighbouring LDS words.
 constexpr int kLineBlock = 512;
-template <class T, bool PER>
+template <class T, bool PER, bool SMALL>
 __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K, int axis, int enforced_input,
                                                      const T* __restrict__ ql, const uint8_t* __restrict__ ml,
                                                      const double* __restrict__ scal, T* __restrict__ out, int line, int nl) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int BL = kLineBlock;
+  constexpr int BL = kLineBlock, NWV = kLineBlock / 64;
   const int n = K.n, S = n + 1;
   T* A = reinterpret_cast<T*>(smem);
   T* B = A + 4 * S;
-  T* C = B + 4 * S;
-  T* PW = C + 4 * S;                // 66
+  T* PW = B + 4 * S;                // 66
   T* scratch = PW + 66;             // 8*nch + 20
   int* wl = reinterpret_cast<int*>(scratch + 8 * K.nch + 20);  // WENO work list, n+1
   uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + S);
-  __shared__ int wcnt;
+  __shared__ int wcnt[NWV];
   const size_t NL = (size_t)nl * n;
   const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
   const T h = axis == 0 ? P.dx : P.dy;
+  const int wv_id = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int k = threadIdx.x; k < 66; k += BL) PW[k] = K.pw[k];
-  if (threadIdx.x == 0) wcnt = 0;
   // 1. (enforce), primitives, point flux
   for (int i = threadIdx.x; i < n; i += BL) {
     const size_t g = (size_t)line * n + i;
@@ -894,60 +901,94 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
     return (Mk[clampi(k - 1, 0, n - 1)] | Mk[clampi(k, 0, n - 1)]) != 0;
   };
   auto nidx = [&](int j) -> int { return PER ? wrap(j, n) : clampi(j, 0, n - 1); };
-  // 2a. compact the interfaces whose WENO7 flux a flagged interface's
-  //     compactification reads (i-1, i, i+1 of every flagged i)
-  for (int k = threadIdx.x; k < nI; k += BL)
-    if (P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1)) wl[atomicAdd(&wcnt, 1)] = k;
-  __syncthreads();
-  // 2b. WENO7 of the Lax-Friedrichs split, one (interface, component) per item
-  const int nw = wcnt;
-  for (int t = threadIdx.x; t < 4 * nw; t += BL) {
-    const int c = t / nw, k = wl[t - c * nw];
-    const int il = PER ? k : k - 1;
-    const T* qa = A + c * S;
-    const T* fa = B + c * S;
-    T fp[7], fm[7];
+  // 2a. the interfaces whose WENO7 flux a flagged interface's compactification
+  //     reads (i-1, i, i+1 of every flagged i), in increasing order
+  int nw = 0;
+#pragma unroll 1
+  for (int k0 = 0; k0 < nI; k0 += BL) {
+    const int k = k0 + threadIdx.x;
+    const bool fl = k < nI && (P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1));
+    const uint64_t bal = __ballot(fl);
+    const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+    if (lane == 0) wcnt[wv_id] = __popcll(bal);
+    __syncthreads();
+    int off = nw, tot = 0;
 #pragma unroll
-    for (int j = 0; j < 7; ++j) {
-      const int ip = nidx(il - 3 + j);   // f+ stencil i-3 .. i+3
-      const int im = nidx(il + 4 - j);   // f- stencil i+4 .. i-2
-      fp[j] = T(0.5) * (fa[ip] + alpha * qa[ip]);
-      fm[j] = T(0.5) * (fa[im] - alpha * qa[im]);
+    for (int j = 0; j < NWV; ++j) {
+      off += j < wv_id ? wcnt[j] : 0;
+      tot += wcnt[j];
     }
-    C[c * S + k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
-                   weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
+    if (fl) wl[off + pre] = k;
+    nw += tot;
+    __syncthreads();
   }
-  __syncthreads();
-  // 3. hybrid interface flux into A
+  // 2b/3. one component at a time: WENO7 of the Lax-Friedrichs split at the
+  //       listed interfaces into W (the solve's scratch, free until step 4),
+  //       then the hybrid interface flux -- compactified WENO where flagged,
+  //       smooth compact flux elsewhere -- over that component's state plane
+  T* W = scratch;  // n + 1 <= 8 nch + 20 values
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
-  for (int c = 0; c < 4; ++c)
-  for (int k = threadIdx.x; k < nI; k += BL) {
-    const bool fl = P.weno_only || iflag(k);
-    const int il = PER ? k : k - 1;
-    T F;
-    if (fl) {
-      const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
-      const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
-      const T* wv = C + c * S;
-      F = (K.alpha_c * wv[kr] + wv[k]) + K.alpha_c * wv[kl];
-    } else {
-      const T* f = B + c * S;
-      F = (c3 * (f[nidx(il + 3)] + f[nidx(il - 2)]) + c2 * (f[nidx(il + 2)] + f[nidx(il - 1)])) +
-          c1 * (f[nidx(il + 1)] + f[nidx(il)]);
+  // unrolled for float, rolled for double: each keeps the kernel near
+  // 64 VGPRs (eight wavefronts per SIMD)
+  constexpr int CU = sizeof(T) == 4 ? 4 : 1;
+#pragma unroll CU
+  for (int c = 0; c < 4; ++c) {
+    const T* qa = A + c * S;
+    const T* fa = B + c * S;
+#pragma unroll 1
+    for (int t = threadIdx.x; t < nw; t += BL) {
+      const int k = wl[t];
+      const int il = PER ? k : k - 1;
+      T fp[7], fm[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const int ip = nidx(il - 3 + j);   // f+ stencil i-3 .. i+3
+        const int im = nidx(il + 4 - j);   // f- stencil i+4 .. i-2
+        fp[j] = T(0.5) * (fa[ip] + alpha * qa[ip]);
+        fm[j] = T(0.5) * (fa[im] - alpha * qa[im]);
+      }
+      W[k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
+             weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
     }
-    A[c * S + k] = F;
+    __syncthreads();
+    T* Fa = A + c * S;  // the state plane of c is no longer read
+#pragma unroll 1
+    for (int k = threadIdx.x; k < nI; k += BL) {
+      const bool fl = P.weno_only || iflag(k);
+      const int il = PER ? k : k - 1;
+      T F;
+      if (fl) {
+        const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
+        const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
+        F = (K.alpha_c * W[kr] + W[k]) + K.alpha_c * W[kl];
+      } else {
+        F = (c3 * (fa[nidx(il + 3)] + fa[nidx(il - 2)]) + c2 * (fa[nidx(il + 2)] + fa[nidx(il - 1)])) +
+            c1 * (fa[nidx(il + 1)] + fa[nidx(il)]);
+      }
+      Fa[k] = F;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // 4. divided difference, compact solve, store
-  solve_store<T, PER>(A, S, h, K, PW, scratch, out, NL, (size_t)line * n);
+  solve_store<T, PER, SMALL>(A, S, h, K, PW, scratch, out, NL, (size_t)line * n);
 }
 
-template <class T, bool PER>
-__global__ void __launch_bounds__(kLineBlock, 4) k_line(Params<T> P, LineK<T> K, int axis, int enforced_input,
-                                                     const T* __restrict__ ql, const uint8_t* __restrict__ ml,
-                                                     const double* __restrict__ scal, T* __restrict__ out) {
-  line_body<T, PER>(P, K, axis, enforced_input, ql, ml, scal, out, blockIdx.x, gridDim.x);
+// Line-sweep launch.  AX = 2: both sweeps of a stage in one launch, blocks
+// [0, nlx) take the x-lines of q (enforced on load), blocks [nlx, nlx + nly)
+// the y-lines of the enforced transposed state qT (either count may be 0);
+// AX = 0 or 1: one sweep (one inlined body: the general line shape needs
+// more registers than two bodies in one kernel leave it).
+template <class T, bool PER, bool SMALL, int AX>
+__global__ void __launch_bounds__(kLineBlock, 4)
+    k_line(Params<T> P, LineK<T> KX, LineK<T> KY, const T* __restrict__ qx, const uint8_t* __restrict__ mx,
+           const T* __restrict__ qy, const uint8_t* __restrict__ my, const double* __restrict__ scal,
+           T* __restrict__ ox, T* __restrict__ oy, int nlx, int nly) {
+  const int b = blockIdx.x;
+  if (AX == 0 || (AX == 2 && b < nlx))
+    line_body<T, PER, SMALL>(P, KX, 0, 0, qx, mx, scal, ox, b, P.ny);
+  else
+    line_body<T, PER, SMALL>(P, KY, 1, 1, qy, my, scal, oy, AX == 2 ? b - nlx : b, P.nx);
 }
 
 // ================================================================ stage update

@@ -274,10 +274,11 @@ int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* w
  * at least 2048 such lines.  Both compute the same derivative to rounding. */
 int hitx_set_line_form(hitx_plan* plan, int32_t form);
 
-/* Measurement support (no reference counterpart): when enabled, every
- * line-sweep kernel launch of this plan is bracketed by HIP events on its
- * stream.  hitx_timing_read synchronises those events, returns the summed
- * kernel time and the number of launches since the last read, and resets. */
+/* Measurement support (no reference counterpart): when enabled, the line
+ * sweeps of each stage (x and y in one launch when every line is in the
+ * one-chunk shape, n <= 512) are bracketed by HIP events on their stream.
+ * hitx_timing_read synchronises those events, returns the summed time and
+ * the number of bracketed stages since the last read, and resets. */
 int hitx_timing_enable(hitx_plan* plan, int32_t on);
 int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
 
