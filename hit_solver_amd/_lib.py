@@ -77,6 +77,7 @@ SIGNATURES = {
     "hitx_riemann_initial_state": (c_i32, [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp]),
     "hitx_timing_enable": (c_i32, [c_vp, c_i32]),
     "hitx_set_line_form": (c_i32, [c_vp, c_i32]),
+    "hitx_set_sensor_form": (c_i32, [c_vp, c_i32]),
     "hitx_normals_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
     "hitx_normals": (c_i32, [c_vp, c_vp, ctypes.c_int64, c_i32, c_vp, ctypes.c_size_t, c_vp]),
     "hitx_batch_create": (c_i32, [ctypes.POINTER(HitxDesc), c_vp, c_i32, ctypes.POINTER(c_vp)]),

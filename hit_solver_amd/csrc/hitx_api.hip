@@ -142,6 +142,7 @@ struct hitx_plan {
   int nblk_red = 0, nblk_proj = 0, nblk_stage = 0, nblk_sr = 0;
   int rb = 2, rbp = 4;                // rows per wave of the column walks
   int line_form = 0;                  // 0: auto, 1: block form, 2: wave form where the line allows it
+  int sensor_form = 0;                // 0: auto, 1: fused tile, 2: split (raw-mask column walk + dilation tile)
   int dkx = 0, dky = 0, fkx = 0, fky = 0;
   size_t off_drows = 0, off_frows = 0;
   size_t line_lds_x = 0, line_lds_y = 0;
@@ -319,6 +320,14 @@ dim3 grid16(const hitx_plan* pl) { return dim3((pl->d.nx + 15) / 16, (pl->d.ny +
 
 // stage reductions (one kernel, last block finalizes) + fused sensor tile:
 // mask and maskT of q, and the enforced transposed state when qT != nullptr
+// The fused tile recomputes primitives and jump indicators over its halo
+// (2.1x); the split form computes them once per node, then dilates.
+constexpr long long kSplitSensorMinNodes = 0;
+bool use_split_sensor(const hitx_plan* pl) {
+  if (pl->sensor_form != 0) return pl->sensor_form == 2;
+  return (long long)pl->N >= kSplitSensorMinNodes;
+}
+
 template <class T>
 int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, T* qT, const WS& w, hipStream_t s) {
   const Params<T> P = make_params<T>(pl);
@@ -331,7 +340,12 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
     k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_sr, w.scal);
   }
   dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
-  k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
+  if (use_split_sensor(pl)) {
+    k_rawmask<T><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
+    k_dilate<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.raw, mask_out, w.maskT, qT);
+  } else {
+    k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
+  }
   LAUNCHCHK();
   return HITX_OK;
 }
@@ -962,6 +976,13 @@ int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* w
   k_rng_scatter<<<(J.nch + kRngScatterThreads - 1) / kRngScatterThreads, kRngScatterThreads, 0, s>>>(J);
   k_rng_finish<<<1, 64, 0, s>>>(J);
   LAUNCHCHK();
+  return HITX_OK;
+}
+
+int hitx_set_sensor_form(hitx_plan* pl, int32_t form) {
+  if (!pl) return fail(HITX_EINVAL, "null plan");
+  if (form < 0 || form > 2) return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused) or 2 (split)");
+  pl->sensor_form = form;
   return HITX_OK;
 }
 

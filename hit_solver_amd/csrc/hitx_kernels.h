@@ -656,6 +656,20 @@ __device__ __forceinline__ void dilate_body(const Params<T>& P, const T* __restr
 }
 
 template <class T>
+__global__ void __launch_bounds__(kBlock) k_rawmask(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                    const double* __restrict__ scal, uint8_t* __restrict__ raw) {
+  rawmask_body<T>(P, q, enforce_first, scal, raw, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_dilate(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                   const uint8_t* __restrict__ raw, uint8_t* __restrict__ mask,
+                                                   uint8_t* __restrict__ maskT, T* __restrict__ qT) {
+  const Tile2 t_ = tile_xcd();
+  dilate_body<T>(P, q, enforce_first, raw, mask, maskT, qT, t_.bx, t_.by);
+}
+
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                    const double* __restrict__ scal, uint8_t* __restrict__ mask,
                                                    uint8_t* __restrict__ maskT, T* __restrict__ qT) {

@@ -264,6 +264,10 @@ class Plan:
         64, at most 512), else the block form; 1: the block form everywhere."""
         check(lib().hitx_set_line_form(self.handle, int(form)))
 
+    def set_sensor_form(self, form: int):
+        """0: default (split); 1: fused tile; 2: raw-mask column walk + dilation tile."""
+        check(lib().hitx_set_sensor_form(self.handle, int(form)))
+
     def timing(self, on: bool):
         check(lib().hitx_timing_enable(self.handle, int(on)))
 

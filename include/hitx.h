@@ -274,6 +274,13 @@ int hitx_normals(uint64_t* state, double* out, int64_t n, int32_t flags, void* w
  * at least 2048 such lines.  Both compute the same derivative to rounding. */
 int hitx_set_line_form(hitx_plan* plan, int32_t form);
 
+/* Launch-shape support (no reference counterpart): which form of the shock
+ * sensor the plan launches.  1: one fused tile kernel (primitives and jump
+ * indicators recomputed over the dilation halo); 2: a column walk writing
+ * the undilated mask once per node, then a dilation tile; 0 (default): the
+ * split form.  Both give the same mask. */
+int hitx_set_sensor_form(hitx_plan* plan, int32_t form);
+
 /* Measurement support (no reference counterpart): when enabled, the line
  * sweeps of each stage (x and y in one launch when every line is in the
  * one-chunk shape, n <= 512) are bracketed by HIP events on their stream.

@@ -274,6 +274,38 @@ def test_nonperiodic_rhs_vs_oracle_sizes(nx, ny, scheme):
     assert_rhs(host(op.rhs(dev(q))), ON.rhs(q, Gas(), p.dx, p.dy, sp, scheme), what=f"{nx}x{ny}")
 
 
+@pytest.mark.parametrize("case", ["periodic", "riemann"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_sensor_forms_agree(case, dtype):
+    """The fused tile sensor and the split form (raw-mask column walk, then
+    the dilation tile) give the same mask, bit for bit, and the same rhs."""
+    if case == "periodic":
+        nx, ny = 96, 80
+        p = OH.HITParams(nx=nx, ny=ny, target_mach=0.3, initial_kmin=3, initial_kmax=5)
+        q = OH.initial_state(p, np.random.default_rng(3))
+        x = np.arange(nx) * p.dx
+        q[0] *= 1.0 + 0.6 * (np.sin(x)[None, :] > 0.3)
+        q = o_enforce(q, Gas())
+        _, _, sensor, op, _ = _hit_objs(nx, ny)
+    else:
+        nx, ny = 72, 56
+        p = OR.config3(nx=nx, ny=ny, shear_threshold=None)
+        run = OR.RiemannRun(p)
+        run.run(max_steps=3)
+        q = run.q
+        _, _, sensor, op, _ = _rie_objs(nx, ny)
+    qd = dev(q, dtype)
+    plan = op.plan(dtype)
+    out = {}
+    for form in (1, 2):
+        plan.set_sensor_form(form)
+        out[form] = (host(plan.sensor(qd, enforce_first=True)), host(op.rhs(qd)))
+    plan.set_sensor_form(0)
+    np.testing.assert_array_equal(out[1][0], out[2][0])
+    assert out[1][0].any() and not out[1][0].all()
+    np.testing.assert_array_equal(out[1][1], out[2][1])
+
+
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 2e-6)])
 @pytest.mark.parametrize("case", ["periodic", "nonperiodic"])
 def test_line_forms_agree(case, dtype, tol):
