@@ -856,10 +856,10 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
 // axis 1: y-lines of qT [4][nx][ny], flux_y, alpha_y, dy.
 // OOP/spatial_operator.py:179-249, :321-342; non-periodic
 // OOP/problems/riemann_config3.py:165-265, :344-357.
-// LDS holds two 4 x (n+1) planes, the state and the point flux; per
-// component the WENO7 values are staged in the compact solve's scratch and
-// the hybrid fluxes overwrite that component's state plane, so four line
-// blocks fit a CU.  The WENO work list is built in interface order (wave
+// LDS holds two 4 x (n+1) planes, the state and the point flux; the WENO7
+// values of one or two components at a time are staged in the compact
+// solve's scratch and the hybrid fluxes overwrite those components' state
+// planes, so four line blocks fit a CU.  The WENO work list is built in interface order (wave
 // ballots), so neighbouring lanes read neighbouring LDS words.
 constexpr int kLineBlock = 512;
 template <class T, bool PER, bool SMALL>
@@ -936,51 +936,60 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
     nw += tot;
     __syncthreads();
   }
-  // 2b/3. one component at a time: WENO7 of the Lax-Friedrichs split at the
-  //       listed interfaces into W (the solve's scratch, free until step 4),
-  //       then the hybrid interface flux -- compactified WENO where flagged,
-  //       smooth compact flux elsewhere -- over that component's state plane
-  T* W = scratch;  // n + 1 <= 8 nch + 20 values
+  // 2b/3. CPP components per pass: WENO7 of the Lax-Friedrichs split at the
+  //       listed interfaces into W (the solve's scratch, free until step 4;
+  //       it holds two planes in the one-chunk shape, L <= 4), then the hybrid
+  //       interface flux -- compactified WENO where flagged, smooth compact
+  //       flux elsewhere -- over those components' state planes.  Each WENO
+  //       item runs on a lane pair: the even lane the f+ stencil, the odd
+  //       lane f-, summed (f+ first) through a lane exchange.
+  constexpr int CPP = SMALL ? 2 : 1;
+  T* W = scratch;  // CPP * (n + 1) <= 8 nch + 20 values
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
   // unrolled for float, rolled for double: each keeps the kernel near
   // 64 VGPRs (eight wavefronts per SIMD)
-  constexpr int CU = sizeof(T) == 4 ? 4 : 1;
+  constexpr int CU = sizeof(T) == 4 ? 4 / CPP : 1;
 #pragma unroll CU
-  for (int c = 0; c < 4; ++c) {
-    const T* qa = A + c * S;
-    const T* fa = B + c * S;
+  for (int c0 = 0; c0 < 4; c0 += CPP) {
+    const int nt = 2 * CPP * nw;  // even: both lanes of a pair are live together
 #pragma unroll 1
-    for (int t = threadIdx.x; t < nw; t += BL) {
-      const int k = wl[t];
+    for (int t = threadIdx.x; t < nt; t += BL) {
+      const int cc = t / (2 * nw), r = t - cc * (2 * nw);
+      const int k = wl[r >> 1], side = r & 1;
       const int il = PER ? k : k - 1;
-      T fp[7], fm[7];
+      const T* qa = A + (c0 + cc) * S;
+      const T* fa = B + (c0 + cc) * S;
+      T vs[7];
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
-        const int ip = nidx(il - 3 + j);   // f+ stencil i-3 .. i+3
-        const int im = nidx(il + 4 - j);   // f- stencil i+4 .. i-2
-        fp[j] = T(0.5) * (fa[ip] + alpha * qa[ip]);
-        fm[j] = T(0.5) * (fa[im] - alpha * qa[im]);
+        // f+ stencil i-3 .. i+3: 0.5 (f + alpha q); f- stencil i+4 .. i-2: 0.5 (f - alpha q)
+        const int ix = side ? nidx(il + 4 - j) : nidx(il - 3 + j);
+        const T aq = alpha * qa[ix];
+        vs[j] = T(0.5) * (fa[ix] + (side ? -aq : aq));
       }
-      W[k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
-             weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
+      const T wv = weno7(vs[0], vs[1], vs[2], vs[3], vs[4], vs[5], vs[6]);
+      const T wo = __shfl_xor(wv, 1, 64);
+      if (!side) W[cc * S + k] = wv + wo;
     }
     __syncthreads();
-    T* Fa = A + c * S;  // the state plane of c is no longer read
 #pragma unroll 1
-    for (int k = threadIdx.x; k < nI; k += BL) {
+    for (int t = threadIdx.x; t < CPP * nI; t += BL) {
+      const int cc = t / nI, k = t - cc * nI;
+      const T* Wc = W + cc * S;
+      const T* fa = B + (c0 + cc) * S;
       const bool fl = P.weno_only || iflag(k);
       const int il = PER ? k : k - 1;
       T F;
       if (fl) {
         const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
         const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
-        F = (K.alpha_c * W[kr] + W[k]) + K.alpha_c * W[kl];
+        F = (K.alpha_c * Wc[kr] + Wc[k]) + K.alpha_c * Wc[kl];
       } else {
         F = (c3 * (fa[nidx(il + 3)] + fa[nidx(il - 2)]) + c2 * (fa[nidx(il + 2)] + fa[nidx(il - 1)])) +
             c1 * (fa[nidx(il + 1)] + fa[nidx(il)]);
       }
-      Fa[k] = F;
+      A[(c0 + cc) * S + k] = F;  // the state planes of this pass are no longer read
     }
     __syncthreads();
   }
