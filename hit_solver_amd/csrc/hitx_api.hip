@@ -321,7 +321,8 @@ dim3 grid16(const hitx_plan* pl) { return dim3((pl->d.nx + 15) / 16, (pl->d.ny +
 // stage reductions (one kernel, last block finalizes) + fused sensor tile:
 // mask and maskT of q, and the enforced transposed state when qT != nullptr
 // The fused tile recomputes primitives and jump indicators over its halo
-// (2.1x); the split form computes them once per node, then dilates.
+// (2.1x); the split form forms the undilated mask on tiles with a one-node
+// halo (1.2x), then dilates.
 constexpr long long kSplitSensorMinNodes = 0;
 bool use_split_sensor(const hitx_plan* pl) {
   if (pl->sensor_form != 0) return pl->sensor_form == 2;
@@ -341,7 +342,7 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
   }
   dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
   if (use_split_sensor(pl)) {
-    k_rawmask<T><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
+    k_rawtile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
     k_dilate<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.raw, mask_out, w.maskT, qT);
   } else {
     k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);

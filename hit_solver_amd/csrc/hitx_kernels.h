@@ -585,6 +585,72 @@ __device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __rest
   }
 }
 
+// 1'. Tile form of step 1 (single plans): a 32x16 tile with a one-node halo,
+//     every node's primitives loaded and formed once into LDS in one round,
+//     then the undilated mask of the tile's nodes into raw (the same per-node
+//     arithmetic as the column walk and the fused tile).
+template <class T>
+__device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
+                                             const double* __restrict__ scal, uint8_t* __restrict__ raw, int bx,
+                                             int by) {
+  constexpr int EX = kSTX + 2, EY = kSTY + 2;
+  __shared__ T Lr[EX * EY], Lu[EX * EY], Lv[EX * EY], Lp[EX * EY], Le[EX * EY];
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int x0 = bx * kSTX, y0 = by * kSTY;
+  const bool per = P.periodic;
+  for (int t = threadIdx.x; t < EX * EY; t += kBlock) {
+    const int r = t / EX, c = t - r * EX;
+    int gy = y0 - 1 + r, gx = x0 - 1 + c;
+    if (per) {
+      gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+      gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+    } else {
+      gy = clampi(gy, 0, ny - 1);
+      gx = clampi(gx, 0, nx - 1);
+    }
+    const Prim<T> pr = prim_at(P, q, N, gy * nx + gx, enforce_first);
+    Lr[t] = pr.rho;
+    Lu[t] = pr.u;
+    Lv[t] = pr.v;
+    Lp[t] = pr.p;
+    Le[t] = dv(pr.p, pr.rho * P.gm1);
+  }
+  __syncthreads();
+  const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
+  const bool comp_on = scal[S_COMP_ON] != 0.0, shear_on = scal[S_SHEAR_ON] != 0.0;
+  const double div_thr = scal[S_DIV_THR], shear_thr = scal[S_SHEAR_THR], thr = P.jump_thr;
+  for (int t = threadIdx.x; t < kSTX * kSTY; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gx >= nx || gy >= ny) continue;
+    bool m = false;
+    const int ic = (r + 1) * EX + (c + 1);
+    const int ixl = ic - 1, ixr = ic + 1, iyl = ic - EX, iyr = ic + EX;
+    const bool xin = per || (gx > 0 && gx < nx - 1), yin = per || (gy > 0 && gy < ny - 1);
+    const T ux = dv(Lu[ixr] - Lu[ixl], xin ? P.two_dx : P.dx);
+    const T vx = dv(Lv[ixr] - Lv[ixl], xin ? P.two_dx : P.dx);
+    const T uy = dv(Lu[iyr] - Lu[iyl], yin ? P.two_dy : P.dy);
+    const T vy = dv(Lv[iyr] - Lv[iyl], yin ? P.two_dy : P.dy);
+    const T div = ux + vy;
+    if (comp_on && (double)div < div_thr) m = true;
+    if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
+    const bool xlo = !per && gx == 0, xhi = !per && gx == nx - 1;
+    const bool ylo = !per && gy == 0, yhi = !per && gy == ny - 1;
+    const T jr = npmax(rel_jump(Lr[ixl], Lr[ic], Lr[ixr], er, per, xlo, xhi),
+                       rel_jump(Lr[iyl], Lr[ic], Lr[iyr], er, per, ylo, yhi));
+    const T jp = npmax(rel_jump(Lp[ixl], Lp[ic], Lp[ixr], ep, per, xlo, xhi),
+                       rel_jump(Lp[iyl], Lp[ic], Lp[iyr], ep, per, ylo, yhi));
+    const T je = npmax(rel_jump(Le[ixl], Le[ic], Le[ixr], ee, per, xlo, xhi),
+                       rel_jump(Le[iyl], Le[ic], Le[iyr], ee, per, ylo, yhi));
+    if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+    if (!per && P.bguard > 0) {
+      const int g = P.bguard;
+      if (gy < g || gy >= ny - g || gx < g || gx >= nx - g) m = true;
+    }
+    raw[gy * nx + gx] = m ? 1 : 0;
+  }
+}
+
 // 2. 32x16 tile: the (2w+1)^2 OR dilation of raw (out-of-domain nodes of the
 //    non-periodic grid false), mask and maskT stores, and (qT not null) the
 //    enforced state written transposed for the y sweep.
@@ -659,6 +725,13 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_rawmask(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                     const double* __restrict__ scal, uint8_t* __restrict__ raw) {
   rawmask_body<T>(P, q, enforce_first, scal, raw, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_rawtile(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                    const double* __restrict__ scal, uint8_t* __restrict__ raw) {
+  const Tile2 t_ = tile_xcd();
+  rawtile_body<T>(P, q, enforce_first, scal, raw, t_.bx, t_.by);
 }
 
 template <class T>
@@ -916,15 +989,27 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
   };
   auto nidx = [&](int j) -> int { return PER ? wrap(j, n) : clampi(j, 0, n - 1); };
   // 2a. the interfaces whose WENO7 flux a flagged interface's compactification
-  //     reads (i-1, i, i+1 of every flagged i), in increasing order
+  //     reads (i-1, i, i+1 of every flagged i), in increasing order; each
+  //     thread takes two consecutive interfaces (one round up to 1024)
   int nw = 0;
 #pragma unroll 1
-  for (int k0 = 0; k0 < nI; k0 += BL) {
-    const int k = k0 + threadIdx.x;
-    const bool fl = k < nI && (P.weno_only || iflag(k - 1) || iflag(k) || iflag(k + 1));
-    const uint64_t bal = __ballot(fl);
-    const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-    if (lane == 0) wcnt[wv_id] = __popcll(bal);
+  for (int k0 = 0; k0 < nI; k0 += 2 * BL) {
+    const int ka = k0 + 2 * threadIdx.x, kb = ka + 1;
+    bool fa = false, fb = false;
+    if (ka < nI) {
+      if (P.weno_only) {
+        fa = true;
+        fb = kb < nI;
+      } else {
+        const bool g0 = iflag(ka - 1), g1 = iflag(ka), g2 = iflag(kb);
+        fa = g0 || g1 || g2;
+        fb = kb < nI && (g1 || g2 || iflag(kb + 1));
+      }
+    }
+    const uint64_t ba = __ballot(fa), bb = __ballot(fb);
+    const int pre = (int)(__builtin_amdgcn_mbcnt_hi((unsigned)(ba >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ba, 0u)) +
+                          __builtin_amdgcn_mbcnt_hi((unsigned)(bb >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bb, 0u)));
+    if (lane == 0) wcnt[wv_id] = __popcll(ba) + __popcll(bb);
     __syncthreads();
     int off = nw, tot = 0;
 #pragma unroll
@@ -932,7 +1017,8 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
       off += j < wv_id ? wcnt[j] : 0;
       tot += wcnt[j];
     }
-    if (fl) wl[off + pre] = k;
+    if (fa) wl[off + pre] = ka;
+    if (fb) wl[off + pre + (fa ? 1 : 0)] = kb;
     nw += tot;
     __syncthreads();
   }

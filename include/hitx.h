@@ -276,8 +276,8 @@ int hitx_set_line_form(hitx_plan* plan, int32_t form);
 
 /* Launch-shape support (no reference counterpart): which form of the shock
  * sensor the plan launches.  1: one fused tile kernel (primitives and jump
- * indicators recomputed over the dilation halo); 2: a column walk writing
- * the undilated mask once per node, then a dilation tile; 0 (default): the
+ * indicators recomputed over the dilation halo); 2: a tile kernel writing
+ * the undilated mask (one-node halo), then a dilation tile; 0 (default): the
  * split form.  Both give the same mask. */
 int hitx_set_sensor_form(hitx_plan* plan, int32_t form);
 
