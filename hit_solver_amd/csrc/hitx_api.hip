@@ -337,8 +337,9 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
     k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_sr, w.scal);
     k_drows<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.scal, w.drows);
   } else {
-    k_stage_reduce<T, false><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_sr, w.scal);
+    const dim3 gt((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
+    k_stage_tile<T><<<gt, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, (int)(gt.x * gt.y), w.scal);
   }
   dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
   if (use_split_sensor(pl)) {
@@ -595,8 +596,9 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->off_qT = o; o = align256(o + 4 * N * E);
   pl->off_dfx = o; o = align256(o + 4 * N * E);
   pl->off_dfyT = o; o = align256(o + 4 * N * E);
+  const size_t ntiles = (size_t)((desc->nx + kSTX - 1) / kSTX) * ((desc->ny + kSTY - 1) / kSTY);
   const size_t part = std::max({(size_t)pl->nblk_red * 64, (size_t)pl->nblk_sr * 64,
-                                 (size_t)pl->nblk_proj * 2 * std::max(1, desc->n_force_modes)});
+                                (size_t)pl->nblk_proj * 2 * std::max(1, desc->n_force_modes), ntiles * R_DRAG});
   pl->off_part = o; o = align256(o + part * sizeof(double));
   pl->off_scal = o; o = align256(o + S_COUNT * sizeof(double));
   pl->off_misc = o; o = align256(o + 64 * sizeof(double));

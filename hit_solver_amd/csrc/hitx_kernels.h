@@ -651,6 +651,72 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
   }
 }
 
+// Tile form of the stage reduction without the drag projection (single
+// plans): a 32x16 tile with a one-node halo loaded in one round, the same
+// per-node quantities as the column walk (maxima of |rho|, |p|, |e|,
+// |u|+c, |v|+c; sums of div^2, vort^2 and, for cooling, the pressure of the
+// state as given), one partial row per tile.
+template <class T>
+__device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
+                                                double* __restrict__ part, int bx, int by, int bid) {
+  constexpr int EX = kSTX + 2, EY = kSTY + 2;
+  __shared__ T Lu[EX * EY], Lv[EX * EY];
+  double v[R_DRAG];
+#pragma unroll
+  for (int k = 0; k < R_DRAG; ++k) v[k] = 0.0;
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int x0 = bx * kSTX, y0 = by * kSTY;
+  const bool per = P.periodic;
+  for (int t = threadIdx.x; t < EX * EY; t += kBlock) {
+    const int r = t / EX, c = t - r * EX;
+    const int uy = y0 - 1 + r, ux = x0 - 1 + c;  // unwrapped
+    int gy = uy, gx = ux;
+    if (per) {
+      gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+      gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+    } else {
+      gy = clampi(gy, 0, ny - 1);
+      gx = clampi(gx, 0, nx - 1);
+    }
+    T a, b, cc, d;
+    load4(q, N, gy * nx + gx, a, b, cc, d);
+    const bool own = r >= 1 && r <= kSTY && c >= 1 && c <= kSTX && uy < ny && ux < nx;
+    if (own && P.cool_on) v[R_PSUM] += (double)prim(P, a, b, cc, d).p;
+    if (enforce_first) enforce(P, a, b, cc, d);
+    const Prim<T> pr = prim(P, a, b, cc, d);
+    Lu[t] = pr.u;
+    Lv[t] = pr.v;
+    if (own) {
+      const T cs = sqrt(dv(P.gamma * pr.p, pr.rho));
+      v[R_MAXR] = fmax(v[R_MAXR], (double)absT(pr.rho));
+      v[R_MAXP] = fmax(v[R_MAXP], (double)absT(pr.p));
+      const T e = dv(pr.p, pr.rho * P.gm1);
+      v[R_MAXE] = fmax(v[R_MAXE], (double)absT(e));
+      v[R_AX] = fmax(v[R_AX], (double)(absT(pr.u) + cs));
+      v[R_AY] = fmax(v[R_AY], (double)(absT(pr.v) + cs));
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSTX * kSTY; t += kBlock) {
+    const int r = t / kSTX, c = t - r * kSTX;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gx >= nx || gy >= ny) continue;
+    const int ic = (r + 1) * EX + (c + 1);
+    const bool xin = per || (gx > 0 && gx < nx - 1), yin = per || (gy > 0 && gy < ny - 1);
+    const T ux = dv(Lu[ic + 1] - Lu[ic - 1], xin ? P.two_dx : P.dx);
+    const T vx = dv(Lv[ic + 1] - Lv[ic - 1], xin ? P.two_dx : P.dx);
+    const T uy = dv(Lu[ic + EX] - Lu[ic - EX], yin ? P.two_dy : P.dy);
+    const T vy = dv(Lv[ic + EX] - Lv[ic - EX], yin ? P.two_dy : P.dy);
+    const T div = ux + vy;
+    v[R_DIV2] += (double)div * (double)div;
+    if (P.shear_on) {
+      const T w = vx - uy;
+      v[R_VORT2] += (double)w * (double)w;
+    }
+  }
+  block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * R_DRAG);
+}
+
 // 2. 32x16 tile: the (2w+1)^2 OR dilation of raw (out-of-domain nodes of the
 //    non-periodic grid false), mask and maskT stores, and (qT not null) the
 //    enforced state written transposed for the y sweep.
@@ -725,6 +791,13 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) k_rawmask(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                     const double* __restrict__ scal, uint8_t* __restrict__ raw) {
   rawmask_body<T>(P, q, enforce_first, scal, raw, blockIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                       double* __restrict__ part) {
+  const Tile2 t_ = tile_xcd();
+  stage_tile_body<T>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
 }
 
 template <class T>
