@@ -153,7 +153,7 @@ def hit_measurements(steps):
     res["hit_batched_fp32"] = {
         "value": round(v, 3), "unit": "Mcell-updates/s", "ms_per_step": round(T / n * 1e3, 4),
         "instances": len(cs), "cells": cells, "tiles": {"rb": best[0], "rbp": best[1], "line_form": ("block", "wave")[best[2] - 1],
-                                                     "sensor_form": ("fused", "split")[best[3] - 1]},
+                                                     "sensor_form": ("fused", "split", "split-tile")[best[3] - 1]},
         "vs_hit512_fp64_plain": round(v / plain, 3),
         "step_roofline": {"alg_bytes_per_cell_update": HIT_STEP_BYTES["fp32"], "achieved_GBs": round(ach, 1),
                           "frac": round(ach / HBM_PEAK_GBS, 4)},

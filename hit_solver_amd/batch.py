@@ -284,7 +284,8 @@ class BatchRunner:
         """Launch shapes of every instance: rows per wave of the stage
         reductions and of the forcing projection, the line-sweep form (0 auto,
         1 block per line, 2 wavefront per line) and the sensor form (0 auto,
-        1 fused tile, 2 split)."""
+        1 fused tile, 2 split with the column-walk raw mask, 3 split with the
+        tile raw mask)."""
         torch.cuda.synchronize()
         check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp), int(line_form), int(sensor_form)))
         self._graph = None  # grid sizes changed
@@ -294,7 +295,7 @@ class BatchRunner:
         on a copy of the batch's device state, restore the state and keep the
         fastest candidate.  Returns (best, {candidate: ms per step})."""
         if candidates is None:
-            candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2) for rb in (4, 16, 32)
+            candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2, 3) for rb in (4, 16, 32)
                           for rbp in ((16, 64) if self.hit else (16,))]
         torch.cuda.synchronize()
         backup = self.ws.clone()

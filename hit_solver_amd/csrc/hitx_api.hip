@@ -1037,8 +1037,8 @@ struct hitx_batch {
   bool line_small[2] = {true, true};  // every block-form line of the axis in one-chunk shape
   int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave
   bool wave_x = false, wave_y = false;
-  int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split
-  bool split_sensor = false;
+  int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split (column walk), 3 split (tiles)
+  bool split_sensor = false, raw_tile = false;
   void* ws = nullptr;
 };
 
@@ -1076,7 +1076,8 @@ void batch_offsets(hitx_batch* b) {
   }
   long long cells = 0;
   for (const hitx_plan* pl : b->plans) cells += pl->N;
-  b->split_sensor = b->sensor_form == 2 || (b->sensor_form == 0 && cells >= (1ll << 20));
+  b->split_sensor = b->sensor_form >= 2 || (b->sensor_form == 0 && cells >= (1ll << 20));
+  b->raw_tile = b->sensor_form == 3;
   b->wave_x = b->line_form == 2 || (b->line_form == 0 && lx >= kWaveFormMinLines);
   b->wave_y = b->line_form == 2 || (b->line_form == 0 && ly >= kWaveFormMinLines);
   if (b->line_form == 1) b->wave_x = b->wave_y = false;
@@ -1113,6 +1114,7 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   I.nblk_stage = pl->nblk_stage;
   I.nblk_sr = pl->nblk_sr;
   I.nblk_proj = pl->nblk_proj;
+  I.ntile = ((pl->d.nx + kSTX - 1) / kSTX) * ((pl->d.ny + kSTY - 1) / kSTY);
 }
 
 template <class T>
@@ -1155,11 +1157,17 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const Inst<T>* tab = btab<T>(b);
   const int* offs = boffs(b);
   const int n = b->n;
-  b_stage_reduce<T, DRAG><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
+  if (DRAG)
+    b_stage_reduce<T, DRAG><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
+  else
+    b_stage_tile<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
   b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st);
   if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
   if (b->split_sensor) {
-    b_rawmask<T><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
+    if (b->raw_tile)
+      b_rawtile<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+    else
+      b_rawmask<T><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
     b_dilate<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
   } else {
     b_sensor<T><<<b->total[OFF_SENS], kBlock, b->sensor_lds, s>>>(tab, offs, n, st);
@@ -1305,7 +1313,9 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     // partials sized for the smallest tile heights hitx_batch_tune accepts
     const int sr_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbMin, 1);
     const int pj_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbpMin, 0);
-    u.part = take((size_t)std::max({pl->nblk_stage, pl->nblk_sr, sr_max}) * 64 * sizeof(double));
+    const size_t ntile = (size_t)((pl->d.nx + kSTX - 1) / kSTX) * ((pl->d.ny + kSTY - 1) / kSTY);
+    u.part = take(std::max((size_t)std::max({pl->nblk_stage, pl->nblk_sr, sr_max}) * 64, ntile * R_DRAG) *
+                  sizeof(double));
     u.scal = take(S_COUNT * sizeof(double));
     u.ctl = take(C_COUNT * sizeof(double));
     u.kpar = take(K_COUNT * sizeof(double));
@@ -1369,7 +1379,8 @@ int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, i
   if ((rb != 0 && (rb < kTuneRbMin || rb > kTuneRbMax)) || (rbp != 0 && (rbp < kTuneRbpMin || rbp > kTuneRbpMax)))
     return fail(HITX_EINVAL, "tile heights out of range (rb 0 or 2..64, rbp 0 or 8..256)");
   if (line_form < 0 || line_form > 2) return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block) or 2 (wave)");
-  if (sensor_form < 0 || sensor_form > 2) return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused) or 2 (split)");
+  if (sensor_form < 0 || sensor_form > 3)
+    return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused), 2 (split, column walk) or 3 (split, tiles)");
   b->line_form = line_form;
   b->sensor_form = sensor_form;
   for (hitx_plan* pl : b->plans) {

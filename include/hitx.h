@@ -327,7 +327,8 @@ int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
  * (0 for either: each instance's own default, set by its grid alone),
  * the line-sweep form (0 auto, 1 block, 2 one wavefront per line where the
  * line allows it) and the sensor form (0 auto, 1 fused tile, 2 split: the
- * undilated mask by column walk, then the dilation).  Changes the summation
+ * undilated mask by column walk, then the dilation; 3 split with the
+ * undilated mask on tiles with a one-node halo).  Changes the summation
  * order of the reductions and the rounding of the line solve only.  Blocking
  * when bound (re-uploads the instance table); no work may be in flight. */
 int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_form, int32_t sensor_form);

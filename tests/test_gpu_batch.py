@@ -47,21 +47,22 @@ def test_riemann_batch_matches_single_instances():
 
 
 def test_riemann_batch_split_sensor_bitexact():
-    """The split sensor (undilated mask by column walk, then the dilation)
-    gives the fused tile's mask: with the single plan's reduction tiles the
-    batch still reproduces single instances bit for bit."""
+    """The split sensor (undilated mask by column walk or on tiles, then the
+    dilation) gives the same mask as the single plan: the batch reproduces
+    single instances bit for bit (both reduce the stage on the same tiles)."""
     from hit_solver_amd.batch import BatchRunner
     from hit_solver_amd.riemann import RiemannConfig3, RiemannConfig6, RiemannSolver
 
     cfgs = [RiemannConfig3(nx=64, ny=48, shear_threshold=None, progress_every=10**9),
             RiemannConfig6(nx=40, ny=72, shear_threshold=None, progress_every=10**9)]
-    b = BatchRunner(cfgs, max_steps=5)
-    b.tune(2, 8, 0, 2)
-    b.run()
-    for i, c in enumerate(cfgs):
-        s = RiemannSolver(c)
-        s.run(max_steps=5)
-        np.testing.assert_array_equal(host(b.q[i]), host(s.q))
+    for form in (2, 3):
+        b = BatchRunner(cfgs, max_steps=5)
+        b.tune(2, 8, 0, form)
+        b.run()
+        for i, c in enumerate(cfgs):
+            s = RiemannSolver(c)
+            s.run(max_steps=5)
+            np.testing.assert_array_equal(host(b.q[i]), host(s.q))
 
 
 def test_hit_batch_matches_single_instances():
@@ -147,11 +148,15 @@ def test_hit_batch_tuned_tiles_agree():
     a, b = BatchRunner(cfgs), BatchRunner(cfgs)
     a.tune(2, 8, 1, 1)
     b.tune(8, 32, 2, 2)
+    c = BatchRunner(cfgs)
+    c.tune(4, 16, 2, 3)
     a.run()
     b.run()
+    c.run()
     for i in range(2):
         for k in range(4):
             assert rel_l2(host(b.q[i])[k], host(a.q[i])[k]) <= 1e-12, (i, k)
+            assert rel_l2(host(c.q[i])[k], host(a.q[i])[k]) <= 1e-12, (i, k)
 
 
 def test_batch_graph_replay_matches_eager():
