@@ -68,6 +68,31 @@ __device__ __forceinline__ u128 pcg_advance(u128 s, u128 inc, unsigned long long
   return am * s + ap;
 }
 
+// Jumps of the LCG by 32*t steps, t < kRngAttemptThreads: s -> a s + inc m
+// with a = M^(32t), m = sum_{i < 32t} M^i (mod 2^128).  Jumps compose as
+// (a1, m1) then (a2, m2) = (a1 a2, a2 m1 + m2).
+constexpr u128 kPcgM = ((u128)0x2360ED051FC65DA4ull << 64) | (u128)0x4385DF649FCCF645ull;
+struct PcgJumps {
+  u128 a[256], m[256];
+};
+constexpr PcgJumps make_pcg_jumps() {
+  u128 a32 = 1, m32 = 0;
+  for (int i = 0; i < 32; ++i) {
+    m32 = m32 + a32;
+    a32 = a32 * kPcgM;
+  }
+  PcgJumps t{};
+  u128 a = 1, m = 0;
+  for (int i = 0; i < 256; ++i) {
+    t.a[i] = a;
+    t.m[i] = m;
+    m = a32 * m + m32;
+    a = a * a32;
+  }
+  return t;
+}
+__device__ constexpr PcgJumps kPcgJumps = make_pcg_jumps();
+
 __device__ __forceinline__ Pcg pcg_load(const uint64_t* st) {
   Pcg g;
   g.s = ((u128)st[0] << 64) | (u128)st[1];
@@ -144,12 +169,22 @@ struct RngJob {
 __host__ __device__ inline int rng_words(long long n) { return (int)(n + n / 16 + 64); }
 __host__ __device__ inline int rng_chunks(long long n) { return (rng_words(n) + kRngChunk - 1) / kRngChunk; }
 
-// Steps 1 and 2 for chunk c (one thread).  lds: kRngChunk bytes per thread.
-__device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c, uint8_t* lds) {
-  if (c >= J.nch) return;
-  const int p0 = c * kRngChunk;
+// Steps 1 and 2 for chunk c = c0 + threadIdx.x of a block whose first chunk
+// is c0 (every thread of the block calls it).  One thread advances the state
+// to the block's first word; each thread then jumps by 32 threadIdx.x words
+// with the table.  lds: kRngChunk bytes per thread.
+__device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_t* lds) {
+  __shared__ uint64_t sb[2];
+  const int c = c0 + threadIdx.x;
   Pcg g = pcg_load(J.state);
-  g.s = pcg_advance(g.s, g.inc, (unsigned long long)p0);
+  if (threadIdx.x == 0) {
+    const u128 s0 = pcg_advance(g.s, g.inc, (unsigned long long)c0 * kRngChunk);
+    sb[0] = (uint64_t)(s0 >> 64);
+    sb[1] = (uint64_t)s0;
+  }
+  __syncthreads();
+  if (c >= J.nch) return;
+  g.s = kPcgJumps.a[threadIdx.x] * (((u128)sb[0] << 64) | (u128)sb[1]) + g.inc * kPcgJumps.m[threadIdx.x];
   for (int j = 0; j < kRngChunk; ++j) {
     const uint64_t w = g.next();
     const ZigAttempt a = zig_attempt<true>(w, g);
@@ -310,9 +345,10 @@ __device__ __forceinline__ void rng_finish_body(const RngJob& J) {
 
 constexpr int kRngAttemptThreads = 256;
 
+static_assert(kRngAttemptThreads <= 256, "jump table covers 256 threads");
 __global__ void __launch_bounds__(kRngAttemptThreads) k_rng_attempts(RngJob J) {
   __shared__ uint8_t lds[kRngAttemptThreads * kRngChunk];
-  rng_attempt_body(J, blockIdx.x * kRngAttemptThreads + threadIdx.x, lds + threadIdx.x * kRngChunk);
+  rng_attempt_body(J, blockIdx.x * kRngAttemptThreads, lds + threadIdx.x * kRngChunk);
 }
 __global__ void __launch_bounds__(kRngScan) k_rng_scan(RngJob J, int force_serial) { rng_scan_body(J, force_serial); }
 __global__ void __launch_bounds__(kRngScatterThreads) k_rng_scatter(RngJob J) {
