@@ -45,7 +45,12 @@ HIT_STEP_BYTES = {"fp64": 635.0, "fp32": 320.0}
 # HBM bytes per line-sweep launch (both sweeps of a stage) measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (scaled by a 512 MiB copy kernel's readings; profiles/k_line_riemann_c3_512_fp64.md), for the workloads that
 # were profiled
-LINE_TRAFFIC: dict = {}
+LINE_TRAFFIC = {(512, "fp64"): 34_535_872}
+
+
+def progress(msg):
+    """One line per finished phase on stderr (the result line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -307,18 +312,23 @@ def main():
                           "achieved_GBs": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9, 1),
                           "frac": round(value * 1e6 * STEP_BYTES[args.dtype] / 1e9 / HBM_PEAK_GBS, 4)},
     }
+    progress(f"headline {out['value']} Mcell-updates/s")
     if not args.no_hit:
         del s
         torch.cuda.empty_cache()
         out["hit"] = hit_measurements(args.hit_steps)
+        progress("forced-HIT measurements done")
     if not args.no_fp32_error:
         torch.cuda.empty_cache()
         out["fp32_error"] = fp32_error(args.fp32_steps, args.save_states)
+        progress("FP32 error run done")
     if not args.no_serving:
         torch.cuda.empty_cache()
         out["serving"] = serving_measurement(args.serving_requests, args.serving_rate, args.serving_steps)
+        progress("serving stream done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(nx, args.cpu_seconds)
+        progress("CPU baseline done")
     print(json.dumps(out), flush=True)
 
 
