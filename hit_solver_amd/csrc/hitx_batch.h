@@ -32,13 +32,13 @@ template <class T>
 struct Inst {
   Params<T> P;
   LineK<T> Kx, Ky;
-  T *Q, *Q1, *Q2, *S;  // state, stage 1 and 2 results, stage 3 result (before hyperviscosity)
-  T *qT, *dfx, *dfyT, *fx, *fy;
-  uint8_t *mask, *maskT, *rawm;  // dilated mask, its transpose, undilated mask (split sensor)
-  double *part, *scal, *ctl, *kpar, *phi, *fhat, *noise, *fpart, *kxy;
-  uint64_t* rng;  // PCG64 state: state hi, state lo, increment hi, increment lo
-  RngJob R;       // forcing noise request: nx*ny normals of the rng stream into noise
-  double *drows, *frows;  // per-row synthesis tables: drag low-pass, forcing
+  GPtr<T> Q, Q1, Q2, S;  // state, stage 1 and 2 results, stage 3 result (before hyperviscosity)
+  GPtr<T> qT, dfx, dfyT, fx, fy;
+  GPtr<uint8_t> mask, maskT, rawm;  // dilated mask, its transpose, undilated mask (split sensor)
+  GPtr<double> part, scal, ctl, kpar, phi, fhat, noise, fpart, kxy;
+  GPtr<uint64_t> rng;  // PCG64 state: state hi, state lo, increment hi, increment lo
+  RngJob R;            // forcing noise request: nx*ny normals of the rng stream into noise
+  GPtr<double> drows, frows;  // per-row synthesis tables: drag low-pass, forcing
   int nblk_stage, nblk_sr, nblk_proj;
   int ntile;  // 32x16 tiles of the grid
 };
@@ -61,12 +61,13 @@ __device__ __forceinline__ int find_inst(const int* __restrict__ o, int n, int b
   return lo;
 }
 
+// (selected as global pointers, converted once)
 template <class T>
-__device__ __forceinline__ const T* stage_in(const Inst<T>& I, int st) {
+__device__ __forceinline__ GPtr<T> stage_in(const Inst<T>& I, int st) {
   return st == 0 ? I.Q : st == 1 ? I.Q1 : st == 2 ? I.Q2 : I.S;
 }
 template <class T>
-__device__ __forceinline__ T* stage_out(const Inst<T>& I, int st) {
+__device__ __forceinline__ GPtr<T> stage_out(const Inst<T>& I, int st) {
   return st == 0 ? I.Q1 : st == 1 ? I.Q2 : I.S;
 }
 
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ t
   const int by = lb / ntx, bx = lb - by * ntx;
   // the last stage writes Q in place when no hyperviscosity pass follows and
   // there is no outflow guard (each node then reads Q only at itself)
-  T* out = (st == 2 && I.ctl[C_HVNOW] == 0.0 && I.P.guard == 0) ? I.Q : stage_out(I, st);
+  const GPtr<T> out = (st == 2 && I.ctl[C_HVNOW] == 0.0 && I.P.guard == 0) ? I.Q : stage_out(I, st);
   update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
                  out, bx, by, (T)I.ctl[C_SCALE], (T)I.ctl[C_ALPHA]);
 }
@@ -413,7 +414,7 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
     const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
     I.fx[idx] = fxv;
     I.fy[idx] = fyv;
-    const Prim<T> pr = prim_at(P, I.Q, N, idx, 0);
+    const Prim<T> pr = prim_at<T>(P, I.Q, N, idx, 0);
     acc[0] += (double)(fxv * fxv + fyv * fyv);
     acc[1] += (double)(pr.rho * (fxv * pr.u + fyv * pr.v));
     acc[2] += (double)(fxv * fxv);

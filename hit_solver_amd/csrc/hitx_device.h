@@ -15,6 +15,29 @@ namespace hitx {
 constexpr int kBlock = 256;  // threads per block (4 wave64s)
 constexpr int kWaves = kBlock / 64;
 
+// A pointer to global memory held in a descriptor.  Kernels of the batched
+// path read their buffers' addresses from per-instance descriptors in device
+// memory; a plain pointer loaded that way is generic and compiles to flat
+// memory instructions (which also count against the LDS wait counter).
+// Stored in the global address space, accesses through it compile to global
+// loads and stores.  Converts to and from T* on both host and device.
+template <class T>
+struct GPtr {
+  __attribute__((address_space(1))) T* p;
+  __host__ __device__ operator T*() const { return (T*)p; }
+  __host__ __device__ GPtr& operator=(T* q) {
+    p = (__attribute__((address_space(1))) T*)q;
+    return *this;
+  }
+};
+// A load known to read global memory: where one expression reads either LDS
+// or global memory, a plain load would become one flat load of a selected
+// address.
+template <class T>
+__device__ __forceinline__ T ld_global(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // ---------------------------------------------------------------- parameters
 // Per-plan constants passed by value to every kernel.  T is the element type
 // of the state (double or float); thresholds compared against FP64
@@ -40,10 +63,10 @@ struct Params {
   double cool_pref, cool_tc, gamma_d; // cooling target pressure, time; gamma as double
   int ndrag, nforce;
   int kmx, kmy;                       // twiddle half-ranges (|k index| <= kmx, kmy)
-  const double* twx;                  // [(2*kmx+1)][nx][2]  cos, sin of 2*pi*k*i/nx
-  const double* twy;                  // [(2*kmy+1)][ny][2]
-  const int* drag_modes;              // [ndrag][2]
-  const int* force_modes;             // [nforce][2]
+  GPtr<const double> twx;             // [(2*kmx+1)][nx][2]  cos, sin of 2*pi*k*i/nx
+  GPtr<const double> twy;             // [(2*kmy+1)][ny][2]
+  GPtr<const int> drag_modes;         // [ndrag][2]
+  GPtr<const int> force_modes;        // [nforce][2]
   // separable mode sums (column walks and per-row tables)
   int rb, rbp;                        // rows per wave: stage reductions, forcing projection
   int dkx, dky;                       // max |kx|, |ky| of the drag modes (<= 2)

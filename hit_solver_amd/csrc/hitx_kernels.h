@@ -1342,7 +1342,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   T Lv[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const T gy = own ? sD[c][ty][tx] : dfyT[c * N + (size_t)cx * ny + cy];
+    const T gy = own ? sD[c][ty][tx] : ld_global(dfyT + c * N + (size_t)cx * ny + cy);
     Lv[c] = (-gx[c]) - gy;
   }
   if (mode == MODE_RHS) {
@@ -1492,7 +1492,7 @@ __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict_
         const T lx_ = dv((lap_at(P, a, cy, xr) - T(2.0) * Lc) + lap_at(P, a, cy, xl), P.dx2);
         const T ly_ = dv((lap_at(P, a, yr, cx) - T(2.0) * Lc) + lap_at(P, a, yl, cx), P.dy2);
         bih = lx_ + ly_;
-        ac = a[cy * nx + cx];
+        ac = ld_global(a + cy * nx + cx);
       }
       const T coef = act ? P.hv_coef[c] : T(0);
       o[c] = ac - (coef * bih);

@@ -153,14 +153,14 @@ __device__ __forceinline__ ZigAttempt zig_attempt(uint64_t w, Pcg g) {
 
 // Buffers of one request of n normals (all in device memory).
 struct RngJob {
-  uint64_t* state;  // s hi, s lo, inc hi, inc lo; advanced past the n values
-  double* out;      // n values
-  double* vals;     // [kRngChunk][nch] attempt values (transposed: coalesced)
-  uint8_t* att;     // [kRngChunk][nch] attempt length | yields << 7
-  uint16_t* map;    // [nch][kRngEntries] spill | count << 8 (spill 0xff: overflow)
-  uint8_t* entry;   // [nch] resolved entry offset
-  int* base;        // [nch] index of the chunk's first value
-  int* stat;        // [0] flag: 0 parallel result, 1 overflow / too few words, 2 forced serial;
+  GPtr<uint64_t> state;  // s hi, s lo, inc hi, inc lo; advanced past the n values
+  GPtr<double> out;      // n values
+  GPtr<double> vals;     // [kRngChunk][nch] attempt values (transposed: coalesced)
+  GPtr<uint8_t> att;     // [kRngChunk][nch] attempt length | yields << 7
+  GPtr<uint16_t> map;    // [nch][kRngEntries] spill | count << 8 (spill 0xff: overflow)
+  GPtr<uint8_t> entry;   // [nch] resolved entry offset
+  GPtr<int> base;        // [nch] index of the chunk's first value
+  GPtr<int> stat;        // [0] flag: 0 parallel result, 1 overflow / too few words, 2 forced serial;
                     // [1] words consumed by the n values
   long long n;      // values requested
   int W, nch;       // words evaluated, chunks
