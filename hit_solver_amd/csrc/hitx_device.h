@@ -120,10 +120,15 @@ __device__ __forceinline__ float dv<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
 
-// np.maximum: NaN-propagating maximum.
+// np.maximum: NaN-propagating maximum (a if a is NaN, else b if b is NaN,
+// else the larger).  Written as selects of precomputed values so that it
+// compiles to compares and v_cndmask, not to branches that stop the
+// scheduler from moving loads across it.
 template <class T>
 __device__ __forceinline__ T npmax(T a, T b) {
-  return (a != a) ? a : ((b != b) ? b : (a > b ? a : b));
+  const T m = a > b ? a : b;
+  const T nb = (b != b) ? b : m;
+  return (a != a) ? a : nb;
 }
 
 template <class T>
