@@ -602,7 +602,9 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->off_part = o; o = align256(o + part * sizeof(double));
   pl->off_scal = o; o = align256(o + S_COUNT * sizeof(double));
   pl->off_misc = o; o = align256(o + 64 * sizeof(double));
-  pl->off_drows = o; o = align256(o + (pl->dkx + pl->dky > 0 ? (size_t)desc->ny * 10 : 1) * sizeof(double));
+  // per-row drag table: 10 doubles a row whenever there are drag modes (also
+  // when the only mode is k = 0)
+  pl->off_drows = o; o = align256(o + (desc->n_drag_modes > 0 ? (size_t)desc->ny * 10 : 1) * sizeof(double));
   pl->off_frows = o; o = align256(o + (desc->n_force_modes > 0 ? (size_t)desc->ny * 2 * (2 * pl->fkx + 1) : 1) *
                                           sizeof(double));
   pl->ws_bytes = o;
@@ -1326,7 +1328,7 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     u.kxy = take(2 * nf * sizeof(double));
     u.rng = take(4 * sizeof(uint64_t));
     u.rngws = take(pl->d.n_force_modes > 0 ? rng_layout(N).total : 8);
-    u.drows = take((pl->dkx + pl->dky > 0 ? (size_t)pl->d.ny * 10 : 1) * sizeof(double));
+    u.drows = take((pl->d.n_drag_modes > 0 ? (size_t)pl->d.ny * 10 : 1) * sizeof(double));
     u.frows = take((pl->d.n_force_modes > 0 ? (size_t)pl->d.ny * 2 * (2 * pl->fkx + 1) : 1) * sizeof(double));
     b->buf.push_back(u);
   }

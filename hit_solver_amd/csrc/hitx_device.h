@@ -120,6 +120,13 @@ __device__ __forceinline__ float dv<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
 
+// a*b + c: fused for float (one rounding; the FP32 mode is held to its error
+// budget), two IEEE operations for double (the library is built without
+// contraction, so the FP64 mode keeps the reference's rounding).  Call sites
+// only reorder the operands of additions, which IEEE addition leaves exact.
+__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double madd(double a, double b, double c) { return a * b + c; }
+
 // np.maximum: NaN-propagating maximum (a if a is NaN, else b if b is NaN,
 // else the larger).  Written as selects of precomputed values so that it
 // compiles to compares and v_cndmask, not to branches that stop the
@@ -186,33 +193,35 @@ __device__ __forceinline__ Prim<T> prim_at(const Params<T>& P, const T* q, int N
 template <class T>
 __device__ __forceinline__ T weno7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
   const T eps = T(1e-10);
-  T q0 = (((T(-(1.0 / 4.0)) * v1 + T(13.0 / 12.0) * v2) - T(23.0 / 12.0) * v3) + T(25.0 / 12.0) * v4);
-  T q1 = (((T(1.0 / 12.0) * v2 - T(5.0 / 12.0) * v3) + T(13.0 / 12.0) * v4) + T(1.0 / 4.0) * v5);
-  T q2 = (((T(-(1.0 / 12.0)) * v3 + T(7.0 / 12.0) * v4) + T(7.0 / 12.0) * v5) - T(1.0 / 12.0) * v6);
-  T q3 = (((T(1.0 / 4.0) * v4 + T(13.0 / 12.0) * v5) - T(5.0 / 12.0) * v6) + T(1.0 / 12.0) * v7);
-  T b0 = ((v1 * (((T(544) * v1 - T(3882) * v2) + T(4642) * v3) - T(1854) * v4)
-           + v2 * ((T(7043) * v2 - T(17246) * v3) + T(7042) * v4))
-          + v3 * (T(11003) * v3 - T(9402) * v4))
-         + T(2107) * (v4 * v4);
-  T b1 = ((v2 * (((T(267) * v2 - T(1642) * v3) + T(1602) * v4) - T(494) * v5)
-           + v3 * ((T(2843) * v3 - T(5966) * v4) + T(1922) * v5))
-          + v4 * (T(3443) * v4 - T(2522) * v5))
-         + T(547) * (v5 * v5);
-  T b2 = ((v3 * (((T(547) * v3 - T(2522) * v4) + T(1922) * v5) - T(494) * v6)
-           + v4 * ((T(3443) * v4 - T(5966) * v5) + T(1602) * v6))
-          + v5 * (T(2843) * v5 - T(1642) * v6))
-         + T(267) * (v6 * v6);
-  T b3 = ((v4 * (((T(2107) * v4 - T(9402) * v5) + T(7042) * v6) - T(1854) * v7)
-           + v5 * ((T(11003) * v5 - T(17246) * v6) + T(4642) * v7))
-          + v6 * (T(7043) * v6 - T(3882) * v7))
-         + T(547) * (v7 * v7);
+  // (each madd(a, b, c) is a*b + c; a difference x - k*v is madd(-k, v, x))
+  T q0 = madd(T(25.0 / 12.0), v4, madd(T(-23.0 / 12.0), v3, madd(T(13.0 / 12.0), v2, T(-(1.0 / 4.0)) * v1)));
+  T q1 = madd(T(1.0 / 4.0), v5, madd(T(13.0 / 12.0), v4, madd(T(-5.0 / 12.0), v3, T(1.0 / 12.0) * v2)));
+  T q2 = madd(T(-1.0 / 12.0), v6, madd(T(7.0 / 12.0), v5, madd(T(7.0 / 12.0), v4, T(-(1.0 / 12.0)) * v3)));
+  T q3 = madd(T(1.0 / 12.0), v7, madd(T(-5.0 / 12.0), v6, madd(T(13.0 / 12.0), v5, T(1.0 / 4.0) * v4)));
+  // sum of a row (v_a * (c1 v_a + c2 v_b + ...)) in the reference's order
+  T b0 = madd(T(2107), v4 * v4,
+              madd(v3, madd(T(-9402), v4, T(11003) * v3),
+                   madd(v2, madd(T(7042), v4, madd(T(-17246), v3, T(7043) * v2)),
+                        v1 * madd(T(-1854), v4, madd(T(4642), v3, madd(T(-3882), v2, T(544) * v1))))));
+  T b1 = madd(T(547), v5 * v5,
+              madd(v4, madd(T(-2522), v5, T(3443) * v4),
+                   madd(v3, madd(T(1922), v5, madd(T(-5966), v4, T(2843) * v3)),
+                        v2 * madd(T(-494), v5, madd(T(1602), v4, madd(T(-1642), v3, T(267) * v2))))));
+  T b2 = madd(T(267), v6 * v6,
+              madd(v5, madd(T(-1642), v6, T(2843) * v5),
+                   madd(v4, madd(T(1602), v6, madd(T(-5966), v5, T(3443) * v4)),
+                        v3 * madd(T(-494), v6, madd(T(1922), v5, madd(T(-2522), v4, T(547) * v3))))));
+  T b3 = madd(T(547), v7 * v7,
+              madd(v6, madd(T(-3882), v7, T(7043) * v6),
+                   madd(v5, madd(T(4642), v7, madd(T(-17246), v6, T(11003) * v5)),
+                        v4 * madd(T(-1854), v7, madd(T(7042), v6, madd(T(-9402), v5, T(2107) * v4))))));
   T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2, e3 = eps + b3;
   T a0 = dv(T(1.0 / 35.0), e0 * e0);
   T a1 = dv(T(12.0 / 35.0), e1 * e1);
   T a2 = dv(T(18.0 / 35.0), e2 * e2);
   T a3 = dv(T(4.0 / 35.0), e3 * e3);
   T asum = ((a0 + a1) + a2) + a3;
-  return dv((((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3), asum);
+  return dv(madd(a3, q3, madd(a2, q2, madd(a1, q1, a0 * q0))), asum);
 }
 
 // ---------------------------------------------------------------- reductions

@@ -881,11 +881,11 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
             if (i == 0 || i == n - 1) {
               xv = T(0);
             } else {
-              if (i == 1) xv = xv - K.alpha_c * r0;
-              if (i == n - 2) xv = xv - K.alpha_c * rn1;
+              if (i == 1) xv = madd(-K.alpha_c, r0, xv);
+              if (i == n - 2) xv = madd(-K.alpha_c, rn1, xv);
             }
           }
-          acc = xv + rho * acc;
+          acc = madd(rho, acc, xv);
           y[u][k] = acc;
         }
       }
@@ -906,22 +906,22 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
         int j = ch;
         for (int m = 1; m <= mm; ++m) {
           j = (j == 0) ? nch - 1 : j - 1;
-          carry = carry + pwk(PW, (m - 1) * L) * el[j];
+          carry = madd(pwk(PW, (m - 1) * L), el[j], carry);
         }
         carry = carry * K.inv1mrn;
       } else {
         const int mm = ch < M ? ch : M;
-        for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch - m];
+        for (int m = 1; m <= mm; ++m) carry = madd(pwk(PW, (m - 1) * L), el[ch - m], carry);
       }
 #pragma unroll
       for (int k = 0; k < LM; ++k)
-        if (k < L) y[u][k] = y[u][k] + pwk(PW, k + 1) * carry;
+        if (k < L) y[u][k] = madd(pwk(PW, k + 1), carry, y[u][k]);
       if (!PER && ch == nch - 1) bnd[line * 3 + 0] = (rho * y[u][L - 1]) / K.one_m_rho2;
       T acc = T(0);
 #pragma unroll
       for (int k = LM - 1; k >= 0; --k) {
         if (k < L) {
-          acc = y[u][k] + rho * acc;
+          acc = madd(rho, acc, y[u][k]);
           y[u][k] = acc;
         }
       }
@@ -942,18 +942,18 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
         int j = ch;
         for (int m = 1; m <= mm; ++m) {
           j = (j == nch - 1) ? 0 : j + 1;
-          carry = carry + pwk(PW, (m - 1) * L) * el[j];
+          carry = madd(pwk(PW, (m - 1) * L), el[j], carry);
         }
         carry = carry * K.inv1mrn;
       } else {
         const int rem = nch - 1 - ch;
         const int mm = rem < M ? rem : M;
-        for (int m = 1; m <= mm; ++m) carry = carry + pwk(PW, (m - 1) * L) * el[ch + m];
-        if (rem < M) carry = carry + pwk(PW, rem * L) * bnd[line * 3 + 0];
+        for (int m = 1; m <= mm; ++m) carry = madd(pwk(PW, (m - 1) * L), el[ch + m], carry);
+        if (rem < M) carry = madd(pwk(PW, rem * L), bnd[line * 3 + 0], carry);
       }
 #pragma unroll
       for (int k = 0; k < LM; ++k)
-        if (k < L) y[u][k] = K.c1p * (y[u][k] + pwk(PW, L - k) * carry);
+        if (k < L) y[u][k] = K.c1p * madd(pwk(PW, L - k), carry, y[u][k]);
       if (!PER) {
         if (ch == 0) bnd[line * 3 + 1] = y[u][0];
         if (ch == nch - 1) bnd[line * 3 + 2] = y[u][L - 1];
@@ -1124,8 +1124,7 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
       for (int j = 0; j < 7; ++j) {
         // f+ stencil i-3 .. i+3: 0.5 (f + alpha q); f- stencil i+4 .. i-2: 0.5 (f - alpha q)
         const int ix = side ? nidx(il + 4 - j) : nidx(il - 3 + j);
-        const T aq = alpha * qa[ix];
-        vs[j] = T(0.5) * (fa[ix] + (side ? -aq : aq));
+        vs[j] = T(0.5) * madd(side ? -alpha : alpha, qa[ix], fa[ix]);
       }
       const T wv = weno7(vs[0], vs[1], vs[2], vs[3], vs[4], vs[5], vs[6]);
       const T wo = __shfl_xor(wv, 1, 64);
@@ -1143,10 +1142,10 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
       if (fl) {
         const int kr = PER ? wrap(k + 1, n) : clampi(k + 1, 0, n);
         const int kl = PER ? wrap(k - 1, n) : clampi(k - 1, 0, n);
-        F = (K.alpha_c * Wc[kr] + Wc[k]) + K.alpha_c * Wc[kl];
+        F = madd(K.alpha_c, Wc[kl], madd(K.alpha_c, Wc[kr], Wc[k]));
       } else {
-        F = (c3 * (fa[nidx(il + 3)] + fa[nidx(il - 2)]) + c2 * (fa[nidx(il + 2)] + fa[nidx(il - 1)])) +
-            c1 * (fa[nidx(il + 1)] + fa[nidx(il)]);
+        F = madd(c1, fa[nidx(il + 1)] + fa[nidx(il)],
+                 madd(c3, fa[nidx(il + 3)] + fa[nidx(il - 2)], c2 * (fa[nidx(il + 2)] + fa[nidx(il - 1)])));
       }
       A[(c0 + cc) * S + k] = F;  // the state planes of this pass are no longer read
     }
@@ -1277,6 +1276,10 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   // latency overlaps it
   T gx[4] = {T(0), T(0), T(0), T(0)}, a0[4] = {T(0), T(0), T(0), T(0)};
   T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0), f1r = T(0), f2r = T(0);
+  // drag low-pass tables: the node's row of the per-row table and the
+  // column twiddles of |kx| = 1, 2 (|kx| <= 2 for drag modes)
+  double dH[10], dtw[2][2];
+  const bool drag_node = in && mode == MODE_STAGE && P.drag_on;
   if (in && mode != MODE_VISC) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) gx[c] = dfx[c * N + idx];
@@ -1288,6 +1291,18 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
         f1r = fx[idx];
         f2r = fy[idx];
       }
+    }
+  }
+  if (drag_node) {
+    const double* H = drows + (size_t)cy * 10;
+#pragma unroll
+    for (int j = 0; j < 10; ++j) dH[j] = H[j];
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+      const int kk = k <= P.dkx ? k : 0;  // an in-range row when |kx| = k is absent
+      const double* tw = P.twx + ((size_t)(kk + P.kmx) * nx + cx) * 2;
+      dtw[k - 1][0] = tw[0];
+      dtw[k - 1][1] = tw[1];
     }
   }
   VTileLoad<T> vt;
@@ -1310,13 +1325,13 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       fsrc[3] = st.rho * (st.u * f1 + st.v * f2);
     }
     if (P.drag_on) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
-      const double* H = drows + (size_t)cy * 10;
-      double lx = H[0], ly = H[5];
-      for (int k = 1; k <= P.dkx; ++k) {
-        const double* tw = P.twx + ((size_t)(k + P.kmx) * nx + cx) * 2;
-        const double c = tw[0], sn = tw[1];
-        lx += H[2 * k - 1] * c - H[2 * k] * sn;
-        ly += H[5 + 2 * k - 1] * c - H[5 + 2 * k] * sn;
+      double lx = dH[0], ly = dH[5];
+#pragma unroll
+      for (int k = 1; k <= 2; ++k) {
+        if (k > P.dkx) break;
+        const double c = dtw[k - 1][0], sn = dtw[k - 1][1];
+        lx += dH[2 * k - 1] * c - dH[2 * k] * sn;
+        ly += dH[5 + 2 * k - 1] * c - dH[5 + 2 * k] * sn;
       }
       const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
       dsrc[1] = d1;

@@ -91,29 +91,29 @@ __device__ __forceinline__ double rdlane(double v, int l) {
 // through their end values (readlane).
 template <class T>
 __device__ __forceinline__ T scan_fwd(T v, const T (&pw)[5], T pf, int lane) {
-  v = v + pw[0] * dpp<0x111>(v);
-  v = v + pw[1] * dpp<0x112>(v);
-  v = v + pw[2] * dpp<0x114>(v);
-  v = v + pw[3] * dpp<0x118>(v);
+  v = madd(pw[0], dpp<0x111>(v), v);
+  v = madd(pw[1], dpp<0x112>(v), v);
+  v = madd(pw[2], dpp<0x114>(v), v);
+  v = madd(pw[3], dpp<0x118>(v), v);
   const T s0 = rdlane(v, 15);
-  const T s1 = rdlane(v, 31) + pw[4] * s0;
-  const T s2 = rdlane(v, 47) + pw[4] * s1;
+  const T s1 = madd(pw[4], s0, rdlane(v, 31));
+  const T s2 = madd(pw[4], s1, rdlane(v, 47));
   const int r = lane >> 4;
   const T prev = r == 0 ? T(0) : (r == 1 ? s0 : (r == 2 ? s1 : s2));
-  return v + pf * prev;
+  return madd(pf, prev, v);
 }
 template <class T>
 __device__ __forceinline__ T scan_bwd(T v, const T (&pw)[5], T pb, int lane) {
-  v = v + pw[0] * dpp<0x101>(v);
-  v = v + pw[1] * dpp<0x102>(v);
-  v = v + pw[2] * dpp<0x104>(v);
-  v = v + pw[3] * dpp<0x108>(v);
+  v = madd(pw[0], dpp<0x101>(v), v);
+  v = madd(pw[1], dpp<0x102>(v), v);
+  v = madd(pw[2], dpp<0x104>(v), v);
+  v = madd(pw[3], dpp<0x108>(v), v);
   const T t3 = rdlane(v, 48);
-  const T t2 = rdlane(v, 32) + pw[4] * t3;
-  const T t1 = rdlane(v, 16) + pw[4] * t2;
+  const T t2 = madd(pw[4], t3, rdlane(v, 32));
+  const T t1 = madd(pw[4], t2, rdlane(v, 16));
   const int r = lane >> 4;
   const T next = r == 3 ? T(0) : (r == 2 ? t3 : (r == 1 ? t2 : t1));
-  return v + pb * next;
+  return madd(pb, next, v);
 }
 
 template <class T, bool PER, int L>
@@ -185,6 +185,13 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     if (need) wl[nw + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)k;
     nw += __popcll(bal);
   }
+  // flags of the lane's interfaces kb .. kb+L (weno-only: all)
+  const int kb = PER ? lane * L - 1 : lane * L;
+  const int il0 = PER ? kb : kb - 1;  // first node of interface kb's smooth stencil, + 2
+  const bool wo = P.weno_only != 0;
+  uint32_t fl = 0u;
+#pragma unroll
+  for (int j = 0; j <= L; ++j) fl |= (uint32_t)(wo | iflag(kb + j)) << j;
   // 3. per component: interface fluxes, divided difference, compact solve, store
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
@@ -199,10 +206,29 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     wave_sync();  // previous component's LDS reads are done
+    // component c of the lane's state and point flux into the LDS rows:
+    // one branch on the uniform c, each arm storing its own registers (the
+    // empty asm with a distinct text per arm keeps the compiler from merging
+    // the arms into per-value selects)
+    auto stage = [&](const T(&qq)[L], const T(&ff)[L]) {
 #pragma unroll
-    for (int j = 0; j < L; ++j) {  // selects, not a dynamic register index
-      qb[A(lane * L + j)] = c == 0 ? q[0][j] : (c == 1 ? q[1][j] : (c == 2 ? q[2][j] : q[3][j]));
-      fb[A(lane * L + j)] = c == 0 ? f[0][j] : (c == 1 ? f[1][j] : (c == 2 ? f[2][j] : f[3][j]));
+      for (int j = 0; j < L; ++j) {
+        qb[A(lane * L + j)] = qq[j];
+        fb[A(lane * L + j)] = ff[j];
+      }
+    };
+    if (c == 0) {
+      stage(q[0], f[0]);
+      asm volatile("; stage component 0" ::: "memory");
+    } else if (c == 1) {
+      stage(q[1], f[1]);
+      asm volatile("; stage component 1" ::: "memory");
+    } else if (c == 2) {
+      stage(q[2], f[2]);
+      asm volatile("; stage component 2" ::: "memory");
+    } else {
+      stage(q[3], f[3]);
+      asm volatile("; stage component 3" ::: "memory");
     }
     wave_sync();
     if (lane < kLineHalo) {  // halos: wrapped or clamped copies of the line ends
@@ -222,8 +248,8 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       for (int j = 0; j < 7; ++j) {
         const int ip = il - 3 + j;  // f+ stencil i-3 .. i+3
         const int im = il + 4 - j;  // f- stencil i+4 .. i-2
-        fp[j] = T(0.5) * (fb[A(ip)] + alpha * qb[A(ip)]);
-        fm[j] = T(0.5) * (fb[A(im)] - alpha * qb[A(im)]);
+        fp[j] = T(0.5) * madd(alpha, qb[A(ip)], fb[A(ip)]);
+        fm[j] = T(0.5) * madd(-alpha, qb[A(im)], fb[A(im)]);
       }
       ws[k] = weno7(fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6]) +
               weno7(fm[0], fm[1], fm[2], fm[3], fm[4], fm[5], fm[6]);
@@ -240,24 +266,34 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       }
     }
     wave_sync();
-    // hybrid interface flux of interfaces kb .. kb+L (one beyond the lane's own)
-    auto iface = [&](int k) -> T {
-      if (P.weno_only || iflag(k)) return (K.alpha_c * ws[k + 1] + ws[k]) + K.alpha_c * ws[k - 1];
-      const int il = PER ? k : k - 1;
-      return (c3 * (fb[A(il + 3)] + fb[A(il - 2)]) + c2 * (fb[A(il + 2)] + fb[A(il - 1)])) +
-             c1 * (fb[A(il + 1)] + fb[A(il)]);
-    };
-    const int kb = PER ? lane * L - 1 : lane * L;
-    T x[L];
+    // hybrid interface flux of interfaces kb .. kb+L (one beyond the lane's
+    // own; periodic lane 0 starts at interface -1, the halo copy of n-1):
+    // the 8th-order smooth flux of all L+1 from one window of L+6 point
+    // fluxes (interface kb+j reads window entries j .. j+5), then the
+    // compactified WENO flux where flagged
+    T Fi[L + 1];
     {
-      T Fprev = iface(kb);  // periodic lane 0: interface -1, the halo copy of interface n-1
+      T Fw[L + 6];
 #pragma unroll
-      for (int j = 0; j < L; ++j) {
-        const T Fn = iface(kb + j + 1);
-        x[j] = dv(Fn - Fprev, h);  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
-        Fprev = Fn;
+      for (int m = 0; m < L + 6; ++m) Fw[m] = fb[A(il0 - 2 + m)];
+#pragma unroll
+      for (int j = 0; j <= L; ++j)
+        Fi[j] = madd(c1, Fw[j + 3] + Fw[j + 2], madd(c3, Fw[j + 5] + Fw[j], c2 * (Fw[j + 4] + Fw[j + 1])));
+    }
+    if (fl != 0u) {  // window of WENO fluxes kb-1 .. kb+L+1 (entries of unlisted interfaces are not used)
+      T Ww[L + 3];
+#pragma unroll
+      for (int m = 0; m < L + 3; ++m) Ww[m] = ws[kb - 1 + m];
+#pragma unroll
+      for (int j = 0; j <= L; ++j) {
+        const T Fc = madd(K.alpha_c, Ww[j], madd(K.alpha_c, Ww[j + 2], Ww[j + 1]));
+        Fi[j] = ((fl >> j) & 1u) ? Fc : Fi[j];
       }
     }
+    T x[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+      x[j] = dv(Fi[j + 1] - Fi[j], h);  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
     // non-periodic: identity end rows, end contributions on the right-hand side
     T r0 = T(0), rn1 = T(0);
     if (!PER) {
@@ -275,11 +311,11 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
         if (i == 0 || i == n - 1) {
           xv = T(0);
         } else {
-          if (i == 1) xv = xv - K.alpha_c * r0;
-          if (i == n - 2) xv = xv - K.alpha_c * rn1;
+          if (i == 1) xv = madd(-K.alpha_c, r0, xv);
+          if (i == n - 2) xv = madd(-K.alpha_c, rn1, xv);
         }
       }
-      acc = xv + rho * acc;
+      acc = madd(rho, acc, xv);
       y[j] = acc;
     }
     // carries: inclusive scan of C_{l+1} = e_l + a C_l, a = rho^L
@@ -288,13 +324,13 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     if (PER) {
       // C_0 = (e_63 + a P_63) / (1 - rho^n) = S_63 / (1 - rho^n); C_l = P_l + a^l C_0
       const T c0 = rdlane(sc, 63) * K.inv1mrn;
-      carry = carry + a_l * c0;
+      carry = madd(a_l, c0, carry);
     }
     {
       T pw = rho;
 #pragma unroll
       for (int j = 0; j < L; ++j) {
-        y[j] = y[j] + pw * carry;
+        y[j] = madd(pw, carry, y[j]);
         pw = pw * rho;
       }
     }
@@ -304,7 +340,7 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     acc = T(0);
 #pragma unroll
     for (int j = L - 1; j >= 0; --j) {
-      acc = y[j] + rho * acc;
+      acc = madd(rho, acc, y[j]);
       y[j] = acc;
     }
     // carries from the right: D_{l-1} = g_l + a D_l
@@ -312,14 +348,14 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     carry = dpp<0x130>(sc);  // exclusive: Q_l (lane 63: 0)
     // wrap (periodic) or the value beyond the end (non-periodic): a^(63-l) times
     if (PER)
-      carry = carry + a_r * (rdlane(sc, 0) * K.inv1mrn);
+      carry = madd(a_r, rdlane(sc, 0) * K.inv1mrn, carry);
     else
-      carry = carry + a_r * zn;
+      carry = madd(a_r, zn, carry);
     {
       T pw = rho;
 #pragma unroll
       for (int j = L - 1; j >= 0; --j) {
-        y[j] = K.c1p * (y[j] + pw * carry);
+        y[j] = K.c1p * madd(pw, carry, y[j]);
         pw = pw * rho;
       }
     }
@@ -393,7 +429,11 @@ __global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
   const int w = threadIdx.x >> 6;
   const int line = blockIdx.x * kLineWaves + w;
   if (line >= nl) return;
-  line_wave_any<T, PER>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, wsm + (size_t)w * slice_bytes);
+  // one body per axis (a constant axis folds the flux selection)
+  if (axis == 0)
+    line_wave_any<T, PER>(P, K, 0, enforced_input, ql, ml, scal, out, line, nl, wsm + (size_t)w * slice_bytes);
+  else
+    line_wave_any<T, PER>(P, K, 1, enforced_input, ql, ml, scal, out, line, nl, wsm + (size_t)w * slice_bytes);
 }
 
 // The block form handles lines that are not a multiple of 64 nodes or longer
