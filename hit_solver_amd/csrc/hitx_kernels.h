@@ -8,6 +8,7 @@
 
 #include "../../include/hitx.h"
 #include "hitx_device.h"
+#include "hitx_wave.h"
 
 namespace hitx {
 
@@ -828,6 +829,141 @@ __global__ void __launch_bounds__(kBlock) k_sensor(Params<T> P, const T* __restr
 template <class T>
 __device__ __forceinline__ T pwk(const T* PW, int k) { return k <= 65 ? PW[k] : T(0); }
 
+// The one-chunk shape of the compact solve (chunks of L <= 4, nch = n/L <=
+// 128 per line, 4 lines): component c takes threads 128c .. 128c+127, two
+// wavefronts, thread 128c + j holding chunk j (none for j >= nch).  Each
+// recurrence sweeps the chunks locally, joins them with the wavefront affine
+// scan (decay a = rho^L) and one exchange of the two wavefronts' end values
+// through LDS, and closes the periodic wrap in closed form: carry into chunk
+// 0 = S_{nch-1} / (1 - rho^n).  Powers rho^k come from the table (zero beyond
+// k = 65, as the truncated carries of the general shape).  Same result as
+// the general shape to rounding.
+template <class T, bool PER>
+__device__ __forceinline__ void solve_store_scan(const T* __restrict__ F, int S, T h, const LineK<T>& K,
+                                                 const T* __restrict__ PW, T* __restrict__ scratch,
+                                                 T* __restrict__ out, size_t NL, size_t g0) {
+  const int n = K.n, L = K.L, nch = K.nch;
+  const T rho = K.rho;
+  const int t = threadIdx.x, line = t >> 7, ch = t & 127, wv = ch >> 6, lane = t & 63;
+  const bool act = ch < nch;
+  const int s0 = ch * L;
+  T* xf = scratch;             // [4][2] wavefront end values of the causal scan
+  T* xb = scratch + 8;         // [4][2] wavefront start values of the anti-causal scan
+  T* bnd = scratch + 8 * nch;  // [4][3]: z_n, p_0, p_{n-1}
+  auto raw = [&](int c, int i) -> T {
+    const T* f = F + c * S;
+    return PER ? (f[i] - f[wrap(i - 1, n)]) / h : (f[i + 1] - f[i]) / h;
+  };
+  // scan powers a^1, a^2, a^4, a^8, a^16 and the lane's a^(r+1), a^(16-r), r = lane % 16
+  T apw[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) apw[k] = pwk(PW, L << k);
+  const T a_f = pwk(PW, L * ((lane & 15) + 1)), a_b = pwk(PW, L * (16 - (lane & 15)));
+  const int last = nch - 1;  // last chunk of the line
+  T y[4];
+  T r0 = T(0), rn1 = T(0);
+  if (!PER) {
+    r0 = raw(line, 0);
+    rn1 = raw(line, n - 1);
+  }
+  // causal pass: local sweep, then the scan of the chunk end values
+  T acc = T(0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k < L && act) {
+      const int i = s0 + k;
+      T xv = raw(line, i);
+      if (!PER) {
+        if (i == 0 || i == n - 1) {
+          xv = T(0);
+        } else {
+          if (i == 1) xv = madd(-K.alpha_c, r0, xv);
+          if (i == n - 2) xv = madd(-K.alpha_c, rn1, xv);
+        }
+      }
+      acc = madd(rho, acc, xv);
+      y[k] = acc;
+    }
+  }
+  T sc = scan_fwd(act ? acc : T(0), apw, a_f, lane);  // inclusive, within the wavefront
+  {
+    const int lw = wv == 0 ? (last < 63 ? last : 63) : last - 64;  // last live lane of this wavefront
+    if (lw >= 0) {
+      const T e = rdlane(sc, lw);
+      if (lane == 0) xf[line * 2 + wv] = e;
+    }
+  }
+  __syncthreads();
+  const T e0 = xf[line * 2];
+  if (wv == 1) sc = madd(pwk(PW, L * (lane + 1)), e0, sc);  // chunks 0..63 into 64 + lane
+  T carry = dpp<0x138>(sc);                                  // exclusive
+  if (wv == 1 && lane == 0) carry = e0;
+  if (PER) {
+    // S_{nch-1} over the line, then the wrap: C_0 * a^ch
+    const T tot = nch > 64 ? madd(pwk(PW, L * (nch - 64)), e0, xf[line * 2 + 1]) : e0;
+    carry = madd(pwk(PW, L * ch), tot * K.inv1mrn, carry);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < L) y[k] = madd(pwk(PW, k + 1), carry, y[k]);
+  if (!PER && ch == last) bnd[line * 3 + 0] = (rho * y[L - 1]) / K.one_m_rho2;
+  // anti-causal pass: local sweep, then the reversed scan of the chunk start values
+  acc = T(0);
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (k < L && act) {
+      acc = madd(rho, acc, y[k]);
+      y[k] = acc;
+    }
+  }
+  sc = scan_bwd(act ? acc : T(0), apw, a_b, lane);  // inclusive, within the wavefront
+  if (wv * 64 <= last && lane == 0) xb[line * 2 + wv] = sc;
+  __syncthreads();
+  const T b1 = nch > 64 ? xb[line * 2 + 1] : T(0);  // chunk 64's inclusive value
+  if (wv == 0 && nch > 64) sc = madd(pwk(PW, L * (64 - lane)), b1, sc);
+  carry = dpp<0x130>(sc);  // exclusive
+  if (wv == 0 && lane == 63) carry = b1;
+  if (PER) {
+    const T tot = wv == 0 ? rdlane(sc, 0) : madd(pwk(PW, L * 64), b1, xb[line * 2]);  // T_0 over the line
+    carry = madd(pwk(PW, L * (last - ch)), tot * K.inv1mrn, carry);
+  } else {
+    carry = madd(pwk(PW, L * (last - ch)), bnd[line * 3 + 0], carry);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < L) y[k] = K.c1p * madd(pwk(PW, L - k), carry, y[k]);
+  if (!PER) {
+    if (ch == 0) bnd[line * 3 + 1] = y[0];
+    if (ch == last) bnd[line * 3 + 2] = y[L - 1];
+    __syncthreads();
+  }
+  if (!act) return;
+  T* o = out + line * NL + g0;
+  if (PER) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < L) o[s0 + k] = y[k];
+  } else {
+    const T p0 = bnd[line * 3 + 1], pn = bnd[line * 3 + 2];
+    const T det = T(1) - K.rnm1 * K.rnm1;
+    const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k < L) {
+        const int i = s0 + k;
+        T v;
+        if (i == 0)
+          v = raw(line, 0);
+        else if (i == n - 1)
+          v = raw(line, n - 1);
+        else
+          v = (y[k] + a * pwk(PW, i)) + bb * pwk(PW, n - 1 - i);
+        o[i] = v;
+      }
+    }
+  }
+}
+
 // Divided difference of the interface flux F[c*S + .] and compact solve of
 // the four (alpha, 1, alpha) systems, stored to out[c*NL + g0 + i].
 // Periodic: cyclic matrix (OOP/spatial_operator.py:227-249 solves it by FFT).
@@ -858,6 +994,10 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
     const T* f = F + c * S;
     return PER ? (f[i] - f[wrap(i - 1, n)]) / h : (f[i + 1] - f[i]) / h;
   };
+  if constexpr (SMALL) {
+    solve_store_scan<T, PER>(F, S, h, K, PW, scratch, out, NL, g0);
+    return;
+  }
   constexpr int NT = SMALL ? 1 : 2, LM = SMALL ? 4 : 8;
   T y[NT][LM];
   // causal pass, local
