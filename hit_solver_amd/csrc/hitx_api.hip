@@ -216,8 +216,16 @@ Params<T> make_params(const hitx_plan* pl) {
 // dynamic LDS of the forcing projection: the lanes' column sums, 2*fky+1 per lane
 size_t proj_lds_bytes(const hitx_plan* pl) { return (size_t)kWaves * 64 * (2 * pl->fky + 1) * sizeof(double); }
 
-size_t line_lds_bytes(int n, int nch, int esize) {
-  return (size_t)(8 * (n + 1) + 66 + 8 * nch + 20) * esize + 4 * (size_t)(n + 1) + n + 64;
+// LDS of a line block (line_body): state and flux planes 8 (n+1), the power
+// table 66, the scratch (one-chunk shape: the WENO staging of CPP planes and
+// the solve's exchange slots; otherwise the chunk results 8 nch + 20, which
+// also hold one staging plane), the work list (uint16, n+1), the mask.
+size_t line_lds_scratch(int n, int nch, bool small, int esize) {
+  const size_t S = (size_t)n + 1, cpp = small ? 2 : 1;
+  return small ? std::max(cpp * S, (size_t)28) : 8 * (size_t)nch + 20;
+}
+size_t line_lds_bytes(int n, int nch, bool small, int esize) {
+  return (size_t)(8 * (n + 1) + 66 + line_lds_scratch(n, nch, small, esize)) * esize + 2 * (size_t)(n + 1) + n + 64;
 }
 
 // Lines in one-chunk shape (every launched line small) go in one launch;
@@ -339,11 +347,15 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
   } else {
     const dim3 gt((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
     k_stage_tile<T><<<gt, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, (int)(gt.x * gt.y), w.scal);
+    // the split sensor's raw-mask tiles finalize the reduction themselves
+    if (!use_split_sensor(pl)) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, (int)(gt.x * gt.y), w.scal);
   }
   dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
   if (use_split_sensor(pl)) {
-    k_rawtile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
+    if (P.drag_on)
+      k_rawtile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
+    else
+      k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, (int)(g.x * g.y), w.scal, w.raw);
     k_dilate<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.raw, mask_out, w.maskT, qT);
   } else {
     k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
@@ -570,8 +582,8 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->ly_d = make_linek<double>(desc->ny);
   pl->lx_f = make_linek<float>(desc->nx);
   pl->ly_f = make_linek<float>(desc->ny);
-  pl->line_lds_x = line_lds_bytes(desc->nx, pl->lx_d.nch, pl->esize);
-  pl->line_lds_y = line_lds_bytes(desc->ny, pl->ly_d.nch, pl->esize);
+  pl->line_lds_x = line_lds_bytes(desc->nx, pl->lx_d.nch, pl->lx_d.small != 0, pl->esize);
+  pl->line_lds_y = line_lds_bytes(desc->ny, pl->ly_d.nch, pl->ly_d.small != 0, pl->esize);
   pl->sensor_lds = sensor_lds_bytes(desc->sensor_width, pl->esize);
   rc = pl->esize == 8 ? set_line_lds<double>(pl) : set_line_lds<float>(pl);
   if (rc) {

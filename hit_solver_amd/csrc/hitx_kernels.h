@@ -235,15 +235,10 @@ __global__ void __launch_bounds__(kBlock) k_stage_reduce(Params<T> P, const T* _
   stage_reduce_body<T, DRAG>(P, q, enforce_first, part, blockIdx.x);
 }
 
-// One block: reduce the partial rows in a fixed order and write the stage
-// scalars (a separate launch: kernel boundaries order the partial writes).
+// The stage scalars from the reduced row r: thread 0 writes the scalar
+// slots, the block the drag slots.
 template <class T, bool DRAG>
-__device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
-                                                    double* __restrict__ scal) {
-  constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
-  __shared__ double r[NR];
-  reduce_partials<NR, 0x1Full>(part, nblk, r);
-  __syncthreads();
+__device__ __forceinline__ void stage_scal_write(const Params<T>& P, const double* r, double* __restrict__ scal) {
   const double N = (double)P.nx * (double)P.ny;
   if (threadIdx.x == 0) {
     scal[S_EPS_R] = 1e-14 + 1e-12 * r[R_MAXR];
@@ -266,6 +261,18 @@ __device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const do
   }
   if (DRAG)
     for (int k = threadIdx.x; k < 4 * P.ndrag; k += blockDim.x) scal[S_DRAG + k] = r[R_DRAG + k] / N;
+}
+
+// One block: reduce the partial rows in a fixed order and write the stage
+// scalars (a separate launch: kernel boundaries order the partial writes).
+template <class T, bool DRAG>
+__device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ scal) {
+  constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
+  __shared__ double r[NR];
+  reduce_partials<NR, 0x1Full>(part, nblk, r);
+  __syncthreads();
+  stage_scal_write<T, DRAG>(P, r, scal);
 }
 
 // Per-row table of the drag low-pass (one thread per row and field): for row
@@ -808,6 +815,26 @@ __global__ void __launch_bounds__(kBlock) k_rawtile(Params<T> P, const T* __rest
   rawtile_body<T>(P, q, enforce_first, scal, raw, t_.bx, t_.by);
 }
 
+// The raw mask with the stage finalize folded in (no drag projection): every
+// block reduces the tile partials itself, in the finalize's order (the same
+// 256-thread reduction, so every block and a separate finalize agree to the
+// bit), keeps the stage scalars in LDS, and block 0 also writes them for the
+// later kernels of the stage.  Saves the one-block finalize launch.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                        const double* __restrict__ part, int nblk,
+                                                        double* __restrict__ scal, uint8_t* __restrict__ raw) {
+  const Tile2 t_ = tile_xcd();
+  __shared__ double r[R_DRAG];
+  __shared__ double sc[S_DRAG];
+  reduce_partials<R_DRAG, 0x1Full>(part, nblk, r);
+  __syncthreads();
+  stage_scal_write<T, false>(P, r, sc);
+  if (blockIdx.x == 0 && blockIdx.y == 0) stage_scal_write<T, false>(P, r, scal);
+  __syncthreads();
+  rawtile_body<T>(P, q, enforce_first, sc, raw, t_.bx, t_.by);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_dilate(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                    const uint8_t* __restrict__ raw, uint8_t* __restrict__ mask,
@@ -847,12 +874,13 @@ __device__ __forceinline__ void solve_store_scan(const T* __restrict__ F, int S,
   const int t = threadIdx.x, line = t >> 7, ch = t & 127, wv = ch >> 6, lane = t & 63;
   const bool act = ch < nch;
   const int s0 = ch * L;
-  T* xf = scratch;             // [4][2] wavefront end values of the causal scan
-  T* xb = scratch + 8;         // [4][2] wavefront start values of the anti-causal scan
-  T* bnd = scratch + 8 * nch;  // [4][3]: z_n, p_0, p_{n-1}
+  T* xf = scratch;        // [4][2] wavefront end values of the causal scan
+  T* xb = scratch + 8;    // [4][2] wavefront start values of the anti-causal scan
+  T* bnd = scratch + 16;  // [4][3]: z_n, p_0, p_{n-1}
+  const T ih = T(1) / h;  // divided differences times the reciprocal (within 2 ulp of the quotient)
   auto raw = [&](int c, int i) -> T {
     const T* f = F + c * S;
-    return PER ? (f[i] - f[wrap(i - 1, n)]) / h : (f[i + 1] - f[i]) / h;
+    return PER ? (f[i] - f[wrap(i - 1, n)]) * ih : (f[i + 1] - f[i]) * ih;
   };
   // scan powers a^1, a^2, a^4, a^8, a^16 and the lane's a^(r+1), a^(16-r), r = lane % 16
   T apw[5];
@@ -990,9 +1018,10 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
   T* e1 = scratch;             // [4*nch] local causal results
   T* e2 = scratch + 4 * nch;   // [4*nch] local anti-causal results
   T* bnd = scratch + 8 * nch;  // [4][3]: z_n, p_0, p_{n-1}
+  const T ih = T(1) / h;  // divided differences times the reciprocal (within 2 ulp of the quotient)
   auto raw = [&](int c, int i) -> T {
     const T* f = F + c * S;
-    return PER ? (f[i] - f[wrap(i - 1, n)]) / h : (f[i + 1] - f[i]) / h;
+    return PER ? (f[i] - f[wrap(i - 1, n)]) * ih : (f[i + 1] - f[i]) * ih;
   };
   if constexpr (SMALL) {
     solve_store_scan<T, PER>(F, S, h, K, PW, scratch, out, NL, g0);
@@ -1158,8 +1187,10 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
   T* A = reinterpret_cast<T*>(smem);
   T* B = A + 4 * S;
   T* PW = B + 4 * S;                // 66
-  T* scratch = PW + 66;             // 8*nch + 20
-  int* wl = reinterpret_cast<int*>(scratch + 8 * K.nch + 20);  // WENO work list, n+1
+  T* scratch = PW + 66;             // one-chunk shape: max(CPP*S, 28); else 8*nch + 20
+  constexpr int CPP = SMALL ? 2 : 1;
+  const int nscr = SMALL ? (CPP * S > 28 ? CPP * S : 28) : 8 * K.nch + 20;
+  uint16_t* wl = reinterpret_cast<uint16_t*>(scratch + nscr);  // WENO work list, n+1
   uint8_t* Mk = reinterpret_cast<uint8_t*>(wl + S);
   __shared__ int wcnt[NWV];
   const size_t NL = (size_t)nl * n;
@@ -1230,20 +1261,21 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
       off += j < wv_id ? wcnt[j] : 0;
       tot += wcnt[j];
     }
-    if (fa) wl[off + pre] = ka;
-    if (fb) wl[off + pre + (fa ? 1 : 0)] = kb;
+    if (fa) wl[off + pre] = (uint16_t)ka;
+    if (fb) wl[off + pre + (fa ? 1 : 0)] = (uint16_t)kb;
     nw += tot;
     __syncthreads();
   }
   // 2b/3. CPP components per pass: WENO7 of the Lax-Friedrichs split at the
   //       listed interfaces into W (the solve's scratch, free until step 4;
-  //       it holds two planes in the one-chunk shape, L <= 4), then the hybrid
+  //       two planes for float in the one-chunk shape, one otherwise, which
+  //       keeps a double line block within a quarter of the CU's LDS: four
+  //       blocks per CU), then the hybrid
   //       interface flux -- compactified WENO where flagged, smooth compact
   //       flux elsewhere -- over those components' state planes.  Each WENO
   //       item runs on a lane pair: the even lane the f+ stencil, the odd
   //       lane f-, summed (f+ first) through a lane exchange.
-  constexpr int CPP = SMALL ? 2 : 1;
-  T* W = scratch;  // CPP * (n + 1) <= 8 nch + 20 values
+  T* W = scratch;  // CPP * (n + 1) values
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
   // unrolled for float, rolled for double: each keeps the kernel near

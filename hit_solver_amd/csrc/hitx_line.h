@@ -58,6 +58,7 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
   const size_t NL = (size_t)nl * n;
   const T alpha = (T)scal[axis == 0 ? S_ALPHA_X : S_ALPHA_Y];
   const T h = axis == 0 ? P.dx : P.dy;
+  const T ih = T(1) / h;  // divided differences times the reciprocal (within 2 ulp of the quotient)
   const T rho = K.rho;
   // 1. (enforce), primitives, point flux of the lane's L nodes
   T q[4][L], f[4][L];
@@ -217,7 +218,7 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     T x[L];
 #pragma unroll
     for (int j = 0; j < L; ++j)
-      x[j] = dv(Fi[j + 1] - Fi[j], h);  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
+      x[j] = (Fi[j + 1] - Fi[j]) * ih;  // periodic (F_i - F_{i-1})/dx; non-periodic (F_{j+1} - F_j)/dx
     // non-periodic: identity end rows, end contributions on the right-hand side
     T r0 = T(0), rn1 = T(0);
     if (!PER) {
