@@ -325,6 +325,7 @@ int check_ws(const hitx_plan* pl, void* ws, size_t ws_bytes) {
 }
 
 dim3 grid16(const hitx_plan* pl) { return dim3((pl->d.nx + 15) / 16, (pl->d.ny + 15) / 16); }
+dim3 grid_update(const hitx_plan* pl) { return dim3((pl->d.nx + kUTX - 1) / kUTX, (pl->d.ny + kUTY - 1) / kUTY); }
 
 // stage reductions (one kernel, last block finalizes) + fused sensor tile:
 // mask and maskT of q, and the enforced transposed state when qT != nullptr
@@ -483,7 +484,7 @@ int rhs_or_stage(hitx_plan* pl, int mode, int stage, const void* q0, const void*
   int rc = run_sweeps<T>(pl, static_cast<const T*>(qs), w, s);
   if (rc) return rc;
   const Params<T> P = make_params<T>(pl);
-  k_update<T><<<grid16(pl), kBlock, 0, s>>>(P, mode, stage, static_cast<const T*>(q0), static_cast<const T*>(qs),
+  k_update<T><<<grid_update(pl), kUBlock, 0, s>>>(P, mode, stage, static_cast<const T*>(q0), static_cast<const T*>(qs),
                                             static_cast<const T*>(w.dfx), static_cast<const T*>(w.dfyT),
                                             static_cast<const T*>(fx), static_cast<const T*>(fy), w.scal, w.drows,
                                             (T)dt, static_cast<T*>(out));
@@ -964,13 +965,13 @@ int hitx_viscous_rhs(hitx_plan* pl, const void* q, void* out, void* stream) {
   if (pl->esize == 8) {
     Params<double> P = make_params<double>(pl);
     P.visc_on = 1;
-    k_update<double><<<grid16(pl), kBlock, 0, s>>>(P, MODE_VISC, 0, nullptr, static_cast<const double*>(q), nullptr,
+    k_update<double><<<grid_update(pl), kUBlock, 0, s>>>(P, MODE_VISC, 0, nullptr, static_cast<const double*>(q), nullptr,
                                                    nullptr, nullptr, nullptr, nullptr, nullptr, 0.0,
                                                    static_cast<double*>(out));
   } else {
     Params<float> P = make_params<float>(pl);
     P.visc_on = 1;
-    k_update<float><<<grid16(pl), kBlock, 0, s>>>(P, MODE_VISC, 0, nullptr, static_cast<const float*>(q), nullptr,
+    k_update<float><<<grid_update(pl), kUBlock, 0, s>>>(P, MODE_VISC, 0, nullptr, static_cast<const float*>(q), nullptr,
                                                   nullptr, nullptr, nullptr, nullptr, nullptr, 0.0f,
                                                   static_cast<float*>(out));
   }
@@ -1214,7 +1215,7 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
       }
     }
   }
-  b_update<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n, st);
+  b_update<T><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
   LAUNCHCHK();
   return HITX_OK;
 }

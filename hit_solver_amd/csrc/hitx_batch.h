@@ -188,11 +188,13 @@ __global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
     line_wave_any<T, PER>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, line, nl, wsm + (size_t)w * slice_bytes);
 }
 
+// update tiles are the sensor's 32x16 tiles (the same block offsets)
+static_assert(kUTX == kSTX && kUTY == kSTY, "update and sensor tiles differ");
 template <class T>
-__global__ void __launch_bounds__(kBlock) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
-                                                   int ninst, int st) {
-  HITX_BATCH_TILE(OFF_T16)
-  const int ntx = (I.P.nx + 15) / 16;
+__global__ void __launch_bounds__(kUBlock) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+                                                    int ninst, int st) {
+  HITX_BATCH_TILE(OFF_SENS)
+  const int ntx = (I.P.nx + kUTX - 1) / kUTX;
   const int by = lb / ntx, bx = lb - by * ntx;
   // the last stage writes Q in place when no hyperviscosity pass follows and
   // there is no outflow guard (each node then reads Q only at itself)

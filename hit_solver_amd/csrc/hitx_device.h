@@ -150,7 +150,7 @@ __device__ __forceinline__ Prim<T> prim(const Params<T>& P, T q0, T q1, T q2, T 
   r.rho = npmax(q0, P.rho_floor);
   r.u = dv(q1, r.rho);
   r.v = dv(q2, r.rho);
-  T ke = (T(0.5) * r.rho) * (r.u * r.u + r.v * r.v);
+  T ke = (T(0.5) * r.rho) * madd(r.u, r.u, r.v * r.v);
   T p = P.gm1 * (q3 - ke);
   r.p = npmax(p, P.p_floor);
   return r;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void enforce(const Params<T>& P, T& q0, T& q1, T& q2,
   T rho = npmax(q0, P.rho_floor);
   T u = dv(q1, rho);
   T v = dv(q2, rho);
-  T ke = (T(0.5) * rho) * (u * u + v * v);
+  T ke = (T(0.5) * rho) * madd(u, u, v * v);
   T p = P.gm1 * (q3 - ke);
   q0 = rho;
   q1 = rho * u;
@@ -170,12 +170,26 @@ __device__ __forceinline__ void enforce(const Params<T>& P, T& q0, T& q1, T& q2,
   q3 = (p > P.p_floor) ? q3 : (P.pf_gm1 + ke);
 }
 
+// Element i of a plane as a base pointer plus a 32-bit byte offset: global
+// memory instructions then take the (uniform) base in SGPRs and the offset
+// in one VGPR, with no 64-bit address arithmetic per lane.  Every plane here
+// has fewer than 2^29 elements (nx*ny*4 <= INT32_MAX), so i * sizeof(T) <
+// 2^32.
+template <class T>
+__device__ __forceinline__ const T& at32(const T* p, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (size_t)(i * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ T& at32(T* p, uint32_t i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (size_t)(i * (uint32_t)sizeof(T)));
+}
+
 template <class T>
 __device__ __forceinline__ void load4(const T* q, int N, int idx, T& a, T& b, T& c, T& d) {
-  a = q[idx];
-  b = q[N + idx];
-  c = q[2 * N + idx];
-  d = q[3 * N + idx];
+  a = at32(q, idx);
+  b = at32(q + (size_t)N, idx);
+  c = at32(q + 2 * (size_t)N, idx);
+  d = at32(q + 3 * (size_t)N, idx);
 }
 
 // Primitives of q (enforce_first: of enforce(q)).

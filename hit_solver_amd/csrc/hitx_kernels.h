@@ -1349,7 +1349,11 @@ __global__ void __launch_bounds__(kLineBlock, 4)
 // second-order central differences on the 13-point diamond.  U, V, Tt are
 // the node primitives u, v, p/(rho R) of a tile with a 2-node halo in LDS
 // (row stride kVS); (ly, lx) is the node's position in that tile.
-constexpr int kVS = 21;
+// Update tiles are kUTX x kUTY nodes, one thread each: a row of the tile is
+// one 128-byte line of an FP32 plane (two of an FP64 plane).
+constexpr int kUTX = 32, kUTY = 16, kUBlock = kUTX * kUTY;
+constexpr int kVW = kUTX + 4, kVH = kUTY + 4;  // viscous tile with its 2-node halo
+constexpr int kVS = kVW + 1;
 template <class T>
 __device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, const T* V, const T* Tt, int ly, int lx,
                                             T out[4]) {
@@ -1358,16 +1362,16 @@ __device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, cons
   auto uy = [&](int a, int b) { return dv(at(U, a + 1, b) - at(U, a - 1, b), P.two_dy); };
   auto vx = [&](int a, int b) { return dv(at(V, a, b + 1) - at(V, a, b - 1), P.two_dx); };
   auto vy = [&](int a, int b) { return dv(at(V, a + 1, b) - at(V, a - 1, b), P.two_dy); };
-  auto txx = [&](int a, int b) { return P.two_mu * ux(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
-  auto tyy = [&](int a, int b) { return P.two_mu * vy(a, b) - P.tt_mu * (ux(a, b) + vy(a, b)); };
+  auto txx = [&](int a, int b) { return madd(-P.tt_mu, ux(a, b) + vy(a, b), P.two_mu * ux(a, b)); };
+  auto tyy = [&](int a, int b) { return madd(-P.tt_mu, ux(a, b) + vy(a, b), P.two_mu * vy(a, b)); };
   auto txy = [&](int a, int b) { return P.mu * (uy(a, b) + vx(a, b)); };
   auto ex = [&](int a, int b) {
     const T tx = dv(at(Tt, a, b + 1) - at(Tt, a, b - 1), P.two_dx);
-    return (at(U, a, b) * txx(a, b) + at(V, a, b) * txy(a, b)) + P.kcond * tx;
+    return madd(P.kcond, tx, madd(at(V, a, b), txy(a, b), at(U, a, b) * txx(a, b)));
   };
   auto ey = [&](int a, int b) {
     const T ty = dv(at(Tt, a + 1, b) - at(Tt, a - 1, b), P.two_dy);
-    return (at(U, a, b) * txy(a, b) + at(V, a, b) * tyy(a, b)) + P.kcond * ty;
+    return madd(P.kcond, ty, madd(at(V, a, b), tyy(a, b), at(U, a, b) * txy(a, b)));
   };
   out[0] = T(0);
   out[1] = dv(txx(0, 1) - txx(0, -1), P.two_dx) + dv(txy(1, 0) - txy(-1, 0), P.two_dy);
@@ -1375,10 +1379,10 @@ __device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, cons
   out[3] = dv(ex(0, 1) - ex(0, -1), P.two_dx) + dv(ey(1, 0) - ey(-1, 0), P.two_dy);
 }
 
-// Primitives u, v, p/(rho R) of the 20x20 tile around a 16x16 block
-// (periodic wrap) into LDS.
+// Primitives u, v, p/(rho R) of the kVW x kVH tile around a kUTX x kUTY
+// block (periodic wrap) into LDS.
 // Split in two so that a caller can issue the loads, do other work, then
-// form the primitives: each thread holds at most two of the 400 tile nodes.
+// form the primitives: each thread holds at most two of the tile nodes.
 template <class T>
 struct VTileLoad {
   T a[2][4];
@@ -1390,9 +1394,9 @@ __device__ __forceinline__ VTileLoad<T> viscous_tile_load(const Params<T>& P, co
   VTileLoad<T> v;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const int i = threadIdx.x + s * kBlock;
-    if (i < 400) {
-      const int ly = i / 20, lx = i - ly * 20;
+    const int i = threadIdx.x + s * kUBlock;
+    if (i < kVW * kVH) {
+      const int ly = i / kVW, lx = i - ly * kVW;
       const int gy = wrap(y0 - 2 + ly, ny), gx = wrap(x0 - 2 + lx, nx);
       load4(q, N, gy * nx + gx, v.a[s][0], v.a[s][1], v.a[s][2], v.a[s][3]);
     }
@@ -1403,9 +1407,9 @@ template <class T>
 __device__ __forceinline__ void viscous_tile_store(const Params<T>& P, const VTileLoad<T>& v, T* U, T* V, T* Tt) {
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const int i = threadIdx.x + s * kBlock;
-    if (i < 400) {
-      const int ly = i / 20, lx = i - ly * 20;
+    const int i = threadIdx.x + s * kUBlock;
+    if (i < kVW * kVH) {
+      const int ly = i / kVW, lx = i - ly * kVW;
       const Prim<T> pr = prim(P, v.a[s][0], v.a[s][1], v.a[s][2], v.a[s][3]);
       U[ly * kVS + lx] = pr.u;
       V[ly * kVS + lx] = pr.v;
@@ -1421,8 +1425,8 @@ enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
 // the Shu-Osher combination (OOP/time_operator.py:24-26) and clean =
 // enforce (+ outflow guard: every node takes the value of its clamped
 // interior source, OOP/problems/riemann_config3.py:419-429).
-// One 16x16 tile per block: the y-derivative (stored transposed, [c][x][y])
-// is staged through LDS so both derivative reads are coalesced.
+// One kUTX x kUTY tile per block: the y-derivative (stored transposed,
+// [c][x][y]) is staged through LDS so both derivative reads are coalesced.
 template <class T>
 __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
                                             const T* __restrict__ qs, const T* __restrict__ dfx,
@@ -1430,11 +1434,11 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
                                             const T* __restrict__ fy, const double* __restrict__ scal,
                                             const double* __restrict__ drows, T dt, T* __restrict__ out, int bx,
                                             int by, T fsc = T(1), T fal = T(1)) {
-  __shared__ T sU[20 * kVS], sV[20 * kVS], sT[20 * kVS];
-  __shared__ T sD[4][16][17];
+  __shared__ T sU[kVH * kVS], sV[kVH * kVS], sT[kVH * kVS];
+  __shared__ T sD[4][kUTX][kUTY + 1];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x0 = bx * 16, y0 = by * 16;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int x0 = bx * kUTX, y0 = by * kUTY;
+  const int tx = threadIdx.x % kUTX, ty = threadIdx.x / kUTX;
   const int x = x0 + tx, y = y0 + ty;
   const bool in = x < nx && y < ny;
   int cx = x, cy = y;
@@ -1454,14 +1458,14 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   const bool drag_node = in && mode == MODE_STAGE && P.drag_on;
   if (in && mode != MODE_VISC) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) gx[c] = dfx[c * N + idx];
+    for (int c = 0; c < 4; ++c) gx[c] = at32(dfx + (size_t)c * N, idx);
     if (mode == MODE_STAGE) {
       load4(qs, N, idx, s0, s1, s2, s3);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) a0[c] = q0[c * N + idx];
+      for (int c = 0; c < 4; ++c) a0[c] = at32(q0 + (size_t)c * N, idx);
       if (P.force_on) {
-        f1r = fx[idx];
-        f2r = fy[idx];
+        f1r = at32(fx, idx);
+        f2r = at32(fy, idx);
       }
     }
   }
@@ -1481,10 +1485,10 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   if (visc) vt = viscous_tile_load(P, qs, x0, y0);
   T dsh[4];
   if (mode != MODE_VISC) {
-    // dfyT tile: thread (ty, tx) reads x-row x0+ty, y = y0+tx (unit stride in y)
-    const int rx = x0 + ty, ry = y0 + tx;
+    // dfyT tile: thread t reads x-row x0 + t / kUTY, y = y0 + t % kUTY (unit stride in y)
+    const int rx = x0 + (int)threadIdx.x / kUTY, ry = y0 + (int)threadIdx.x % kUTY;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) dsh[c] = (rx < nx && ry < ny) ? dfyT[c * N + (size_t)rx * ny + ry] : T(0);
+    for (int c = 0; c < 4; ++c) dsh[c] = (rx < nx && ry < ny) ? at32(dfyT + (size_t)c * N, rx * ny + ry) : T(0);
   }
   // node-local source terms while the tile loads are in flight
   T fsrc[4] = {T(0), T(0), T(0), T(0)}, dsrc[4] = {T(0), T(0), T(0), T(0)};
@@ -1494,7 +1498,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       const T f1 = (f1r * fsc) * fal, f2 = (f2r * fsc) * fal;
       fsrc[1] = st.rho * f1;
       fsrc[2] = st.rho * f2;
-      fsrc[3] = st.rho * (st.u * f1 + st.v * f2);
+      fsrc[3] = st.rho * madd(st.v, f2, st.u * f1);
     }
     if (P.drag_on) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
       double lx = dH[0], ly = dH[5];
@@ -1508,13 +1512,13 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
       dsrc[1] = d1;
       dsrc[2] = d2;
-      dsrc[3] = st.u * d1 + st.v * d2;
+      dsrc[3] = madd(st.v, d2, st.u * d1);
     }
   }
   if (visc) viscous_tile_store(P, vt, sU, sV, sT);
   if (mode != MODE_VISC) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sD[c][tx][ty] = dsh[c];
+    for (int c = 0; c < 4; ++c) sD[c][threadIdx.x / kUTY][threadIdx.x % kUTY] = dsh[c];
   }
   __syncthreads();
   if (!in) return;
@@ -1522,19 +1526,19 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
     T vv[4];
     viscous_lds(P, sU, sV, sT, ty + 2, tx + 2, vv);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) out[c * N + oidx] = vv[c];
+    for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = vv[c];
     return;
   }
   const bool own = cx == x && cy == y;  // guard nodes take their interior source's values
   T Lv[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const T gy = own ? sD[c][ty][tx] : ld_global(dfyT + c * N + (size_t)cx * ny + cy);
+    const T gy = own ? sD[c][tx][ty] : ld_global(dfyT + c * N + (size_t)cx * ny + cy);
     Lv[c] = (-gx[c]) - gy;
   }
   if (mode == MODE_RHS) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) out[c * N + oidx] = Lv[c];
+    for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = Lv[c];
     return;
   }
   if (P.visc_on) {
@@ -1558,19 +1562,19 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     if (stage == 0)
-      o[c] = a0[c] + dt * Lv[c];
+      o[c] = madd(dt, Lv[c], a0[c]);
     else if (stage == 1)
-      o[c] = T(0.75) * a0[c] + T(0.25) * (qsv[c] + dt * Lv[c]);
+      o[c] = madd(T(0.75), a0[c], T(0.25) * madd(dt, Lv[c], qsv[c]));
     else
-      o[c] = T(1.0 / 3.0) * a0[c] + T(2.0 / 3.0) * (qsv[c] + dt * Lv[c]);
+      o[c] = madd(T(1.0 / 3.0), a0[c], T(2.0 / 3.0) * madd(dt, Lv[c], qsv[c]));
   }
   enforce(P, o[0], o[1], o[2], o[3]);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) out[c * N + oidx] = o[c];
+  for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = o[c];
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_update(Params<T> P, int mode, int stage, const T* __restrict__ q0,
+__global__ void __launch_bounds__(kUBlock) k_update(Params<T> P, int mode, int stage, const T* __restrict__ q0,
                                                    const T* __restrict__ qs, const T* __restrict__ dfx,
                                                    const T* __restrict__ dfyT, const T* __restrict__ fx,
                                                    const T* __restrict__ fy, const double* __restrict__ scal,
