@@ -409,8 +409,14 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const double invN = 1.0 / ((double)nx * (double)ny);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int idx = lb * kBlock + threadIdx.x; idx < N; idx += I.nblk_stage * kBlock) {
-    const int y = idx / nx, x = idx - y * nx;
+  // a wavefront takes 64 columns of one row at a time, so the row's table
+  // entries are wave-uniform (scalar loads)
+  const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int nseg = (nx + 63) >> 6, npairs = ny * nseg;
+  for (int rs = lb * kWaves + wid; rs < npairs; rs += I.nblk_stage * kWaves) {
+    const int y = rs / nseg, x = (rs - y * nseg) * 64 + lane;
+    if (x >= nx) continue;
+    const int idx = y * nx + x;
     double ax, ay;
     fsynth_at(P, I.frows, y, x, ax, ay);
     const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);

@@ -190,28 +190,42 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
   block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * NR);
   if (DRAG) {
     // Mode (kx, ky), field f: sum over the wave's columns of
-    // exp(-i kx x) * sum_y f exp(-i ky y).  The lanes' column sums go to LDS;
-    // lane j < 2*ndrag then forms output (mode j/2, field j%2) over 64 columns.
+    // exp(-i kx x) * sum_y f exp(-i ky y).  The lanes' column sums and their
+    // columns' twiddles of |kx| = 1, 2 go to LDS (one load per lane); lane
+    // j < 2*ndrag then forms output (mode j/2, field j%2) over the 64 columns
+    // (columns a wave does not own have zero sums).  cos(-a) = cos(a),
+    // sin(-a) = -sin(a).
     __shared__ double cs_[kWaves][10][64];
+    __shared__ double tw_[kWaves][4][64];
     __shared__ double ms_[kWaves][48];
     const int wid = threadIdx.x >> 6;
 #pragma unroll
     for (int f = 0; f < 2; ++f)
 #pragma unroll
       for (int j = 0; j < 5; ++j) cs_[wid][f * 5 + j][lane] = W.own ? bs[f][j] : 0.0;
+    {
+      const int xc = W.own ? W.x : 0;  // any column for lanes with zero sums
+#pragma unroll
+      for (int k = 1; k <= 2; ++k) {
+        if (k <= P.dkx) {
+          const double* t = P.twx + ((size_t)(k + P.kmx) * nx + xc) * 2;
+          tw_[wid][2 * k - 2][lane] = t[0];
+          tw_[wid][2 * k - 1][lane] = t[1];
+        }
+      }
+    }
     __syncthreads();
     if (lane < 2 * P.ndrag) {
       const int m = lane >> 1, f = lane & 1;
       const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
-      const int ka = ky < 0 ? -ky : ky;
+      const int ka = ky < 0 ? -ky : ky, kxa = kx < 0 ? -kx : kx;
       const int sc = f * 5 + (ka == 0 ? 0 : 2 * ka - 1);
       double re = 0.0, im = 0.0;
-      const int x0 = W.x - lane;  // column of lane 0
+#pragma unroll 8
       for (int l = 0; l < 64; ++l) {
-        const int x = x0 + l;
-        if (x < 0 || x >= nx) continue;
-        const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
-        const double cx = tx[0], sx = tx[1];
+        const double cx = kxa == 0 ? 1.0 : tw_[wid][2 * kxa - 2][l];
+        const double sa = kxa == 0 ? 0.0 : tw_[wid][2 * kxa - 1][l];
+        const double sx = kx < 0 ? -sa : sa;
         const double br = cs_[wid][sc][l];
         const double bi = ka == 0 ? 0.0 : (ky > 0 ? -cs_[wid][sc + 1][l] : cs_[wid][sc + 1][l]);
         re += cx * br + sx * bi;
@@ -1785,10 +1799,11 @@ __device__ __forceinline__ void fproject_body(const Params<T>& P, const double* 
     const int ka = ky < 0 ? -ky : ky;
     double re = 0.0, im = 0.0;
     if (W.live) {
+      // every column in turn (lanes past the grid's edge hold zero sums, and
+      // read a clamped twiddle), so the unrolled loads are independent
 #pragma unroll 8
       for (int l = 0; l < 64; ++l) {
-        const int x = x0 + l;
-        if (x >= nx) break;
+        const int x = min(x0 + l, nx - 1);
         const double* tx = P.twx + ((size_t)(kx + P.kmx) * nx + x) * 2;
         const double cx = tx[0], sx = tx[1];
         const double br = ka == 0 ? L[l] : L[(2 * ka - 1) * 64 + l];
