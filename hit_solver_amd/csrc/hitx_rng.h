@@ -211,12 +211,22 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
 // (Hillis-Steele in LDS), and every run then knows its entry offset and the
 // index of its first value.  force_serial makes the last step regenerate
 // serially (a check of that path).
+// Entry e of a chunk's map held as one 16-byte load (8 uint16).
+static_assert(kRngEntries == 8, "a chunk's map is one 16-byte load");
+__device__ __forceinline__ uint32_t map_entry(const uint4& m, int e) {
+  const int h = e >> 1;
+  const uint32_t w = h == 0 ? m.x : (h == 1 ? m.y : (h == 2 ? m.z : m.w));
+  return (e & 1) ? (w >> 16) : (w & 0xffffu);
+}
+constexpr int kRngScanBatch = 8;  // chunk maps loaded together (one latency per batch)
+
 __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial) {
   __shared__ uint8_t sx[kRngScan * kRngEntries];
   __shared__ int sn[kRngScan * kRngEntries];
   const int t = threadIdx.x;
   const int per = (J.nch + kRngScan - 1) / kRngScan;
   const int c0 = min(t * per, J.nch), c1 = min(c0 + per, J.nch);
+  const uint4* maps = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(J.map));
   uint8_t mx[kRngEntries];
   int mn[kRngEntries];
 #pragma unroll
@@ -224,13 +234,22 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
     mx[e] = (uint8_t)e;
     mn[e] = 0;
   }
-  for (int c = c0; c < c1; ++c) {
+  for (int cb = c0; cb < c1; cb += kRngScanBatch) {
+    uint4 mv[kRngScanBatch];
 #pragma unroll
-    for (int e = 0; e < kRngEntries; ++e) {
-      if (mx[e] < kRngEntries) {
-        const uint16_t v = J.map[(size_t)c * kRngEntries + mx[e]];
-        mx[e] = (uint8_t)(v & 0xff);
-        mn[e] += v >> 8;
+    for (int j = 0; j < kRngScanBatch; ++j)
+      if (cb + j < c1) mv[j] = maps[cb + j];
+#pragma unroll
+    for (int j = 0; j < kRngScanBatch; ++j) {
+      if (cb + j < c1) {
+#pragma unroll
+        for (int e = 0; e < kRngEntries; ++e) {
+          if (mx[e] < kRngEntries) {
+            const uint32_t v = map_entry(mv[j], mx[e]);
+            mx[e] = (uint8_t)(v & 0xff);
+            mn[e] += (int)(v >> 8);
+          }
+        }
       }
     }
   }
@@ -272,12 +291,22 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
     e = sx[(t - 1) * kRngEntries];
     cnt = sn[(t - 1) * kRngEntries];
   }
-  for (int c = c0; c < c1 && e < kRngEntries; ++c) {
-    J.entry[c] = (uint8_t)e;
-    J.base[c] = cnt;
-    const uint16_t v = J.map[(size_t)c * kRngEntries + e];
-    e = v & 0xff;
-    cnt += v >> 8;
+  for (int cb = c0; cb < c1 && e < kRngEntries; cb += kRngScanBatch) {
+    uint4 mv[kRngScanBatch];
+#pragma unroll
+    for (int j = 0; j < kRngScanBatch; ++j)
+      if (cb + j < c1) mv[j] = maps[cb + j];
+#pragma unroll
+    for (int j = 0; j < kRngScanBatch; ++j) {
+      const int c = cb + j;
+      if (c < c1 && e < kRngEntries) {
+        J.entry[c] = (uint8_t)e;
+        J.base[c] = cnt;
+        const uint32_t v = map_entry(mv[j], e);
+        e = (int)(v & 0xff);
+        cnt += (int)(v >> 8);
+      }
+    }
   }
   if (t == kRngScan - 1) {
     const bool overflow = sx[t * kRngEntries] >= kRngEntries;
