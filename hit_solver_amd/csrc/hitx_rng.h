@@ -156,7 +156,7 @@ struct RngJob {
   GPtr<uint64_t> state;  // s hi, s lo, inc hi, inc lo; advanced past the n values
   GPtr<double> out;      // n values
   GPtr<double> vals;     // [kRngChunk][nch] attempt values (transposed: coalesced)
-  GPtr<uint8_t> att;     // [kRngChunk][nch] attempt length | yields << 7
+  GPtr<uint8_t> att;     // [nch][kRngChunk] attempt length | yields << 7
   GPtr<uint16_t> map;    // [nch][kRngEntries] spill | count << 8 (spill 0xff: overflow)
   GPtr<uint8_t> entry;   // [nch] resolved entry offset
   GPtr<int> base;        // [nch] index of the chunk's first value
@@ -190,8 +190,13 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
     const ZigAttempt a = zig_attempt<true>(w, g);
     const uint8_t code = (uint8_t)(a.len | (a.out ? 0x80 : 0));
     J.vals[(size_t)j * J.nch + c] = a.v;
-    J.att[(size_t)j * J.nch + c] = code;
     lds[j] = code;
+  }
+  {  // the chunk's codes, contiguous: two 16-byte stores
+    uint4* dst = reinterpret_cast<uint4*>(static_cast<uint8_t*>(J.att) + (size_t)c * kRngChunk);
+    const uint4* src = reinterpret_cast<const uint4*>(lds);
+    dst[0] = src[0];
+    dst[1] = src[1];
   }
   for (int e = 0; e < kRngEntries; ++e) {
     int p = e, cnt = 0;
@@ -205,12 +210,6 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
   }
 }
 
-// Step 3 (one block of kRngScan threads per request).  A chunk's map sends
-// the entry offset e to (spill, count); spill 0xff marks an overflow.  Each
-// thread composes the maps of its run of chunks, the block scans the run maps
-// (Hillis-Steele in LDS), and every run then knows its entry offset and the
-// index of its first value.  force_serial makes the last step regenerate
-// serially (a check of that path).
 // Entry e of a chunk's map held as one 16-byte load (8 uint16).
 static_assert(kRngEntries == 8, "a chunk's map is one 16-byte load");
 __device__ __forceinline__ uint32_t map_entry(const uint4& m, int e) {
@@ -219,6 +218,13 @@ __device__ __forceinline__ uint32_t map_entry(const uint4& m, int e) {
   return (e & 1) ? (w >> 16) : (w & 0xffffu);
 }
 constexpr int kRngScanBatch = 8;  // chunk maps loaded together (one latency per batch)
+
+// Step 3 (one block of kRngScan threads per request).  A chunk's map sends
+// the entry offset e to (spill, count); spill 0xff marks an overflow.  Each
+// thread composes the maps of its run of chunks, the block scans the run maps
+// (Hillis-Steele in LDS), and every run then knows its entry offset and the
+// index of its first value.  force_serial makes the last step regenerate
+// serially (a check of that path).
 
 __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial) {
   __shared__ uint8_t sx[kRngScan * kRngEntries];
@@ -328,17 +334,26 @@ __device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double*
   const long long lo = J.base[c0];
   long long k = lo;
   if (c < J.nch) {
-    int p = J.entry[c];
+    // the chunk's 32 codes in two 16-byte loads, then the attempt chain from
+    // the entry offset walked over them in registers (position q is an
+    // attempt start when the chain reaches it)
+    const uint4* ap = reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(J.att) + (size_t)c * kRngChunk);
+    const uint4 a0 = ap[0], a1 = ap[1];
+    const uint32_t cw[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    int nxt = J.entry[c];
     k = J.base[c];
-    while (p < kRngChunk) {
-      const uint8_t code = J.att[(size_t)p * J.nch + c];
-      const int len = code & 0x7f;
-      if (code & 0x80) {
-        if (k < J.n) lds[k - lo] = J.vals[(size_t)p * J.nch + c];
-        if (k == J.n - 1) J.stat[1] = c * kRngChunk + p + len;  // words consumed
-        ++k;
+#pragma unroll
+    for (int q = 0; q < kRngChunk; ++q) {
+      if (q == nxt) {
+        const uint32_t code = (cw[q >> 2] >> ((q & 3) * 8)) & 0xffu;
+        const int len = (int)(code & 0x7f);
+        if (code & 0x80) {
+          if (k < J.n) lds[k - lo] = J.vals[(size_t)q * J.nch + c];
+          if (k == J.n - 1) J.stat[1] = c * kRngChunk + q + len;  // words consumed
+          ++k;
+        }
+        nxt = q + len;
       }
-      p += len;
     }
   }
   long long hi = k;
@@ -376,7 +391,7 @@ constexpr int kRngAttemptThreads = 256;
 
 static_assert(kRngAttemptThreads <= 256, "jump table covers 256 threads");
 __global__ void __launch_bounds__(kRngAttemptThreads) k_rng_attempts(RngJob J) {
-  __shared__ uint8_t lds[kRngAttemptThreads * kRngChunk];
+  __shared__ __align__(16) uint8_t lds[kRngAttemptThreads * kRngChunk];
   rng_attempt_body(J, blockIdx.x * kRngAttemptThreads, lds + threadIdx.x * kRngChunk);
 }
 __global__ void __launch_bounds__(kRngScan) k_rng_scan(RngJob J, int force_serial) { rng_scan_body(J, force_serial); }
