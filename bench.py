@@ -45,7 +45,7 @@ HIT_STEP_BYTES = {"fp64": 635.0, "fp32": 320.0}
 # HBM bytes per line-sweep launch (both sweeps of a stage) measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (scaled by a 512 MiB copy kernel's readings; profiles/k_line_riemann_c3_512_fp64.md), for the workloads that
 # were profiled
-LINE_TRAFFIC = {(512, "fp64"): 34_535_872}
+LINE_TRAFFIC = {(512, "fp64"): 34_454_851}
 
 
 def progress(msg):
