@@ -1045,7 +1045,7 @@ struct hitx_batch {
         rngws, drows, frows, rawm;
   };
   std::vector<Buf> buf;
-  size_t off_tab = 0, off_offs = 0, ws_bytes = 0;
+  size_t off_tab = 0, off_offs = 0, ws_bytes = 0, offs_cap = 0;
   int total[OFF_KINDS] = {0};
   std::vector<int> offs;  // OFF_KINDS * (n + 1)
   size_t line_lds = 0, sensor_lds = 0, proj_lds = 0, wave_slice = 16;
@@ -1106,6 +1106,36 @@ void batch_offsets(hitx_batch* b) {
     b->offs[(size_t)k * (n + 1) + n] = acc;
     b->total[k] = acc;
   }
+  // block maps: for kind k, entry b = instance << kMapLbBits | local block
+  // (the map of kind k starts at the index held in slot OFF_KINDS*(n+1) + k)
+  const size_t hdr = (size_t)OFF_KINDS * (n + 1);
+  b->offs.resize(hdr + OFF_KINDS, 0);
+  for (int k = 0; k < OFF_KINDS; ++k) {
+    b->offs[hdr + k] = (int)b->offs.size();
+    for (int i = 0; i < n; ++i) {
+      const int nb = batch_blocks(b, b->plans[i], k);
+      for (int l = 0; l < nb; ++l) b->offs.push_back((int)(((uint32_t)i << kMapLbBits) | (uint32_t)l));
+    }
+  }
+}
+
+// Upper bound of the offsets buffer over every launch-shape choice (tile
+// heights at their minimum, both line forms counted).
+size_t batch_offs_capacity(const hitx_batch* b) {
+  const int n = b->n;
+  size_t cap = (size_t)OFF_KINDS * (n + 1) + OFF_KINDS;
+  for (const hitx_plan* pl : b->plans) {
+    const int nx = pl->d.nx, ny = pl->d.ny;
+    cap += (size_t)pl->nblk_stage + walk_blocks(nx, ny, kTuneRbMin, 1) + ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
+    cap += (size_t)ny + nx + (ny + kLineWaves - 1) / kLineWaves + (nx + kLineWaves - 1) / kLineWaves;
+    cap += (size_t)((nx + 15) / 16) * ((ny + 15) / 16);
+    if (pl->d.n_force_modes > 0)
+      cap += (size_t)walk_blocks(nx, ny, kTuneRbpMin, 0) + (ny * (pl->fkx + 1) + kBlock - 1) / kBlock +
+             2 * ((rng_chunks(pl->N) + kRngScatterThreads - 1) / kRngScatterThreads) +
+             (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads;
+    cap += (size_t)(2 * ny + kBlock - 1) / kBlock;
+  }
+  return cap;
 }
 
 template <class T>
@@ -1274,6 +1304,10 @@ extern "C" {
 int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hitx_batch** out) {
   if (!out || !descs || !kpar || n <= 0) return fail(HITX_EINVAL, "bad batch arguments");
   *out = nullptr;
+  if (n >= (1 << (32 - kMapLbBits))) return fail(HITX_EINVAL, "too many batch instances (at most 4095)");
+  for (int i = 0; i < n; ++i)  // the block map holds local block indices below 2^kMapLbBits
+    if ((long long)((descs[i].nx + 15) / 16) * ((descs[i].ny + 15) / 16) >= (1ll << kMapLbBits))
+      return fail(HITX_EINVAL, "batch instance " + std::to_string(i) + " too large (16x16 tiles must be < 2^20)");
   for (int i = 1; i < n; ++i) {
     const hitx_desc &a = descs[0], &c = descs[i];
     if (a.periodic != c.periodic || a.dtype != c.dtype || a.scheme != c.scheme || a.sensor_width != c.sensor_width ||
@@ -1346,9 +1380,11 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     b->buf.push_back(u);
   }
   batch_offsets(b);
+  b->offs_cap = batch_offs_capacity(b);
+  if (b->offs.size() > b->offs_cap) return fail(HITX_EINVAL, "block map larger than its bound");
   const size_t inst_bytes = b->esize == 8 ? sizeof(Inst<double>) : sizeof(Inst<float>);
   b->off_tab = o; o = align256(o + inst_bytes * n);
-  b->off_offs = o; o = align256(o + b->offs.size() * sizeof(int));
+  b->off_offs = o; o = align256(o + b->offs_cap * sizeof(int));
   b->ws_bytes = o;
   *out = b;
   return HITX_OK;
@@ -1405,6 +1441,7 @@ int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, i
     pl->nblk_proj = walk_blocks(pl->d.nx, pl->d.ny, pl->rbp, 0);
   }
   batch_offsets(b);
+  if (b->offs.size() > b->offs_cap) return fail(HITX_EINVAL, "block map larger than its bound");
   if (b->ws) return b->esize == 8 ? batch_bind_t<double>(b, b->ws, false) : batch_bind_t<float>(b, b->ws, false);
   return HITX_OK;
 }

@@ -49,17 +49,7 @@ enum {
   OFF_RNGS, OFF_KINDS
 };
 
-__device__ __forceinline__ int find_inst(const int* __restrict__ o, int n, int b) {
-  int lo = 0, hi = n;  // o[0] = 0 <= b < o[n]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (o[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
+constexpr int kMapLbBits = 20;  // local blocks per instance and kind < 2^20; instances < 2^12
 
 // (selected as global pointers, converted once)
 template <class T>
@@ -71,13 +61,15 @@ __device__ __forceinline__ GPtr<T> stage_out(const Inst<T>& I, int st) {
   return st == 0 ? I.Q1 : st == 1 ? I.Q2 : I.S;
 }
 
-#define HITX_BATCH_INST_AT(KIND, GB)                        \
-  const int* __restrict__ offs_ = offs + (KIND) * (ninst + 1); \
-  const int gb_ = (GB);                                     \
-  const int ii_ = find_inst(offs_, ninst, gb_);             \
-  const Inst<T>& I = tab[ii_];                              \
-  if (I.ctl[C_ACTIVE] == 0.0) return;                       \
-  const int lb = gb_ - offs_[ii_];
+// The block map of kind KIND (built on the host) gives a block's instance
+// and local block in one load: entry = instance << kMapLbBits | local block.
+#define HITX_BATCH_INST_AT(KIND, GB)                                                  \
+  const int gb_ = (GB);                                                               \
+  const uint32_t code_ = (uint32_t)offs[offs[OFF_KINDS * (ninst + 1) + (KIND)] + gb_]; \
+  const int ii_ = (int)(code_ >> kMapLbBits);                                         \
+  const Inst<T>& I = tab[ii_];                                                        \
+  if (I.ctl[C_ACTIVE] == 0.0) return;                                                 \
+  const int lb = (int)(code_ & ((1u << kMapLbBits) - 1u));
 #define HITX_BATCH_INST(KIND) HITX_BATCH_INST_AT(KIND, (int)blockIdx.x)
 // tile kernels: XCD-contiguous block order
 #define HITX_BATCH_TILE(KIND) HITX_BATCH_INST_AT(KIND, xcd_remap((int)blockIdx.x, (int)gridDim.x))
