@@ -341,22 +341,25 @@ bool use_split_sensor(const hitx_plan* pl) {
 template <class T>
 int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, T* qT, const WS& w, hipStream_t s) {
   const Params<T> P = make_params<T>(pl);
+  // the split sensor's raw-mask tiles (and the drag rows) finalize the
+  // reduction themselves; the fused sensor needs the separate finalize
+  const bool fold = use_split_sensor(pl);
+  const dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
+  const int nblk = P.drag_on ? pl->nblk_sr : (int)(g.x * g.y), stride = P.drag_on ? R_DRAG + kDragCols : R_DRAG;
   if (P.drag_on) {
     k_stage_reduce<T, true><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, pl->nblk_sr, w.scal);
-    k_drows<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.scal, w.drows);
+    if (fold) {
+      k_drows_fin<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.part, nblk, w.scal, w.drows);
+    } else {
+      k_stage_finalize<T, true><<<1, kBlock, 0, s>>>(P, w.part, nblk, w.scal);
+      k_drows<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.scal, w.drows);
+    }
   } else {
-    const dim3 gt((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
-    k_stage_tile<T><<<gt, kBlock, 0, s>>>(P, q, enforce_first, w.part);
-    // the split sensor's raw-mask tiles finalize the reduction themselves
-    if (!use_split_sensor(pl)) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, (int)(gt.x * gt.y), w.scal);
+    k_stage_tile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    if (!fold) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, nblk, w.scal);
   }
-  dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
-  if (use_split_sensor(pl)) {
-    if (P.drag_on)
-      k_rawtile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw);
-    else
-      k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, (int)(g.x * g.y), w.scal, w.raw);
+  if (fold) {
+    k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw);
     k_dilate<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.raw, mask_out, w.maskT, qT);
   } else {
     k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
@@ -1202,6 +1205,8 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const Inst<T>* tab = btab<T>(b);
   const int* offs = boffs(b);
   const int n = b->n;
+  // (batches keep the one-block-per-instance finalize: with tens of tile
+  // blocks per CU, a reduction in every tile block costs more than the launch)
   if (DRAG)
     b_stage_reduce<T, DRAG><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
   else

@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(kBlock) b_drows(const Inst<T>* __restrict__ ta
                                                   int ninst, int st) {
   HITX_BATCH_INST(OFF_DROWS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
-  drows_body<T>(I.P, I.scal, I.drows, lb * kBlock + threadIdx.x);
+  drows_body<T>(I.P, I.scal + S_DRAG, I.drows, lb * kBlock + threadIdx.x);
 }
 
 template <class T>

@@ -92,6 +92,7 @@ enum {
 
 // Stage reduction slots (block partials).
 enum { R_MAXR = 0, R_MAXP, R_MAXE, R_AX, R_AY, R_DIV2, R_VORT2, R_PSUM, R_DRAG };
+constexpr int kDragCols = 4 * 12;  // drag projection columns of a partial row (after R_DRAG)
 
 // ---------------------------------------------------------------- launch mapping
 // Workgroups of a 1-D grid are dispatched round-robin over the 8 XCDs (each
@@ -263,21 +264,28 @@ __device__ __forceinline__ void block_reduce_write(double (&v)[NR], double* out)
   }
 }
 
-// Reduce nblk partial rows of NR values (row-major) with one block; result in
-// out[0..NR).  Fixed order: deterministic run to run.
-template <int NR, uint64_t MAXMASK>
-__device__ __forceinline__ void reduce_partials(const double* part, int nblk, double* out) {
-  double v[NR];
+// Reduce columns col0 .. col0+NC-1 of nblk partial rows (row stride
+// `stride`) with one block; result in out[0..NC).  Each column is reduced on
+// its own in a fixed order, so any subset of a row's columns reduces to the
+// same bits as the whole row, run to run and block to block.
+template <int NC, uint64_t MAXMASK>
+__device__ __forceinline__ void reduce_partial_cols(const double* part, int nblk, int stride, int col0, double* out) {
+  double v[NC];
 #pragma unroll
-  for (int k = 0; k < NR; ++k) v[k] = 0.0;
+  for (int k = 0; k < NC; ++k) v[k] = 0.0;
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      double a = part[(size_t)b * NR + k];
+    for (int k = 0; k < NC; ++k) {
+      double a = part[(size_t)b * stride + col0 + k];
       v[k] = ((MAXMASK >> k) & 1) ? fmax(v[k], a) : v[k] + a;
     }
   }
-  block_reduce_write<NR, MAXMASK>(v, out);
+  block_reduce_write<NC, MAXMASK>(v, out);
+}
+// All NR values of the rows.
+template <int NR, uint64_t MAXMASK>
+__device__ __forceinline__ void reduce_partials(const double* part, int nblk, double* out) {
+  reduce_partial_cols<NR, MAXMASK>(part, nblk, NR, 0, out);
 }
 
 }  // namespace hitx

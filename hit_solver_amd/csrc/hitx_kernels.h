@@ -106,7 +106,7 @@ __device__ __forceinline__ double wave_reduce(double a) {
 template <class T, bool DRAG>
 __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* __restrict__ q,
                                                   int enforce_first, double* __restrict__ part, int bid) {
-  constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
+  constexpr int NR = DRAG ? (R_DRAG + kDragCols) : R_DRAG;
   double v[R_DRAG];
 #pragma unroll
   for (int k = 0; k < R_DRAG; ++k) v[k] = 0.0;
@@ -282,7 +282,7 @@ __device__ __forceinline__ void stage_scal_write(const Params<T>& P, const doubl
 template <class T, bool DRAG>
 __device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const double* __restrict__ part, int nblk,
                                                     double* __restrict__ scal) {
-  constexpr int NR = DRAG ? (R_DRAG + 4 * 12) : R_DRAG;
+  constexpr int NR = DRAG ? (R_DRAG + kDragCols) : R_DRAG;
   __shared__ double r[NR];
   reduce_partials<NR, 0x1Full>(part, nblk, r);
   __syncthreads();
@@ -295,15 +295,16 @@ __device__ __forceinline__ void stage_finalize_body(const Params<T>& P, const do
 // Re(G(1)+G(-1)), Im(G(1)-G(-1)), Re(G(2)+G(-2)), Im(G(2)-G(-2))}, so the
 // low-pass field at (x, y) is H0 + sum_k (Hc_k cos(k x) - Hs_k sin(k x)) (the
 // real part of the inverse transform, OOP/hit2d.py:366-367).
+// dc: the drag coefficients, [mode][field][re, im] (the stage scalars from S_DRAG).
 template <class T>
-__device__ __forceinline__ void drows_body(const Params<T>& P, const double* __restrict__ scal,
+__device__ __forceinline__ void drows_body(const Params<T>& P, const double* __restrict__ dc,
                                            double* __restrict__ drows, int t) {
   if (t >= 2 * P.ny) return;
   const int y = t >> 1, f = t & 1;
   double h[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int m = 0; m < P.ndrag; ++m) {
     const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
-    const double ar = scal[S_DRAG + 4 * m + 2 * f], ai = scal[S_DRAG + 4 * m + 2 * f + 1];
+    const double ar = dc[4 * m + 2 * f], ai = dc[4 * m + 2 * f + 1];
     const double* ty = P.twy + ((size_t)(ky + P.kmy) * P.ny + y) * 2;
     const double gr = ar * ty[0] - ai * ty[1], gi = ar * ty[1] + ai * ty[0];
     const double gs = kx > 0 ? gi : -gi;
@@ -324,7 +325,32 @@ __device__ __forceinline__ void drows_body(const Params<T>& P, const double* __r
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_drows(Params<T> P, const double* __restrict__ scal,
                                                   double* __restrict__ drows) {
-  drows_body<T>(P, scal, drows, blockIdx.x * kBlock + threadIdx.x);
+  drows_body<T>(P, scal + S_DRAG, drows, blockIdx.x * kBlock + threadIdx.x);
+}
+
+// The drag rows with the finalize's drag part folded in: every block reduces
+// the drag columns of the column-walk partial rows itself (the finalize's
+// per-column order), block 0 also writes the drag slots of the stage scalars.
+template <class T>
+__device__ __forceinline__ void drows_fin_body(const Params<T>& P, const double* __restrict__ part, int nblk,
+                                               double* __restrict__ scal, double* __restrict__ drows, int t,
+                                               bool first) {
+  __shared__ double r[kDragCols];
+  __shared__ double dc[kDragCols];
+  reduce_partial_cols<kDragCols, 0>(part, nblk, R_DRAG + kDragCols, R_DRAG, r);
+  __syncthreads();
+  const double N = (double)P.nx * (double)P.ny;
+  for (int k = threadIdx.x; k < 4 * P.ndrag; k += blockDim.x) {
+    dc[k] = r[k] / N;
+    if (first) scal[S_DRAG + k] = dc[k];
+  }
+  __syncthreads();
+  drows_body<T>(P, dc, drows, t);
+}
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_drows_fin(Params<T> P, const double* __restrict__ part, int nblk,
+                                                      double* __restrict__ scal, double* __restrict__ drows) {
+  drows_fin_body<T>(P, part, nblk, scal, drows, blockIdx.x * kBlock + threadIdx.x, blockIdx.x == 0);
 }
 
 template <class T, bool DRAG>
@@ -834,19 +860,29 @@ __global__ void __launch_bounds__(kBlock) k_rawtile(Params<T> P, const T* __rest
 // 256-thread reduction, so every block and a separate finalize agree to the
 // bit), keeps the stage scalars in LDS, and block 0 also writes them for the
 // later kernels of the stage.  Saves the one-block finalize launch.
+// (partial rows of `stride` values: R_DRAG from tiles, R_DRAG + the drag
+// columns from the drag column walk, whose drag part k_drows_fin reduces)
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __restrict__ q, int enforce_first,
-                                                        const double* __restrict__ part, int nblk,
-                                                        double* __restrict__ scal, uint8_t* __restrict__ raw) {
-  const Tile2 t_ = tile_xcd();
+__device__ __forceinline__ void rawtile_fin_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
+                                                 const double* __restrict__ part, int nblk, int stride,
+                                                 double* __restrict__ scal, uint8_t* __restrict__ raw, int bx,
+                                                 int by, bool first) {
   __shared__ double r[R_DRAG];
   __shared__ double sc[S_DRAG];
-  reduce_partials<R_DRAG, 0x1Full>(part, nblk, r);
+  reduce_partial_cols<R_DRAG, 0x1Full>(part, nblk, stride, 0, r);
   __syncthreads();
   stage_scal_write<T, false>(P, r, sc);
-  if (blockIdx.x == 0 && blockIdx.y == 0) stage_scal_write<T, false>(P, r, scal);
+  if (first) stage_scal_write<T, false>(P, r, scal);
   __syncthreads();
-  rawtile_body<T>(P, q, enforce_first, sc, raw, t_.bx, t_.by);
+  rawtile_body<T>(P, q, enforce_first, sc, raw, bx, by);
+}
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                        const double* __restrict__ part, int nblk, int stride,
+                                                        double* __restrict__ scal, uint8_t* __restrict__ raw) {
+  const Tile2 t_ = tile_xcd();
+  rawtile_fin_body<T>(P, q, enforce_first, part, nblk, stride, scal, raw, t_.bx, t_.by,
+                      blockIdx.x == 0 && blockIdx.y == 0);
 }
 
 template <class T>
