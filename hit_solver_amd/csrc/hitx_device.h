@@ -2,9 +2,10 @@
 //
 // Every floating-point expression below reproduces the operation order of the
 // reference NumPy code it cites, so that FP64 results agree bit-for-bit where
-// the algorithm is the same (pointwise algebra, stencils, WENO7) and to
-// rounding where it is not (line solves, reductions).  The library is built
-// with -ffp-contract=off: no multiply-add contraction.
+// the algorithm is the same (pointwise algebra, stencils, the sensor) and to
+// rounding where it is not (WENO7 weights with one division, line solves,
+// reductions).  The library is built with -ffp-contract=off: no multiply-add
+// contraction in FP64; FP32 fuses explicitly (madd).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -231,12 +232,25 @@ __device__ __forceinline__ T weno7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
                    madd(v5, madd(T(4642), v7, madd(T(-17246), v6, T(11003) * v5)),
                         v4 * madd(T(-1854), v7, madd(T(7042), v6, madd(T(-9402), v5, T(2107) * v4))))));
   T e0 = eps + b0, e1 = eps + b1, e2 = eps + b2, e3 = eps + b3;
-  T a0 = dv(T(1.0 / 35.0), e0 * e0);
-  T a1 = dv(T(12.0 / 35.0), e1 * e1);
-  T a2 = dv(T(18.0 / 35.0), e2 * e2);
-  T a3 = dv(T(4.0 / 35.0), e3 * e3);
-  T asum = ((a0 + a1) + a2) + a3;
-  return dv(madd(a3, q3, madd(a2, q2, madd(a1, q1, a0 * q0))), asum);
+  if constexpr (sizeof(T) == 8) {
+    // one division: the weights d_k / E_k (E_k = e_k^2) times E0 E1 E2 E3 are
+    // d_k * prod_{j != k} E_j, and the normalised sum is unchanged (within a
+    // few ulp of the reference's quotients; E_k >= 1e-20 and products of
+    // three stay in range for any flux below ~1e20)
+    const T E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2, E3 = e3 * e3;
+    const T E01 = E0 * E1, E23 = E2 * E3;
+    const T a0 = T(1.0 / 35.0) * (E1 * E23), a1 = T(12.0 / 35.0) * (E0 * E23);
+    const T a2 = T(18.0 / 35.0) * (E3 * E01), a3 = T(4.0 / 35.0) * (E2 * E01);
+    const T asum = ((a0 + a1) + a2) + a3;
+    return madd(a3, q3, madd(a2, q2, madd(a1, q1, a0 * q0))) / asum;
+  } else {
+    T a0 = dv(T(1.0 / 35.0), e0 * e0);
+    T a1 = dv(T(12.0 / 35.0), e1 * e1);
+    T a2 = dv(T(18.0 / 35.0), e2 * e2);
+    T a3 = dv(T(4.0 / 35.0), e3 * e3);
+    T asum = ((a0 + a1) + a2) + a3;
+    return dv(madd(a3, q3, madd(a2, q2, madd(a1, q1, a0 * q0))), asum);
+  }
 }
 
 // ---------------------------------------------------------------- reductions

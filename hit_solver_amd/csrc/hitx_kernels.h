@@ -1024,7 +1024,7 @@ __device__ __forceinline__ void solve_store_scan(const T* __restrict__ F, int S,
   } else {
     const T p0 = bnd[line * 3 + 1], pn = bnd[line * 3 + 2];
     const T det = T(1) - K.rnm1 * K.rnm1;
-    const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
+    const T idet = T(1) / det, a = (-p0 + pn * K.rnm1) * idet, bb = (-pn + p0 * K.rnm1) * idet;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (k < L) {
@@ -1194,7 +1194,7 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
       } else {
         const T p0 = bnd[line * 3 + 1], pn = bnd[line * 3 + 2];
         const T det = T(1) - K.rnm1 * K.rnm1;
-        const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
+        const T idet = T(1) / det, a = (-p0 + pn * K.rnm1) * idet, bb = (-pn + p0 * K.rnm1) * idet;
 #pragma unroll
         for (int k = 0; k < LM; ++k) {
           if (k < L) {

@@ -291,7 +291,7 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
     } else {
       const T p0 = rdlane(y[0], 0), pn = rdlane(y[L - 1], 63);
       const T det = T(1) - K.rnm1 * K.rnm1;
-      const T a = (-p0 + pn * K.rnm1) / det, bb = (-pn + p0 * K.rnm1) / det;
+      const T idet = T(1) / det, a = (-p0 + pn * K.rnm1) * idet, bb = (-pn + p0 * K.rnm1) * idet;
       T pl = ipow(rho, lane * L), pr = ipow(rho, n - 1 - lane * L);  // rho^i, rho^(n-1-i)
       const T irho = T(1) / rho;
 #pragma unroll
