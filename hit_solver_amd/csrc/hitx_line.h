@@ -121,13 +121,20 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
   const T c3 = T(-1.0 / 480.0), c2 = T(1.0 / 20.0 + (-1.0 / 480.0)),
           c1 = T((25.0 / 32.0 + 1.0 / 20.0) + (-1.0 / 480.0));
   const T aL = ipow(rho, L);
-  // per-lane powers of the scans (fixed for the line)
-  T apw[5];
+  // per-lane powers of the scans (fixed for the line), as products of the
+  // uniform squares aL^(2^k) selected by the exponent's bits
+  T apw[6];
   apw[0] = aL;
 #pragma unroll
-  for (int k = 1; k < 5; ++k) apw[k] = apw[k - 1] * apw[k - 1];
-  const T a_f = ipow(aL, (lane & 15) + 1), a_b = ipow(aL, 16 - (lane & 15));
-  const T a_l = PER ? ipow(aL, lane) : T(0), a_r = ipow(aL, 63 - lane);
+  for (int k = 1; k < 6; ++k) apw[k] = apw[k - 1] * apw[k - 1];
+  auto lpow = [&](int e) {  // aL^e, 0 <= e < 64
+    T r = T(1);
+#pragma unroll
+    for (int b = 0; b < 6; ++b) r = ((e >> b) & 1) ? r * apw[b] : r;
+    return r;
+  };
+  const T a_f = lpow((lane & 15) + 1), a_b = lpow(16 - (lane & 15));
+  const T a_l = PER ? lpow(lane) : T(0), a_r = lpow(63 - lane);
 #pragma unroll 1
   for (int c = 0; c < 4; ++c) {
     wave_sync();  // previous component's LDS reads are done

@@ -57,7 +57,7 @@ __device__ __forceinline__ double rdlane(double v, int l) {
 // Kogge-Stone within rows of 16 by DPP row shifts, then the rows joined
 // through their end values (readlane).
 template <class T>
-__device__ __forceinline__ T scan_fwd(T v, const T (&pw)[5], T pf, int lane) {
+__device__ __forceinline__ T scan_fwd(T v, const T* pw, T pf, int lane) {
   v = madd(pw[0], dpp<0x111>(v), v);
   v = madd(pw[1], dpp<0x112>(v), v);
   v = madd(pw[2], dpp<0x114>(v), v);
@@ -70,7 +70,7 @@ __device__ __forceinline__ T scan_fwd(T v, const T (&pw)[5], T pf, int lane) {
   return madd(pf, prev, v);
 }
 template <class T>
-__device__ __forceinline__ T scan_bwd(T v, const T (&pw)[5], T pb, int lane) {
+__device__ __forceinline__ T scan_bwd(T v, const T* pw, T pb, int lane) {
   v = madd(pw[0], dpp<0x101>(v), v);
   v = madd(pw[1], dpp<0x102>(v), v);
   v = madd(pw[2], dpp<0x104>(v), v);
