@@ -333,6 +333,12 @@ dim3 grid_update(const hitx_plan* pl) { return dim3((pl->d.nx + kUTX - 1) / kUTX
 // (2.1x); the split form forms the undilated mask on tiles with a one-node
 // halo (1.2x), then dilates.
 constexpr long long kSplitSensorMinNodes = 0;
+// The drag stage reduction runs on the 32x16 sensor tiles while a launch has
+// few of them (a tile's drag end phase is a serial loop, cheap when tiles are
+// spread thin over the CUs); launches of many tiles use the column walk,
+// whose end phase covers 62 columns of rb rows at once.
+constexpr int kStageTileMaxTiles = 4096;
+bool stage_walk_form(long long tiles) { return tiles > kStageTileMaxTiles; }
 bool use_split_sensor(const hitx_plan* pl) {
   if (pl->sensor_form != 0) return pl->sensor_form == 2;
   return (long long)pl->N >= kSplitSensorMinNodes;
@@ -345,9 +351,13 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
   // reduction themselves; the fused sensor needs the separate finalize
   const bool fold = use_split_sensor(pl);
   const dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
-  const int nblk = P.drag_on ? pl->nblk_sr : (int)(g.x * g.y), stride = P.drag_on ? R_DRAG + kDragCols : R_DRAG;
+  const bool walk = P.drag_on && stage_walk_form((long long)g.x * g.y);
+  const int nblk = walk ? pl->nblk_sr : (int)(g.x * g.y), stride = P.drag_on ? R_DRAG + kDragCols : R_DRAG;
   if (P.drag_on) {
-    k_stage_reduce<T, true><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    if (walk)
+      k_stage_reduce<T, true><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    else
+      k_stage_tile<T, true><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part);
     if (fold) {
       k_drows_fin<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.part, nblk, w.scal, w.drows);
     } else {
@@ -355,7 +365,7 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
       k_drows<T><<<(2 * pl->d.ny + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, w.scal, w.drows);
     }
   } else {
-    k_stage_tile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part);
+    k_stage_tile<T, false><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part);
     if (!fold) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, nblk, w.scal);
   }
   if (fold) {
@@ -614,7 +624,7 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   pl->off_dfyT = o; o = align256(o + 4 * N * E);
   const size_t ntiles = (size_t)((desc->nx + kSTX - 1) / kSTX) * ((desc->ny + kSTY - 1) / kSTY);
   const size_t part = std::max({(size_t)pl->nblk_red * 64, (size_t)pl->nblk_sr * 64,
-                                (size_t)pl->nblk_proj * 2 * std::max(1, desc->n_force_modes), ntiles * R_DRAG});
+                                (size_t)pl->nblk_proj * 2 * std::max(1, desc->n_force_modes), ntiles * (R_DRAG + kDragCols)});
   pl->off_part = o; o = align256(o + part * sizeof(double));
   pl->off_scal = o; o = align256(o + S_COUNT * sizeof(double));
   pl->off_misc = o; o = align256(o + 64 * sizeof(double));
@@ -1041,6 +1051,7 @@ int hitx_timing_read(hitx_plan* pl, double* total_ms, int32_t* launches) {
 // ============================================================== batches
 struct hitx_batch {
   int n = 0, esize = 8, periodic = 1, drag = 0, device = 0;
+  bool stage_walk = false;  // drag stage reductions by column walk (many tiles) instead of on tiles
   std::vector<hitx_plan*> plans;
   std::vector<double> kpar;  // n * K_COUNT
   struct Buf {
@@ -1109,6 +1120,7 @@ void batch_offsets(hitx_batch* b) {
     b->offs[(size_t)k * (n + 1) + n] = acc;
     b->total[k] = acc;
   }
+  b->stage_walk = stage_walk_form(b->total[OFF_SENS]);
   // block maps: for kind k, entry b = instance << kMapLbBits | local block
   // (the map of kind k starts at the index held in slot OFF_KINDS*(n+1) + k)
   const size_t hdr = (size_t)OFF_KINDS * (n + 1);
@@ -1207,11 +1219,11 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const int n = b->n;
   // (batches keep the one-block-per-instance finalize: with tens of tile
   // blocks per CU, a reduction in every tile block costs more than the launch)
-  if (DRAG)
-    b_stage_reduce<T, DRAG><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
+  if (DRAG && b->stage_walk)
+    b_stage_reduce<T, true><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
   else
-    b_stage_tile<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
-  b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st);
+    b_stage_tile<T, DRAG><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+  b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st, DRAG && b->stage_walk ? 1 : 0);
   if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
   if (b->split_sensor) {
     if (b->raw_tile)
@@ -1368,7 +1380,7 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     const int sr_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbMin, 1);
     const int pj_max = walk_blocks(pl->d.nx, pl->d.ny, kTuneRbpMin, 0);
     const size_t ntile = (size_t)((pl->d.nx + kSTX - 1) / kSTX) * ((pl->d.ny + kSTY - 1) / kSTY);
-    u.part = take(std::max((size_t)std::max({pl->nblk_stage, pl->nblk_sr, sr_max}) * 64, ntile * R_DRAG) *
+    u.part = take(std::max((size_t)std::max({pl->nblk_stage, pl->nblk_sr, sr_max}) * 64, ntile * (R_DRAG + kDragCols)) *
                   sizeof(double));
     u.scal = take(S_COUNT * sizeof(double));
     u.ctl = take(C_COUNT * sizeof(double));

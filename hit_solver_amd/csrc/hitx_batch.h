@@ -85,24 +85,24 @@ __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restri
   stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb);
 }
 
-// Without the drag projection the stage reduction runs on the sensor's
-// 32x16 tiles, as in single plans (the same partial rows, so a batch
-// instance and its single plan reduce in the same order).
-template <class T>
+// The stage reduction (with the drag projection when DRAG) runs on the
+// sensor's 32x16 tiles, as in single plans (the same partial rows, so a
+// batch instance and its single plan reduce in the same order).
+template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_tile(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                        int ninst, int st) {
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
-  stage_tile_body<T>(I.P, stage_in(I, st), st < 3, I.part, bx, by, lb);
+  stage_tile_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, bx, by, lb);
 }
 
 template <class T, bool DRAG>
-__global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __restrict__ tab, int st) {
+__global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __restrict__ tab, int st, int walk) {
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || (st == 3 && I.ctl[C_HVNOW] == 0.0)) return;
-  stage_finalize_body<T, DRAG>(I.P, I.part, DRAG ? I.nblk_sr : I.ntile, I.scal);
+  stage_finalize_body<T, DRAG>(I.P, I.part, walk ? I.nblk_sr : I.ntile, I.scal);
 }
 
 template <class T>

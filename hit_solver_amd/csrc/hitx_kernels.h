@@ -704,9 +704,90 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
 // per-node quantities as the column walk (maxima of |rho|, |p|, |e|,
 // |u|+c, |v|+c; sums of div^2, vort^2 and, for cooling, the pressure of the
 // state as given), one partial row per tile.
+// Drag projection of one 32x16 tile (OOP/hit2d.py:391-394, q as given):
+// per column the sums of rho u and rho v times 1, cos/sin(ky y) (ky = 1, 2)
+// over the tile's rows (threads of a column: lanes c and c+32 of each wave,
+// then the four waves through LDS), then lane j < 2*ndrag of wave 0 forms
+// mode j/2, field j%2 over the 32 columns with the columns' twiddles of
+// |kx| = 1, 2 staged in LDS (cos(-a) = cos(a), sin(-a) = -sin(a)), as the
+// column walk's end phase does.
 template <class T>
+__device__ __forceinline__ void drag_tile_project(const Params<T>& P, const T* __restrict__ q,
+                                                  double* __restrict__ out, int x0, int y0) {
+  __shared__ double cw[kWaves][10][32];
+  __shared__ double cs[10][32];
+  __shared__ double tw[4][32];
+  const int nx = P.nx, ny = P.ny, N = nx * ny;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double bs[10];
+#pragma unroll
+  for (int j = 0; j < 10; ++j) bs[j] = 0.0;
+#pragma unroll
+  for (int s = 0; s < kSTX * kSTY / kBlock; ++s) {
+    const int t = threadIdx.x + s * kBlock, r = t / kSTX, c = t - r * kSTX;
+    const int gy = y0 + r, gx = x0 + c;
+    if (gx < nx && gy < ny) {
+      const double fb = (double)at32(q + (size_t)N, gy * nx + gx), fc = (double)at32(q + 2 * (size_t)N, gy * nx + gx);
+      bs[0] += fb;
+      bs[5] += fc;
+#pragma unroll
+      for (int k = 1; k <= 2; ++k) {
+        if (k <= P.dky) {
+          const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + gy) * 2;
+          const double cy = ty[0], sy = ty[1];
+          bs[2 * k - 1] += fb * cy;
+          bs[2 * k] += fb * sy;
+          bs[5 + 2 * k - 1] += fc * cy;
+          bs[5 + 2 * k] += fc * sy;
+        }
+      }
+    }
+  }
+  // lanes c and c + 32 hold the same column
+#pragma unroll
+  for (int j = 0; j < 10; ++j) {
+    const double o = __shfl_xor(bs[j], 32, 64);
+    if (lane < 32) cw[wid][j][lane] = bs[j] + o;
+  }
+  if (threadIdx.x < 4 * 32) {  // the columns' twiddles of |kx| = 1, 2 (any column past the grid)
+    const int k = threadIdx.x >> 6, e = (threadIdx.x >> 5) & 1, c = threadIdx.x & 31;
+    const int xc = min(x0 + c, nx - 1);
+    if (k + 1 <= P.dkx) tw[2 * k + e][c] = P.twx[((size_t)(k + 1 + P.kmx) * nx + xc) * 2 + e];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < 10 * 32; j += kBlock) {
+    const int f = j >> 5, c = j & 31;
+    double a = cw[0][f][c];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) a += cw[w][f][c];
+    cs[f][c] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * P.ndrag) {
+    const int m = threadIdx.x >> 1, f = threadIdx.x & 1;
+    const int kx = P.drag_modes[2 * m], ky = P.drag_modes[2 * m + 1];
+    const int ka = ky < 0 ? -ky : ky, kxa = kx < 0 ? -kx : kx;
+    const int sc = f * 5 + (ka == 0 ? 0 : 2 * ka - 1);
+    double re = 0.0, im = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < kSTX; ++c) {  // columns past the grid have zero sums
+      const double cx = kxa == 0 ? 1.0 : tw[2 * kxa - 2][c];
+      const double sa = kxa == 0 ? 0.0 : tw[2 * kxa - 1][c];
+      const double sx = kx < 0 ? -sa : sa;
+      const double br = cs[sc][c];
+      const double bi = ka == 0 ? 0.0 : (ky > 0 ? -cs[sc + 1][c] : cs[sc + 1][c]);
+      re += cx * br + sx * bi;
+      im += cx * bi - sx * br;
+    }
+    out[4 * m + 2 * f] = re;
+    out[4 * m + 2 * f + 1] = im;
+  }
+}
+
+template <class T, bool DRAG>
 __device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
                                                 double* __restrict__ part, int bx, int by, int bid) {
+  constexpr int NR = DRAG ? R_DRAG + kDragCols : R_DRAG;  // partial row: the scalars, then the drag projection
   constexpr int EX = kSTX + 2, EY = kSTY + 2;
   __shared__ T Lu[EX * EY], Lv[EX * EY];
   double v[R_DRAG];
@@ -762,7 +843,8 @@ __device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __r
       v[R_VORT2] += (double)w * (double)w;
     }
   }
-  block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * R_DRAG);
+  block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * NR);
+  if (DRAG) drag_tile_project(P, q, part + (size_t)bid * NR + R_DRAG, x0, y0);
 }
 
 // 2. 32x16 tile: the (2w+1)^2 OR dilation of raw (out-of-domain nodes of the
@@ -841,11 +923,11 @@ __global__ void __launch_bounds__(kBlock) k_rawmask(Params<T> P, const T* __rest
   rawmask_body<T>(P, q, enforce_first, scal, raw, blockIdx.x);
 }
 
-template <class T>
+template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                        double* __restrict__ part) {
   const Tile2 t_ = tile_xcd();
-  stage_tile_body<T>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
+  stage_tile_body<T, DRAG>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
 }
 
 template <class T>
