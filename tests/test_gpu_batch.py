@@ -185,3 +185,24 @@ def test_batch_graph_replay_matches_eager():
             for i in range(len(cfgs)):
                 np.testing.assert_array_equal(host(b.q[i]), host(a.q[i]))
             np.testing.assert_array_equal(b.control(), a.control())
+
+
+def test_hit_batch_walk_and_tile_drag_reductions_agree():
+    """A batch above 4096 sensor tiles reduces the drag stage by column walk;
+    a batch of one instance reduces it on tiles.  Only the summation order of
+    the stage sums differs: the same instance agrees to rounding."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+
+    print("[batch] walk vs tile drag reductions, 33 x 256^2 FP64", flush=True)
+    cfgs = [HIT2DConfig.production_512(nx=256, ny=256, forcing_seed=s, target_mach=0.25, tfinal=1e9, max_steps=3,
+                                       dtype="fp64") for s in range(33)]
+    big = BatchRunner(cfgs)  # 33 * 8 * 16 = 4224 tiles: column walk
+    big.tune(16, 16, 2, 3)
+    big.run()
+    for i in (0, 32):
+        one = BatchRunner([cfgs[i]])  # 128 tiles: tile form
+        one.tune(16, 16, 2, 3)
+        one.run()
+        for k in range(4):
+            assert rel_l2(host(big.q[i])[k], host(one.q[0])[k]) <= 1e-12, (i, k)
