@@ -930,13 +930,6 @@ __global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __r
   stage_tile_body<T, DRAG>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
 }
 
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_rawtile(Params<T> P, const T* __restrict__ q, int enforce_first,
-                                                    const double* __restrict__ scal, uint8_t* __restrict__ raw) {
-  const Tile2 t_ = tile_xcd();
-  rawtile_body<T>(P, q, enforce_first, scal, raw, t_.bx, t_.by);
-}
-
 // The raw mask with the stage finalize folded in (no drag projection): every
 // block reduces the tile partials itself, in the finalize's order (the same
 // 256-thread reduction, so every block and a separate finalize agree to the
