@@ -930,13 +930,13 @@ __global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __r
   stage_tile_body<T, DRAG>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
 }
 
-// The raw mask with the stage finalize folded in (no drag projection): every
-// block reduces the tile partials itself, in the finalize's order (the same
-// 256-thread reduction, so every block and a separate finalize agree to the
-// bit), keeps the stage scalars in LDS, and block 0 also writes them for the
-// later kernels of the stage.  Saves the one-block finalize launch.
-// (partial rows of `stride` values: R_DRAG from tiles, R_DRAG + the drag
-// columns from the drag column walk, whose drag part k_drows_fin reduces)
+// The raw mask with the stage finalize folded in (single plans): every block
+// reduces the scalar columns of the stage partial rows itself, in the
+// finalize's order (the same 256-thread reduction, so every block and a
+// separate finalize agree to the bit), keeps the stage scalars in LDS, and
+// block 0 also writes them for the later kernels of the stage.  Saves the
+// one-block finalize launch.  Partial rows hold `stride` values: R_DRAG, plus
+// the drag columns with drag (their part is reduced by k_drows_fin).
 template <class T>
 __device__ __forceinline__ void rawtile_fin_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
                                                  const double* __restrict__ part, int nblk, int stride,
