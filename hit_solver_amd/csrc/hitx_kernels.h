@@ -1299,8 +1299,9 @@ __device__ __forceinline__ void solve_store(const T* __restrict__ F, int S, T h,
 // LDS holds two 4 x (n+1) planes, the state and the point flux; the WENO7
 // values of one or two components at a time are staged in the compact
 // solve's scratch and the hybrid fluxes overwrite those components' state
-// planes, so four line blocks fit a CU.  The WENO work list is built in interface order (wave
-// ballots), so neighbouring lanes read neighbouring LDS words.
+// planes (a 512-node line: 42 KiB for double, three blocks per CU; 21 KiB for
+// float).  The WENO work list is built in interface order (wave ballots), so
+// neighbouring lanes read neighbouring LDS words.
 constexpr int kLineBlock = 512;
 template <class T, bool PER, bool SMALL>
 __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K, int axis, int enforced_input,
@@ -1393,9 +1394,9 @@ __device__ __forceinline__ void line_body(const Params<T>& P, const LineK<T>& K,
   }
   // 2b/3. CPP components per pass: WENO7 of the Lax-Friedrichs split at the
   //       listed interfaces into W (the solve's scratch, free until step 4;
-  //       two planes for float in the one-chunk shape, one otherwise, which
-  //       keeps a double line block within a quarter of the CU's LDS: four
-  //       blocks per CU), then the hybrid
+  //       two planes in the one-chunk shape, one otherwise; one plane for a
+  //       512-node double line would fit four blocks per CU, but the two
+  //       extra passes cost more than the occupancy gains), then the hybrid
   //       interface flux -- compactified WENO where flagged, smooth compact
   //       flux elsewhere -- over those components' state planes.  Each WENO
   //       item runs on a lane pair: the even lane the f+ stencil, the odd
