@@ -633,35 +633,97 @@ __device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __rest
   }
 }
 
-// 1'. Tile form of step 1 (single plans): a 32x16 tile with a one-node halo,
-//     every node's primitives loaded and formed once into LDS in one round,
-//     then the undilated mask of the tile's nodes into raw (the same per-node
-//     arithmetic as the column walk and the fused tile).
-template <class T>
+// 1'. Tile form of step 1: a 32x16 tile with a one-node halo, every node's
+//     state loaded at once (all of a thread's loads in flight together) and
+//     its primitives formed once into LDS, then the undilated mask of the
+//     tile's nodes into raw (the same per-node arithmetic as the column walk
+//     and the fused tile).  FIN (single plans): the stage finalize folded in:
+//     the block's first partial rows are loaded with the tile, and the block
+//     reduces the scalar columns of the nblk partial rows (row stride
+//     `stride`) in the finalize's order (the same 256-thread reduction, so
+//     every block and a separate finalize agree to the bit) into the stage
+//     scalars it then uses; block `first` also writes them to scal_out for
+//     the later kernels of the stage.
+template <class T, bool FIN = false>
 __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
                                              const double* __restrict__ scal, uint8_t* __restrict__ raw, int bx,
-                                             int by) {
-  constexpr int EX = kSTX + 2, EY = kSTY + 2;
+                                             int by, const double* __restrict__ part = nullptr, int nblk = 0,
+                                             int stride = 0, double* __restrict__ scal_out = nullptr,
+                                             bool first = false) {
+  constexpr int EX = kSTX + 2, EY = kSTY + 2, NE = EX * EY, NIT = (NE + kBlock - 1) / kBlock;
+  constexpr int PF = 2;  // partial rows per thread loaded with the tile
   __shared__ T Lr[EX * EY], Lu[EX * EY], Lv[EX * EY], Lp[EX * EY], Le[EX * EY];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x0 = bx * kSTX, y0 = by * kSTY;
   const bool per = P.periodic;
-  for (int t = threadIdx.x; t < EX * EY; t += kBlock) {
-    const int r = t / EX, c = t - r * EX;
-    int gy = y0 - 1 + r, gx = x0 - 1 + c;
-    if (per) {
-      gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
-      gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
-    } else {
-      gy = clampi(gy, 0, ny - 1);
-      gx = clampi(gx, 0, nx - 1);
+  double pf[PF][R_DRAG];
+  if (FIN) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      const int b = threadIdx.x + j * kBlock;
+      if (b < nblk) {
+#pragma unroll
+        for (int k = 0; k < R_DRAG; ++k) pf[j][k] = part[(size_t)b * stride + k];
+      }
     }
-    const Prim<T> pr = prim_at(P, q, N, gy * nx + gx, enforce_first);
-    Lr[t] = pr.rho;
-    Lu[t] = pr.u;
-    Lv[t] = pr.v;
-    Lp[t] = pr.p;
-    Le[t] = dv(pr.p, pr.rho * P.gm1);
+  }
+  T qa[NIT], qb[NIT], qc[NIT], qd[NIT];
+#pragma unroll
+  for (int s = 0; s < NIT; ++s) {
+    const int t = threadIdx.x + s * kBlock;
+    if (t < NE) {
+      const int r = t / EX, c = t - r * EX;
+      int gy = y0 - 1 + r, gx = x0 - 1 + c;
+      if (per) {
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+      } else {
+        gy = clampi(gy, 0, ny - 1);
+        gx = clampi(gx, 0, nx - 1);
+      }
+      load4(q, N, gy * nx + gx, qa[s], qb[s], qc[s], qd[s]);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NIT; ++s) {
+    const int t = threadIdx.x + s * kBlock;
+    if (t < NE) {
+      T a = qa[s], b = qb[s], c = qc[s], d = qd[s];
+      if (enforce_first) enforce(P, a, b, c, d);
+      const Prim<T> pr = prim(P, a, b, c, d);
+      Lr[t] = pr.rho;
+      Lu[t] = pr.u;
+      Lv[t] = pr.v;
+      Lp[t] = pr.p;
+      Le[t] = dv(pr.p, pr.rho * P.gm1);
+    }
+  }
+  if (FIN) {
+    __shared__ double rr[R_DRAG];
+    __shared__ double sc[S_DRAG];
+    constexpr uint64_t MM = 0x1Full;  // R_MAXR .. R_AY are maxima
+    double v[R_DRAG];
+#pragma unroll
+    for (int k = 0; k < R_DRAG; ++k) v[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < PF; ++j) {
+      if (threadIdx.x + j * kBlock < nblk) {
+#pragma unroll
+        for (int k = 0; k < R_DRAG; ++k) v[k] = ((MM >> k) & 1) ? fmax(v[k], pf[j][k]) : v[k] + pf[j][k];
+      }
+    }
+    for (int b = threadIdx.x + PF * kBlock; b < nblk; b += kBlock) {
+#pragma unroll
+      for (int k = 0; k < R_DRAG; ++k) {
+        const double a = part[(size_t)b * stride + k];
+        v[k] = ((MM >> k) & 1) ? fmax(v[k], a) : v[k] + a;
+      }
+    }
+    block_reduce_write<R_DRAG, MM>(v, rr);
+    __syncthreads();
+    stage_scal_write<T, false>(P, rr, sc);
+    if (first) stage_scal_write<T, false>(P, rr, scal_out);
+    scal = sc;
   }
   __syncthreads();
   const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
@@ -930,34 +992,13 @@ __global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __r
   stage_tile_body<T, DRAG>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
 }
 
-// The raw mask with the stage finalize folded in (single plans): every block
-// reduces the scalar columns of the stage partial rows itself, in the
-// finalize's order (the same 256-thread reduction, so every block and a
-// separate finalize agree to the bit), keeps the stage scalars in LDS, and
-// block 0 also writes them for the later kernels of the stage.  Saves the
-// one-block finalize launch.  Partial rows hold `stride` values: R_DRAG, plus
-// the drag columns with drag (their part is reduced by k_drows_fin).
-template <class T>
-__device__ __forceinline__ void rawtile_fin_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
-                                                 const double* __restrict__ part, int nblk, int stride,
-                                                 double* __restrict__ scal, uint8_t* __restrict__ raw, int bx,
-                                                 int by, bool first) {
-  __shared__ double r[R_DRAG];
-  __shared__ double sc[S_DRAG];
-  reduce_partial_cols<R_DRAG, 0x1Full>(part, nblk, stride, 0, r);
-  __syncthreads();
-  stage_scal_write<T, false>(P, r, sc);
-  if (first) stage_scal_write<T, false>(P, r, scal);
-  __syncthreads();
-  rawtile_body<T>(P, q, enforce_first, sc, raw, bx, by);
-}
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                         const double* __restrict__ part, int nblk, int stride,
                                                         double* __restrict__ scal, uint8_t* __restrict__ raw) {
   const Tile2 t_ = tile_xcd();
-  rawtile_fin_body<T>(P, q, enforce_first, part, nblk, stride, scal, raw, t_.bx, t_.by,
-                      blockIdx.x == 0 && blockIdx.y == 0);
+  rawtile_body<T, true>(P, q, enforce_first, nullptr, raw, t_.bx, t_.by, part, nblk, stride, scal,
+                        blockIdx.x == 0 && blockIdx.y == 0);
 }
 
 template <class T>
