@@ -1122,9 +1122,11 @@ void batch_offsets(hitx_batch* b) {
   }
   b->stage_walk = stage_walk_form(b->total[OFF_SENS]);
   // block maps: for kind k, entry b = instance << kMapLbBits | local block
-  // (the map of kind k starts at the index held in slot OFF_KINDS*(n+1) + k)
+  // (the map of kind k starts at the index held in slot OFF_KINDS*(n+1) + k);
+  // slot OFF_KINDS*(n+1) + OFF_KINDS is the batch-wide hyperviscosity word
+  // (0 between steps)
   const size_t hdr = (size_t)OFF_KINDS * (n + 1);
-  b->offs.resize(hdr + OFF_KINDS, 0);
+  b->offs.resize(hdr + OFF_KINDS + 1, 0);
   for (int k = 0; k < OFF_KINDS; ++k) {
     b->offs[hdr + k] = (int)b->offs.size();
     for (int i = 0; i < n; ++i) {
@@ -1138,7 +1140,7 @@ void batch_offsets(hitx_batch* b) {
 // heights at their minimum, both line forms counted).
 size_t batch_offs_capacity(const hitx_batch* b) {
   const int n = b->n;
-  size_t cap = (size_t)OFF_KINDS * (n + 1) + OFF_KINDS;
+  size_t cap = (size_t)OFF_KINDS * (n + 1) + OFF_KINDS + 1;
   for (const hitx_plan* pl : b->plans) {
     const int nx = pl->d.nx, ny = pl->d.ny;
     cap += (size_t)pl->nblk_stage + walk_blocks(nx, ny, kTuneRbMin, 1) + ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
@@ -1273,7 +1275,9 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
   const int* offs = boffs(b);
   const int n = b->n;
   b_step_reduce<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
-  b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab);
+  // the batch-wide hyperviscosity word (slot after the block-map starts)
+  int* hvany = const_cast<int*>(offs) + (size_t)OFF_KINDS * (n + 1) + OFF_KINDS;
+  b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab, hvany);
   if (b->periodic && b->total[OFF_PROJ] > 0) {
     b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
     b_rng_scan<T><<<n, kRngScan, 0, s>>>(tab);
@@ -1290,8 +1294,10 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
     int rc = b->drag ? batch_stage_sequence<T, true>(b, st, s) : batch_stage_sequence<T, false>(b, st, s);
     if (rc) return rc;
   }
-  b_finish<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n);
-  b_ctl_end<T><<<(n + 63) / 64, 64, 0, s>>>(tab, n);
+  int guard_any = 0;  // outflow guards copy S -> Q after every step
+  for (const hitx_plan* pl : b->plans) guard_any |= (!pl->d.periodic && pl->d.guard_cells > 0) ? 1 : 0;
+  b_finish<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n, guard_any);
+  b_ctl_end<T><<<(n + 63) / 64, 64, 0, s>>>(tab, n, hvany);
   LAUNCHCHK();
   return HITX_OK;
 }

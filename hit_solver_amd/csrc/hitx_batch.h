@@ -74,12 +74,24 @@ __device__ __forceinline__ GPtr<T> stage_out(const Inst<T>& I, int st) {
 // tile kernels: XCD-contiguous block order
 #define HITX_BATCH_TILE(KIND) HITX_BATCH_INST_AT(KIND, xcd_remap((int)blockIdx.x, (int)gridDim.x))
 
+// Batch-wide word after the block-map starts of the offsets buffer: nonzero
+// when some instance runs hyperviscosity this step (set by b_ctl_begin,
+// cleared by b_ctl_end).  The post-step launches read it first, so on the
+// steps without hyperviscosity every block exits after one load instead of
+// the block map, instance table and control loads.
+__device__ __forceinline__ int hv_any(const int* offs, int ninst) {
+  return offs[OFF_KINDS * (ninst + 1) + OFF_KINDS];
+}
+#define HITX_POST_STEP_EXIT \
+  if (st == 3 && hv_any(offs, ninst) == 0) return;
+
 // ---------------------------------------------------------------- stage path
 // st = 0, 1, 2: RK stages on Q, Q1, Q2 (enforced input, transposed copy);
 // st = 3: the post-step sensor on S for hyperviscosity (as given, no copy).
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                          int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_INST(OFF_SR)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb);
@@ -91,6 +103,7 @@ __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restri
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_tile(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                        int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
@@ -108,6 +121,7 @@ __global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __rest
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_drows(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                   int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_INST(OFF_DROWS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   drows_body<T>(I.P, I.scal + S_DRAG, I.drows, lb * kBlock + threadIdx.x);
@@ -116,6 +130,7 @@ __global__ void __launch_bounds__(kBlock) b_drows(const Inst<T>* __restrict__ ta
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_sensor(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
@@ -127,6 +142,7 @@ __global__ void __launch_bounds__(kBlock) b_sensor(const Inst<T>* __restrict__ t
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_rawmask(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                     int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_INST(OFF_SR)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   rawmask_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, lb);
@@ -135,6 +151,7 @@ __global__ void __launch_bounds__(kBlock) b_rawmask(const Inst<T>* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_rawtile(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                     int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
@@ -145,6 +162,7 @@ __global__ void __launch_bounds__(kBlock) b_rawtile(const Inst<T>* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_dilate(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst, int st) {
+  HITX_POST_STEP_EXIT
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
@@ -199,7 +217,8 @@ __global__ void __launch_bounds__(kUBlock) b_update(const Inst<T>* __restrict__ 
 // copy when the last stage could not write Q in place.
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_finish(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
-                                                   int ninst) {
+                                                   int ninst, int guard_any) {
+  if (!guard_any && hv_any(offs, ninst) == 0) return;  // nothing to do in any instance
   HITX_BATCH_TILE(OFF_T16)
   const int ntx = (I.P.nx + 15) / 16;
   const int by = lb / ntx, bx = lb - by * ntx;
@@ -231,7 +250,7 @@ __device__ __forceinline__ bool is_none(double v) { return v != v; }
 // Mach-controlled target power (OOP/hit2d.py:293-349), the OU decay and
 // increment (OOP/forcing.py:75-76), and whether hyperviscosity runs this step.
 template <class T>
-__global__ void __launch_bounds__(kBlock) b_ctl_begin(const Inst<T>* __restrict__ tab) {
+__global__ void __launch_bounds__(kBlock) b_ctl_begin(const Inst<T>* __restrict__ tab, int* __restrict__ hvany) {
   const Inst<T>& I = tab[blockIdx.x];
   double* c = I.ctl;
   if (c[C_ACTIVE] == 0.0) return;
@@ -270,6 +289,7 @@ __global__ void __launch_bounds__(kBlock) b_ctl_begin(const Inst<T>* __restrict_
   }
   const int hv_int = (int)k[K_HVINT];
   c[C_HVNOW] = (k[K_HVON] != 0.0 && hv_int > 0 && (((long long)c[C_STEP] + 1) % hv_int) == 0) ? 1.0 : 0.0;
+  if (c[C_HVNOW] != 0.0) atomicOr(hvany, 1);
 }
 
 // Fixed dt of the Riemann loop from the initial state: n = ceil(T/dt0),
@@ -294,8 +314,9 @@ __global__ void __launch_bounds__(kBlock) b_fixed_dt(const Inst<T>* __restrict__
 // time, step and activity after the step (OOP/hit2d.py:870-871; the Riemann
 // loop's time = (step+1)*dt, OOP/problems/riemann_config3.py:741)
 template <class T>
-__global__ void b_ctl_end(const Inst<T>* __restrict__ tab, int ninst) {
+__global__ void b_ctl_end(const Inst<T>* __restrict__ tab, int ninst, int* __restrict__ hvany) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *hvany = 0;
   if (i >= ninst) return;
   double* c = tab[i].ctl;
   const double* k = tab[i].kpar;

@@ -88,6 +88,32 @@ def test_hit_batch_matches_single_instances():
             assert rel_l2(host(b.q[i])[k], host(s.q)[k]) <= 1e-12, (i, k)
 
 
+def test_hit_batch_mixed_hyperviscosity_schedules():
+    """Instances with hyperviscosity every 2, 3 and 4 steps in one batch: on
+    steps 1 and 5 no instance runs it (every post-step launch exits at the
+    batch-wide word), on the others only some do."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig, HITSolver
+
+    base = dict(cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0, viscosity=7.5e-4,
+                forcing_correlation_time=0.5, mach_control=True, large_scale_drag=0.10, cooling_time=5.0,
+                tfinal=1.0, max_steps=6)
+    cfgs = [HIT2DConfig(nx=32, ny=32, target_mach=0.1, forcing_seed=3, hyperviscosity_interval=2, **base),
+            HIT2DConfig(nx=48, ny=40, target_mach=0.25, forcing_seed=4, hyperviscosity_interval=3, **base),
+            HIT2DConfig(nx=40, ny=48, target_mach=0.4, forcing_seed=5, hyperviscosity_interval=4, **base)]
+    print("[batch] mixed hyperviscosity schedules 2/3/4, 6 steps", flush=True)
+    b = BatchRunner(cfgs)
+    b.run()
+    ctl = b.control()
+    for i, (c, napp) in enumerate(zip(cfgs, (3, 2, 1))):
+        s = HITSolver(c)
+        s.run()
+        assert ctl[i, 4] == s.steps == 6
+        assert ctl[i, 7] == napp
+        for k in range(4):
+            assert rel_l2(host(b.q[i])[k], host(s.q)[k]) <= 1e-12, (i, k)
+
+
 def test_hit_batch_vs_oracle():
     from hit_solver_amd.batch import BatchRunner
     from hit_solver_amd.hit2d import HIT2DConfig
