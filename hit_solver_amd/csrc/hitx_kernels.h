@@ -1922,16 +1922,26 @@ __device__ __forceinline__ void fproject_body(const Params<T>& P, const double* 
     as[k] = 0.0;
   }
   if (W.live) {
-#pragma unroll 4
-    for (int y = W.ya; y < W.yb; ++y) {
-      const double n = W.own ? noise[(size_t)y * nx + W.x] : 0.0;
-      ac[0] += n;
+    // rows in groups of 8 with the group's noise loads in flight together;
+    // every k up to kFK without a branch: the sums past fky read the table
+    // row of fky and are never stored
+    const int fk = P.fky;
+    for (int y0 = W.ya; y0 < W.yb; y0 += 8) {
+      double nv[8];
 #pragma unroll
-      for (int k = 1; k <= kFK; ++k) {
-        if (k <= P.fky) {
-          const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + y) * 2;
-          ac[k] += n * ty[0];
-          as[k] += n * ty[1];
+      for (int j = 0; j < 8; ++j) nv[j] = (W.own && y0 + j < W.yb) ? noise[(size_t)(y0 + j) * nx + W.x] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int y = y0 + j;
+        if (y < W.yb) {
+          const double n = nv[j];
+          ac[0] += n;
+#pragma unroll
+          for (int k = 1; k <= kFK; ++k) {
+            const double* ty = P.twy + ((size_t)(min(k, fk) + P.kmy) * ny + y) * 2;
+            ac[k] += n * ty[0];
+            as[k] += n * ty[1];
+          }
         }
       }
     }
