@@ -1073,11 +1073,20 @@ struct hitx_batch {
 
 namespace {
 
+// Grid-stride reduction blocks of a batch instance (b_step_reduce,
+// b_fsynth): 1024 nodes per block, at most 128 (a batch runs tens of
+// thousands of tile blocks anyway, and each block first resolves its
+// instance through the block map, so fewer and fuller blocks pay that once
+// per four nodes a thread).
+int batch_red_blocks(const hitx_plan* pl) {
+  return (int)std::min<size_t>((pl->N + 4 * kBlock - 1) / (4 * kBlock), 128);
+}
+
 int batch_blocks(const hitx_batch* b, const hitx_plan* pl, int kind) {
   const int nx = pl->d.nx, ny = pl->d.ny;
   const bool wx = b->wave_x && line_wave_ok(nx), wy = b->wave_y && line_wave_ok(ny);
   switch (kind) {
-    case OFF_RED: return pl->nblk_stage;
+    case OFF_RED: return batch_red_blocks(pl);
     case OFF_SR: return pl->nblk_sr;
     case OFF_DROWS: return pl->d.drag_enabled && pl->d.n_drag_modes > 0 ? (2 * ny + kBlock - 1) / kBlock : 0;
     case OFF_FROWS: return pl->d.n_force_modes > 0 ? (ny * (pl->fkx + 1) + kBlock - 1) / kBlock : 0;
@@ -1173,7 +1182,7 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   I.R = make_rngjob(base + o.rngws, rng_layout(pl->N), I.rng, I.noise, pl->N);
   I.drows = d(o.drows);
   I.frows = d(o.frows);
-  I.nblk_stage = pl->nblk_stage;
+  I.nblk_stage = batch_red_blocks(pl);
   I.nblk_sr = pl->nblk_sr;
   I.nblk_proj = pl->nblk_proj;
   I.ntile = ((pl->d.nx + kSTX - 1) / kSTX) * ((pl->d.ny + kSTY - 1) / kSTY);

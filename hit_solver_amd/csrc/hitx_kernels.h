@@ -397,15 +397,29 @@ template <class T>
 __device__ __forceinline__ void step_reduce_body(const Params<T>& P, const T* __restrict__ q,
                                                         double* __restrict__ part, int bid, int nblk) {
   double v[3] = {0.0, 0.0, 0.0};
-  const int N = P.nx * P.ny;
-  for (int idx = bid * kBlock + threadIdx.x; idx < N; idx += nblk * kBlock) {
-    const Prim<T> c = prim_at(P, q, N, idx, 0);
+  const int N = P.nx * P.ny, stride = nblk * kBlock;
+  auto node = [&](T a, T b, T cc, T d) {
+    const Prim<T> c = prim(P, a, b, cc, d);
     const T cs = sqrt((P.gamma * c.p) / c.rho);
     T sr = (absT(c.u) + cs) / P.dx;
     sr = sr + (absT(c.v) + cs) / P.dy;
     v[0] = fmax(v[0], (double)sr);
     v[1] += (double)(c.u * c.u + c.v * c.v);
     v[2] += (double)cs;
+  };
+  int idx = bid * kBlock + threadIdx.x;
+  // four nodes' loads in flight together, accumulated in node order
+  for (; idx + 3 * stride < N; idx += 4 * stride) {
+    T a[4], b[4], c[4], d[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) load4(q, N, idx + j * stride, a[j], b[j], c[j], d[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) node(a[j], b[j], c[j], d[j]);
+  }
+  for (; idx < N; idx += stride) {
+    T a, b, c, d;
+    load4(q, N, idx, a, b, c, d);
+    node(a, b, c, d);
   }
   block_reduce_write<3, 1>(v, part + (size_t)bid * 3);
 }
