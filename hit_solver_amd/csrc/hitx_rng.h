@@ -227,8 +227,10 @@ constexpr int kRngScanBatch = 8;  // chunk maps loaded together (one latency per
 // serially (a check of that path).
 
 __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial) {
-  __shared__ uint8_t sx[kRngScan * kRngEntries];
-  __shared__ int sn[kRngScan * kRngEntries];
+  // entry-major ([e][t]): lanes touch consecutive words, also where a lane
+  // reads its own map at a data-dependent entry
+  __shared__ uint8_t sx[kRngEntries * kRngScan];
+  __shared__ int sn[kRngEntries * kRngScan];
   const int t = threadIdx.x;
   const int per = (J.nch + kRngScan - 1) / kRngScan;
   const int c0 = min(t * per, J.nch), c1 = min(c0 + per, J.nch);
@@ -261,8 +263,8 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
   }
 #pragma unroll
   for (int e = 0; e < kRngEntries; ++e) {
-    sx[t * kRngEntries + e] = mx[e];
-    sn[t * kRngEntries + e] = mn[e];
+    sx[e * kRngScan + t] = mx[e];
+    sn[e * kRngScan + t] = mn[e];
   }
   __syncthreads();
   for (int d = 1; d < kRngScan; d <<= 1) {
@@ -271,10 +273,10 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
       // (left run then own run): own map read from LDS at the left's spill
 #pragma unroll
       for (int e = 0; e < kRngEntries; ++e) {
-        const int x1 = sx[(t - d) * kRngEntries + e];
+        const int x1 = sx[e * kRngScan + (t - d)];
         if (x1 < kRngEntries) {
-          mx[e] = sx[t * kRngEntries + x1];
-          mn[e] = sn[(t - d) * kRngEntries + e] + sn[t * kRngEntries + x1];
+          mx[e] = sx[x1 * kRngScan + t];
+          mn[e] = sn[e * kRngScan + (t - d)] + sn[x1 * kRngScan + t];
         } else {
           mx[e] = 0xff;
           mn[e] = 0;
@@ -285,8 +287,8 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
     if (has) {
 #pragma unroll
       for (int e = 0; e < kRngEntries; ++e) {
-        sx[t * kRngEntries + e] = mx[e];
-        sn[t * kRngEntries + e] = mn[e];
+        sx[e * kRngScan + t] = mx[e];
+        sn[e * kRngScan + t] = mn[e];
       }
     }
     __syncthreads();
@@ -294,8 +296,8 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
   // exclusive prefix: the stream enters chunk 0 at offset 0
   int e = 0, cnt = 0;
   if (t > 0) {
-    e = sx[(t - 1) * kRngEntries];
-    cnt = sn[(t - 1) * kRngEntries];
+    e = sx[t - 1];
+    cnt = sn[t - 1];
   }
   for (int cb = c0; cb < c1 && e < kRngEntries; cb += kRngScanBatch) {
     uint4 mv[kRngScanBatch];
@@ -315,8 +317,8 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
     }
   }
   if (t == kRngScan - 1) {
-    const bool overflow = sx[t * kRngEntries] >= kRngEntries;
-    const bool short_ = sn[t * kRngEntries] < J.n;
+    const bool overflow = sx[t] >= kRngEntries;
+    const bool short_ = sn[t] < J.n;
     J.stat[0] = (overflow || short_) ? 1 : (force_serial ? 2 : 0);
   }
 }
