@@ -1793,14 +1793,17 @@ __device__ __forceinline__ T lap_at(const Params<T>& P, const T* __restrict__ a,
 template <class T>
 __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict__ q, const uint8_t* __restrict__ active,
                                         int guard, int invert, T* __restrict__ out, int bx, int by) {
-  __shared__ T sq[20][21];
+  __shared__ T sq[4][20][21];
   __shared__ T sl[18][19];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x0 = bx * 16, y0 = by * 16;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int x = x0 + tx, y = y0 + ty;
   const bool per = P.periodic;
-  auto gmap = [&](int g, int n) { return per ? ((g % n) + n) % n : clampi(g, 0, n - 1); };
+  auto gmap = [&](int g, int n) {
+    if ((unsigned)g < (unsigned)n) return g;
+    return per ? ((g % n) + n) % n : clampi(g, 0, n - 1);
+  };
   int cx = x, cy = y;
   if (guard && P.guard > 0) {
     cx = clampi(x, P.guard, nx - 1 - P.guard);
@@ -1813,25 +1816,41 @@ __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict_
   // position itself; clamped: the clamped node)
   auto tcx = [&](int g) { return per ? g - (x0 - 2) : clampi(g, 0, nx - 1) - (x0 - 2); };
   auto tcy = [&](int g) { return per ? g - (y0 - 2) : clampi(g, 0, ny - 1) - (y0 - 2); };
+  // the four components' tiles (state plus a 2-node halo), every load in
+  // flight together
+  constexpr int NT = 4 * 400, NL = (NT + kBlock - 1) / kBlock;
+  T v[NL];
+#pragma unroll
+  for (int s = 0; s < NL; ++s) {
+    const int i = threadIdx.x + s * kBlock;
+    if (i < NT) {
+      const int c = i / 400, r = i - c * 400, ly = r / 20, lx = r - ly * 20;
+      v[s] = q[(size_t)c * N + gmap(y0 - 2 + ly, ny) * nx + gmap(x0 - 2 + lx, nx)];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NL; ++s) {
+    const int i = threadIdx.x + s * kBlock;
+    if (i < NT) {
+      const int c = i / 400, r = i - c * 400, ly = r / 20, lx = r - ly * 20;
+      sq[c][ly][lx] = v[s];
+    }
+  }
   T o[4];
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < 4; ++c) {
     const T* a = q + (size_t)c * N;
-    __syncthreads();
-    for (int i = threadIdx.x; i < 400; i += kBlock) {
-      const int ly = i / 20, lx = i - ly * 20;
-      sq[ly][lx] = a[gmap(y0 - 2 + ly, ny) * nx + gmap(x0 - 2 + lx, nx)];
-    }
-    __syncthreads();
+    T(*sqc)[21] = sq[c];
+    __syncthreads();  // tiles stored; sl free (the previous component's biharmonic has read it)
     for (int i = threadIdx.x; i < 324; i += kBlock) {
       const int ly = i / 18, lx = i - ly * 18;
       const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
       const int ey = tcy(gy), ex = tcx(gx);  // node evaluated, tile coordinates
       const int exl = tcx(per ? gx - 1 : clampi(gx, 0, nx - 1) - 1), exr = tcx(per ? gx + 1 : clampi(gx, 0, nx - 1) + 1);
       const int eyl = tcy(per ? gy - 1 : clampi(gy, 0, ny - 1) - 1), eyr = tcy(per ? gy + 1 : clampi(gy, 0, ny - 1) + 1);
-      const T cc = sq[ey][ex];
-      const T lx_ = dv((sq[ey][exr] - T(2.0) * cc) + sq[ey][exl], P.dx2);
-      const T ly_ = dv((sq[eyr][ex] - T(2.0) * cc) + sq[eyl][ex], P.dy2);
+      const T cc = sqc[ey][ex];
+      const T lx_ = dv((sqc[ey][exr] - T(2.0) * cc) + sqc[ey][exl], P.dx2);
+      const T ly_ = dv((sqc[eyr][ex] - T(2.0) * cc) + sqc[eyl][ex], P.dy2);
       sl[ly][lx] = lx_ + ly_;
     }
     __syncthreads();
@@ -1851,7 +1870,7 @@ __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict_
         const T lx_ = dv((sl[lcy][lxr] - T(2.0) * Lc) + sl[lcy][lxl], P.dx2);
         const T ly_ = dv((sl[lyr][lcx] - T(2.0) * Lc) + sl[lyl][lcx], P.dy2);
         bih = lx_ + ly_;
-        ac = sq[cy - y0 + 2][cx - x0 + 2];
+        ac = sqc[cy - y0 + 2][cx - x0 + 2];
       } else {
         int xl, xr, yl, yr;
         if (per) {
