@@ -24,6 +24,8 @@ __device__ __forceinline__ Tile2 tile_xcd() {
   return Tile2{b % gx, b / gx};
 }
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+// g wrapped into [0, n) (periodic grids); in-range indices skip the modulo
+__device__ __forceinline__ int wrapn(int g, int n) { return (unsigned)g < (unsigned)n ? g : ((g % n) + n) % n; }
 
 template <class T>
 __device__ __forceinline__ T absT(T a) { return fabs(a); }
@@ -479,8 +481,8 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
     const int r = t / EX, c = t - r * EX;
     int gy = y0 - H + r, gx = x0 - H + c;
     if (per) {
-      gy = ((gy % ny) + ny) % ny;
-      gx = ((gx % nx) + nx) % nx;
+      gy = wrapn(gy, ny);
+      gx = wrapn(gx, nx);
     } else {
       gy = clampi(gy, 0, ny - 1);
       gx = clampi(gx, 0, nx - 1);
@@ -943,8 +945,8 @@ __device__ __forceinline__ void dilate_body(const Params<T>& P, const T* __restr
     int gy = y0 - w + r, gx = x0 - w + c;
     uint8_t m = 0;
     if (per) {
-      gy = ((gy % ny) + ny) % ny;
-      gx = ((gx % nx) + nx) % nx;
+      gy = wrapn(gy, ny);
+      gx = wrapn(gx, nx);
       m = rawg[gy * nx + gx];
     } else if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
       m = rawg[gy * nx + gx];
@@ -1801,8 +1803,7 @@ __device__ __forceinline__ void hv_body(const Params<T>& P, const T* __restrict_
   const int x = x0 + tx, y = y0 + ty;
   const bool per = P.periodic;
   auto gmap = [&](int g, int n) {
-    if ((unsigned)g < (unsigned)n) return g;
-    return per ? ((g % n) + n) % n : clampi(g, 0, n - 1);
+    return per ? wrapn(g, n) : clampi(g, 0, n - 1);
   };
   int cx = x, cy = y;
   if (guard && P.guard > 0) {
