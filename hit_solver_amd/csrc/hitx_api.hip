@@ -1234,7 +1234,8 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
     b_stage_reduce<T, true><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
   else
     b_stage_tile<T, DRAG><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
-  b_stage_finalize<T, DRAG><<<n, kBlock, 0, s>>>(tab, st, DRAG && b->stage_walk ? 1 : 0);
+  b_stage_finalize<T, DRAG><<<dim3(n, DRAG ? 1 + kFinDragGroups : 1), kBlock, 0, s>>>(tab, st,
+                                                                                  DRAG && b->stage_walk ? 1 : 0);
   if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
   if (b->split_sensor) {
     if (b->raw_tile)

@@ -111,11 +111,34 @@ __global__ void __launch_bounds__(kBlock) b_stage_tile(const Inst<T>* __restrict
   stage_tile_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, bx, by, lb);
 }
 
+// Grid (instances, 1 + kFinDragGroups with drag): block y = 0 reduces the
+// stage scalar columns and writes the stage scalars, block y = g >= 1 the
+// drag columns [(g-1) kFinDragCols, g kFinDragCols).  Each column is reduced
+// on its own in the fixed order, so the split gives the one-block bits.
+constexpr int kFinDragCols = 12, kFinDragGroups = kDragCols / kFinDragCols;
+static_assert(kDragCols % kFinDragCols == 0, "drag columns in whole groups");
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __restrict__ tab, int st, int walk) {
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || (st == 3 && I.ctl[C_HVNOW] == 0.0)) return;
-  stage_finalize_body<T, DRAG>(I.P, I.part, walk ? I.nblk_sr : I.ntile, I.scal);
+  const int nblk = walk ? I.nblk_sr : I.ntile;
+  constexpr int NR = DRAG ? (R_DRAG + kDragCols) : R_DRAG;
+  __shared__ double r[DRAG ? (R_DRAG > kFinDragCols ? R_DRAG : kFinDragCols) : R_DRAG];
+  if (blockIdx.y == 0) {
+    reduce_partial_cols<R_DRAG, 0x1Full>(I.part, nblk, NR, 0, r);
+    __syncthreads();
+    stage_scal_write<T, false>(I.P, r, I.scal);
+    return;
+  }
+  if (DRAG) {
+    const int c0 = ((int)blockIdx.y - 1) * kFinDragCols;
+    if (c0 >= 4 * I.P.ndrag) return;
+    reduce_partial_cols<kFinDragCols, 0>(I.part, nblk, NR, R_DRAG + c0, r);
+    __syncthreads();
+    const double N = (double)I.P.nx * (double)I.P.ny;
+    for (int k = threadIdx.x; k < kFinDragCols && c0 + k < 4 * I.P.ndrag; k += blockDim.x)
+      I.scal[S_DRAG + c0 + k] = r[k] / N;
+  }
 }
 
 template <class T>
