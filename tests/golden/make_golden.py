@@ -225,7 +225,7 @@ def f4_hit_runs(R, out):
         diagnostics_every=5, snapshot_every=10_000,
     )
     with tempfile.TemporaryDirectory() as tmp:
-        # configs[0]-style: plain Euler, no drag/cooling, tfinal 0.02 (~12 steps, 2 HV passes)
+        # the small CPU-runnable HIT case: plain Euler, no drag/cooling, tfinal 0.02 (~12 steps, 2 HV passes)
         q, t, steps, hist = _run_hit(R, tmp, tfinal=0.02, viscosity=0.0, **common)
         out["F4_plain_q"] = q
         out["F4_plain_t"] = np.float64(t)
