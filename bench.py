@@ -234,7 +234,7 @@ def serving_measurement(count, rate, steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is design-only "
+        print(json.dumps({"error": "hit_solver_amd runs on exactly one GPU; multi-GPU sharding is design-only "
                                    "(instances are independent: shard the instance table)", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch

@@ -2,7 +2,7 @@
 
 This script is the only place that imports the reference. It runs in the
 build container, where the reference tree is mounted at /root/reference, and
-writes small ``.npz`` fixtures next to itself.  The GPU box never needs the
+writes small ``.npz`` fixtures next to itself.  GPU runs never need the
 reference: the tests read the fixtures.
 
 Which functions need an import.  The single-function pieces of the hot path
