@@ -2078,6 +2078,7 @@ __device__ __forceinline__ void fsynth_at(const Params<T>& P, const double* __re
   const double* H = frows + (size_t)y * 2 * FR;
   ax = H[0];
   ay = H[FR];
+#pragma unroll 4
   for (int k = 1; k <= P.fkx; ++k) {
     const double* tx = P.twx + ((size_t)(k + P.kmx) * P.nx + x) * 2;
     const double c = tx[0], sn = tx[1];
