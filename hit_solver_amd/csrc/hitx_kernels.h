@@ -122,9 +122,18 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
   const int lane = threadIdx.x & 63;
   auto row_of = [&](int y) { return P.periodic ? (y < 0 ? y + ny : (y >= ny ? y - ny : y)) : clampi(y, 0, ny - 1); };
   if (W.live) {
+    auto prim_e = [&](T a, T b, T c, T d) {
+      if (enforce_first) enforce(P, a, b, c, d);
+      return prim(P, a, b, c, d);
+    };
     Prim<T> pv = prim_at(P, q, N, row_of(W.ya - 1) * nx + W.xl, enforce_first);
-    Prim<T> cu = prim_at(P, q, N, W.ya * nx + W.xl, enforce_first);
-    Prim<T> nn = prim_at(P, q, N, row_of(W.ya + 1) * nx + W.xl, enforce_first);
+    // the state as given of the current and the next row stays in registers
+    // (cooling and the drag sums read it; no reload)
+    T c0, c1, c2, c3, m0, m1, m2, m3;
+    load4(q, N, W.ya * nx + W.xl, c0, c1, c2, c3);
+    load4(q, N, row_of(W.ya + 1) * nx + W.xl, m0, m1, m2, m3);
+    Prim<T> cu = prim_e(c0, c1, c2, c3);
+    Prim<T> nn = prim_e(m0, m1, m2, m3);
     for (int y = W.ya; y < W.yb; ++y) {
       // next row's loads in flight while this row is reduced
       T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0);
@@ -160,8 +169,7 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
           v[R_VORT2] += (double)w * (double)w;
         }
         if (P.cool_on || DRAG) {
-          T a, b, c, d;
-          load4(q, N, y * nx + x, a, b, c, d);
+          const T a = c0, b = c1, c = c2, d = c3;
           if (P.cool_on) v[R_PSUM] += (double)prim(P, a, b, c, d).p;
           if (DRAG) {
             const double fb = (double)b, fc = (double)c;
@@ -183,9 +191,16 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       }
       pv = cu;
       cu = nn;
+      c0 = m0;
+      c1 = m1;
+      c2 = m2;
+      c3 = m3;
       if (y + 1 < W.yb) {
-        if (enforce_first) enforce(P, n0, n1, n2, n3);
-        nn = prim(P, n0, n1, n2, n3);
+        m0 = n0;
+        m1 = n1;
+        m2 = n2;
+        m3 = n3;
+        nn = prim_e(n0, n1, n2, n3);
       }
     }
   }
