@@ -344,13 +344,17 @@ __device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double*
     const uint32_t cw[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
     int nxt = J.entry[c];
     k = J.base[c];
+    // the chunk's attempt values, every load in flight together
+    double vv[kRngChunk];
+#pragma unroll
+    for (int q = 0; q < kRngChunk; ++q) vv[q] = J.vals[(size_t)q * J.nch + c];
 #pragma unroll
     for (int q = 0; q < kRngChunk; ++q) {
       if (q == nxt) {
         const uint32_t code = (cw[q >> 2] >> ((q & 3) * 8)) & 0xffu;
         const int len = (int)(code & 0x7f);
         if (code & 0x80) {
-          if (k < J.n) lds[k - lo] = J.vals[(size_t)q * J.nch + c];
+          if (k < J.n) lds[k - lo] = vv[q];
           if (k == J.n - 1) J.stat[1] = c * kRngChunk + q + len;  // words consumed
           ++k;
         }
