@@ -217,7 +217,7 @@ __device__ __forceinline__ uint32_t map_entry(const uint4& m, int e) {
   const uint32_t w = h == 0 ? m.x : (h == 1 ? m.y : (h == 2 ? m.z : m.w));
   return (e & 1) ? (w >> 16) : (w & 0xffffu);
 }
-constexpr int kRngScanBatch = 8;  // chunk maps loaded together (one latency per batch)
+constexpr int kRngScanBatch = 16;  // chunk maps loaded together (one latency per batch)
 
 // Step 3 (one block of kRngScan threads per request).  A chunk's map sends
 // the entry offset e to (spill, count); spill 0xff marks an overflow.  Each
