@@ -225,6 +225,28 @@ class BatchRunner:
                 self.phi[i].copy_(other.phi[j])
                 self.rng_state[i].copy_(other.rng_state[j])
 
+    def export_instance(self, j):
+        """Copies of what continues instance j elsewhere (state, control
+        scalars, OU coefficients, noise stream): a few arrays of the state's
+        size, so the batch workspace can be freed before the next batch is
+        built."""
+        with self._ctx():
+            out = {"q": self.q[j].clone(), "ctl": self.ctl_views[j].clone()}
+            if self.hit:
+                out["phi"] = self.phi[j].clone()
+                out["rng"] = self.rng_state[j].clone()
+        return out
+
+    def import_instance(self, i, saved):
+        """Continue an exported instance as this batch's instance i (same
+        configuration)."""
+        with self._ctx():
+            self.q[i].copy_(saved["q"])
+            self.ctl_views[i].copy_(saved["ctl"])
+            if self.hit:
+                self.phi[i].copy_(saved["phi"])
+                self.rng_state[i].copy_(saved["rng"])
+
     def fixed_dt(self):
         """Fixed dt of the Riemann instances that have none yet."""
         check(lib().hitx_batch_fixed_dt(self.handle, self._stream()))
@@ -289,6 +311,15 @@ class BatchRunner:
         torch.cuda.synchronize()
         check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp), int(line_form), int(sensor_form)))
         self._graph = None  # grid sizes changed
+
+    def set_stage_form(self, form: int):
+        """Form of the drag stage reductions: 0 auto (tiles while the batch has
+        at most 4096 sensor tiles, column walk above), 1 tiles, 2 column walk.
+        Fixing it keeps an instance's summation order independent of the
+        rest of the batch."""
+        torch.cuda.synchronize()
+        check(lib().hitx_batch_stage_form(self.handle, int(form)))
+        self._graph = None
 
     def autotune(self, candidates=None, steps: int = 3):
         """Time a few steps per candidate (rb, rbp, line_form, sensor_form)

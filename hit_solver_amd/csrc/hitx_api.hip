@@ -1052,6 +1052,7 @@ int hitx_timing_read(hitx_plan* pl, double* total_ms, int32_t* launches) {
 struct hitx_batch {
   int n = 0, esize = 8, periodic = 1, drag = 0, device = 0;
   bool stage_walk = false;  // drag stage reductions by column walk (many tiles) instead of on tiles
+  int stage_form = 0;       // 0 auto (column walk above kStageTileMaxTiles tiles), 1 tiles, 2 column walk
   std::vector<hitx_plan*> plans;
   std::vector<double> kpar;  // n * K_COUNT
   struct Buf {
@@ -1129,7 +1130,7 @@ void batch_offsets(hitx_batch* b) {
     b->offs[(size_t)k * (n + 1) + n] = acc;
     b->total[k] = acc;
   }
-  b->stage_walk = stage_walk_form(b->total[OFF_SENS]);
+  b->stage_walk = b->stage_form == 2 || (b->stage_form == 0 && stage_walk_form(b->total[OFF_SENS]));
   // block maps: for kind k, entry b = instance << kMapLbBits | local block
   // (the map of kind k starts at the index held in slot OFF_KINDS*(n+1) + k);
   // slot OFF_KINDS*(n+1) + OFF_KINDS is the batch-wide hyperviscosity word
@@ -1473,6 +1474,16 @@ int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, i
     pl->nblk_sr = walk_blocks(pl->d.nx, pl->d.ny, pl->rb, 1);
     pl->nblk_proj = walk_blocks(pl->d.nx, pl->d.ny, pl->rbp, 0);
   }
+  batch_offsets(b);
+  if (b->offs.size() > b->offs_cap) return fail(HITX_EINVAL, "block map larger than its bound");
+  if (b->ws) return b->esize == 8 ? batch_bind_t<double>(b, b->ws, false) : batch_bind_t<float>(b, b->ws, false);
+  return HITX_OK;
+}
+
+int hitx_batch_stage_form(hitx_batch* b, int32_t form) {
+  if (!b) return fail(HITX_EINVAL, "null batch");
+  if (form < 0 || form > 2) return fail(HITX_EINVAL, "stage form must be 0 (auto), 1 (tiles) or 2 (column walk)");
+  b->stage_form = form;
   batch_offsets(b);
   if (b->offs.size() > b->offs_cap) return fail(HITX_EINVAL, "block map larger than its bound");
   if (b->ws) return b->esize == 8 ? batch_bind_t<double>(b, b->ws, false) : batch_bind_t<float>(b, b->ws, false);

@@ -119,9 +119,18 @@ class Plan:
         check(lib().hitx_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
         self.ws_bytes = int(lib().hitx_workspace_bytes(h))
-        self.ws = torch.zeros(self.ws_bytes, dtype=torch.uint8, device="cuda")  # ticket starts at 0
+        self._ws = None
         self.red = torch.zeros(64, dtype=torch.float64, device="cuda")
         self.device = torch.cuda.current_device()
+
+    @property
+    def ws(self):
+        """The plan's workspace, allocated (zeroed: the reduction ticket starts
+        at 0) on the first call that needs it, so plans used only for
+        pointwise work (enforce, initial states) hold no grid-sized buffers."""
+        if self._ws is None:
+            self._ws = torch.zeros(self.ws_bytes, dtype=torch.uint8, device="cuda")
+        return self._ws
 
     def __del__(self):
         try:

@@ -11,7 +11,8 @@ membership changes its batch is rebuilt and the running solves move over
 (state, control scalars, OU coefficients, noise stream: copy_from).
 
 Launch shapes are pinned per instance (each grid's own reduction tiles, the
-wave form of the line sweep where the line allows it), so a solve's
+wave form of the line sweep where the line allows it, the column-walk form
+of the drag stage sums at every batch size), so a solve's
 arithmetic does not depend on what else is in the mix: a result is the same
 alone and in any mix, bit for bit.
 
@@ -168,25 +169,38 @@ class ServingEngine:
 
     # ---------------------------------------------------------------- batches
     def _rebuild(self, key, g: _Group):
+        """Rebuild a class's batch after its membership changed.  The running
+        solves' state (state, control scalars, OU coefficients, noise stream)
+        is staged in small copies and the old batch workspace is freed before
+        the new one is allocated, so a rebuild never holds two workspaces."""
         old, old_slots = g.runner, g.slots
         g.runner, g.slots = None, []
+        carry = {}
+        if old is not None:
+            torch.cuda.synchronize()
+            where = {id(m): j for j, m in enumerate(old_slots)}
+            for m in g.members:
+                if id(m) in where:
+                    carry[id(m)] = old.export_instance(where[id(m)])
+            old._sync()
+            del old
         if g.members:
             if g.stream is None:
                 g.stream = torch.cuda.Stream()
             cfgs = [m.cfg for m in g.members]
-            where = {id(m): j for j, m in enumerate(old_slots)} if old is not None else {}
-            init = [id(m) not in where for m in g.members]
+            init = [id(m) not in carry for m in g.members]
             caps = [m.result.request.steps for m in g.members]
             torch.cuda.synchronize()
             r = BatchRunner(cfgs, max_steps=caps, init=init, stream=g.stream)
             for i, m in enumerate(g.members):
                 if not init[i]:
-                    r.copy_from(i, old, where[id(m)])
-            # pinned launch shapes: per-instance tiles, wave form where the line allows
+                    r.import_instance(i, carry.pop(id(m)))
+            # pinned launch shapes: per-instance tiles, wave form where the line
+            # allows, drag stage sums by column walk whatever the class's size
             r.tune(0, 0, 2, 0)
+            r.set_stage_form(2)
             r.capture(self.round_steps)
             g.runner, g.slots = r, list(g.members)
-        del old
         g.dirty = False
 
     # ---------------------------------------------------------------- loop

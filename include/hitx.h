@@ -332,6 +332,13 @@ int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
  * order of the reductions and the rounding of the line solve only.  Blocking
  * when bound (re-uploads the instance table); no work may be in flight. */
 int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_form, int32_t sensor_form);
+/* Form of the drag stage reductions (no reference counterpart): 0 auto (on
+ * the sensor tiles while the batch has at most 4096 of them, by column walk
+ * above), 1 always on tiles, 2 always by column walk.  The two forms sum the
+ * stage scalars in different orders; fixing the form makes an instance's
+ * arithmetic independent of what else is in its batch.  Blocking like
+ * hitx_batch_tune. */
+int hitx_batch_stage_form(hitx_batch* batch, int32_t form);
 /* Device pointer of one of instance i's buffers inside the bound workspace. */
 int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
 /* Fixed dt of every instance from its current state (Riemann loop). */

@@ -22,6 +22,8 @@ Riemann step counts).  What only the reference can pin is the composition:
 * Mach control and the HIT step loop with hyperviscosity scheduling -> F4
 * the Riemann fixed-dt loop with local hyperviscosity and guards    -> F5
 * ``initialize_hit_2d``                                               -> F6
+* the CPU-runnable forced-HIT configuration end to end                -> F7
+* the Riemann run diagnostics and vorticity plane                     -> F5 ``diag`` / ``omega``
 
 Run:  python tests/golden/make_golden.py   (numpy version is recorded)
 """
@@ -245,6 +247,11 @@ def f4_hit_runs(R, out):
         out["F4_prod_hist_alpha"] = hist["forcing_alpha"]
 
 
+F5_DIAG_KEYS = ("hyperviscosity_enabled", "hyperviscosity_applications", "final_sensor_fraction", "density_min",
+                "density_max", "pressure_min", "pressure_max", "max_abs_vorticity", "has_nonfinite",
+                "density_positive", "pressure_positive", "time", "steps")
+
+
 def f5_riemann_runs(R, out):
     rc3 = R["rc3"]
     for tag, (cls, kw) in {
@@ -259,6 +266,28 @@ def f5_riemann_runs(R, out):
         out[f"F5_{tag}_steps"] = np.int64(diag.steps)
         out[f"F5_{tag}_hvapps"] = np.int64(diag.hyperviscosity_applications)
         out[f"F5_{tag}_sensor_fraction"] = np.float64(diag.final_sensor_fraction)
+        # the reference's end-of-run diagnostics (riemann_config3.py:629-650)
+        # and its vorticity plane (one-sided differences at the ends, :459-463)
+        out[f"F5_{tag}_diag"] = np.array([float(getattr(diag, k)) for k in F5_DIAG_KEYS])
+        out[f"F5_{tag}_omega"] = rc3.vorticity_z(q, cfg.domain, cfg.equation)
+
+
+def f7_config1(R, out):
+    """The CPU-runnable forced-HIT case as the benchmark states it: 64x64,
+    FP64, plain Euler (viscosity 0), Mach 0.25, shells 3-5, seed 1234, CFL
+    0.05, T 0.05, hyperviscosity mn 0.002 every 5 steps, P_target 1e-3,
+    tau_OU 1.0, alpha memory 0.2, no drag, cooling or Mach control."""
+    with tempfile.TemporaryDirectory() as tmp:
+        q, t, steps, hist = _run_hit(
+            R, tmp, nx=64, ny=64, cfl=0.05, tfinal=0.05, target_mach=0.25, initial_kmin=3, initial_kmax=5,
+            forcing_kmin=3.0, forcing_kmax=5.0, forcing_seed=1234, viscosity=0.0, hyperviscosity_mn=0.002,
+            hyperviscosity_interval=5, target_energy_injection=1.0e-3, forcing_correlation_time=1.0,
+            forcing_alpha_memory=0.2, diagnostics_every=10, snapshot_every=10_000,
+        )
+    out["F7_q"] = q
+    out["F7_t"] = np.float64(t)
+    out["F7_steps"] = np.int64(steps)
+    out["F7_hist_KE"] = hist["kinetic_energy"]
 
 
 def f6_init(R, out):
@@ -287,6 +316,7 @@ def main():
     f4_hit_runs(R, out)
     f5_riemann_runs(R, out)
     f6_init(R, out)
+    f7_config1(R, out)
     path = OUT / "reference_vectors.npz"
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({path.stat().st_size/1024:.1f} KiB, {len(out)} arrays)")

@@ -63,7 +63,9 @@ def test_hit_initial_512_diagnostics_match_reference(golden):
     assert d["vorticity_rms"] == pytest.approx(vort, rel=1e-12)
     assert abs(d["divergence_rms"] - div) <= 1e-12
     ref_count = int(golden["F6_512_mask_count"])
-    assert abs(count - ref_count) <= 81, f"mask count {count} vs reference {ref_count}"
+    # the device initial state agrees with the reference's to 1e-13, and no
+    # node sits that close to a sensor threshold: the count is the same
+    assert count == ref_count, f"mask count {count} vs reference {ref_count}"
     print(f"[init] mask count {count} (reference {ref_count})", flush=True)
 
 

@@ -152,3 +152,36 @@ def test_hit_initial_state(golden):
     p = H.HITParams(nx=32, ny=24, target_mach=0.25, initial_kmin=3, initial_kmax=5)
     q = enforce(H.initial_state(p, np.random.default_rng(1234)), p.gas())
     _eq(q, golden["F6_q_32x24"])
+
+
+F5_DIAG_KEYS = ("hyperviscosity_enabled", "hyperviscosity_applications", "final_sensor_fraction", "density_min",
+                "density_max", "pressure_min", "pressure_max", "max_abs_vorticity", "has_nonfinite",
+                "density_positive", "pressure_positive", "time", "steps")
+
+
+@pytest.mark.parametrize("tag,factory,kw", [
+    ("c3", R.config3, dict(nx=48, ny=40)),
+    ("c308", R.config3_08, dict(nx=40, ny=40)),
+    ("c6", R.config6, dict(nx=40, ny=48)),
+])
+def test_riemann_diagnostics_and_vorticity(golden, tag, factory, kw):
+    """The oracle's run diagnostics and vorticity plane equal the reference's
+    (riemann_config3.py:459-463, 629-650) on the reference's own end states."""
+    p = factory(tfinal=0.02, shear_threshold=None, progress_every=10_000, **kw)
+    q = golden[f"F5_{tag}_q"]
+    _eq(R.vorticity_z(q, p.gas(), p.dx, p.dy), golden[f"F5_{tag}_omega"])
+    steps = int(golden[f"F5_{tag}_steps"])
+    d = R.diagnostics(q, p, steps * R.RiemannRun(p).dt, steps, int(golden[f"F5_{tag}_hvapps"]))
+    _eq(np.array([float(d[k]) for k in F5_DIAG_KEYS]), golden[f"F5_{tag}_diag"])
+
+
+def test_hit_config1_run(golden):
+    """The CPU-runnable forced-HIT configuration (64x64, plain Euler, T 0.05)
+    end to end, bit for bit."""
+    p = H.HITParams(nx=64, ny=64, cfl=0.05, tfinal=0.05, target_mach=0.25, initial_kmin=3, initial_kmax=5,
+                    forcing_kmin=3.0, forcing_kmax=5.0, forcing_seed=1234, viscosity=0.0, hyperviscosity_mn=0.002,
+                    hyperviscosity_interval=5, target_energy_injection=1.0e-3, forcing_correlation_time=1.0,
+                    forcing_alpha_memory=0.2)
+    q, t, steps = H.HITRun(p).run()
+    assert steps == golden["F7_steps"] and t == golden["F7_t"]
+    _eq(q, golden["F7_q"])
