@@ -60,6 +60,7 @@ SIGNATURES = {
     "hitx_sensor": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, ctypes.c_size_t, c_vp]),
     "hitx_rhs": (c_i32, [c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),
     "hitx_rk3_stage": (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_rk3_stage_static": (c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp]),
     "hitx_hyperviscosity": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, ctypes.c_size_t, c_vp]),
     "hitx_enforce": (c_i32, [c_vp, c_vp, c_vp, c_i32, c_vp]),
     "hitx_step_reduce": (c_i32, [c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp]),

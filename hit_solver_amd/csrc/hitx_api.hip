@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -682,6 +683,62 @@ int hitx_rk3_stage(hitx_plan* pl, int32_t stage, const void* q0, const void* qs,
   hipStream_t s = static_cast<hipStream_t>(stream);
   return pl->esize == 8 ? rhs_or_stage<double>(pl, MODE_STAGE, stage, q0, qs, out, fx, fy, dt, ws, s)
                         : rhs_or_stage<float>(pl, MODE_STAGE, stage, q0, qs, out, fx, fy, dt, ws, s);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- stage without allocation
+// hitx_rk3_stage_static runs hitx_rk3_stage with a file-scope device array in
+// place of the caller's workspace.  In host code `scratch` names a host-side
+// stand-in, not device memory: the array's device address comes from
+// hipGetSymbolAddress, taken once per device on the entry point's first call
+// there (with that device current) and kept per device.  Every call on a
+// device shares that device's array; the stage writes every workspace byte it
+// reads within the call (stage partials, mask, transposed state, sweep
+// results), so nothing must start at zero.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices] = {};
+
+int scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= kScratchMaxDevices) return fail(HITX_EDEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(HITX_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HITX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hitx_rk3_stage_static(hitx_plan* pl, int32_t stage, const void* q0, const void* qs, void* out, const void* fx,
+                          const void* fy, double dt, void* stream) {
+  if (!pl) return fail(HITX_EINVAL, "null plan");
+  if (pl->ws_bytes > kScratchBytes)
+    return fail(HITX_EINVAL, "grid needs " + std::to_string(pl->ws_bytes) + " bytes of workspace; the static scratch "
+                             "holds " + std::to_string(kScratchBytes) + " (use hitx_rk3_stage with a workspace)");
+  if (pl->timing) return fail(HITX_EINVAL, "sweep timing creates events: turn it off for hitx_rk3_stage_static");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != pl->device)
+    return fail(HITX_EINVAL, "plan belongs to device " + std::to_string(pl->device) + ", current device is " +
+                             std::to_string(dev));
+  void* ws = nullptr;
+  const int rc = scratch_address(dev, &ws);
+  if (rc) return rc;
+  return hitx_rk3_stage(pl, stage, q0, qs, out, fx, fy, dt, ws, kScratchBytes, stream);
 }
 
 int hitx_hyperviscosity(hitx_plan* pl, const void* q, const uint8_t* active, int32_t invert, void* out,

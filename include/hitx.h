@@ -122,6 +122,20 @@ int hitx_rk3_stage(hitx_plan* plan, int32_t stage, const void* q0, const void* q
                    const void* fx, const void* fy, double dt, void* ws, size_t ws_bytes,
                    void* stream);
 
+/* hitx_rk3_stage without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a static device array of 32 MiB
+ * (33554432 bytes) that each device the library
+ * runs on holds until the process ends; grids whose hitx_workspace_bytes
+ * exceeds it are rejected with HITX_EINVAL (512x512 FP64 and 768x768 FP32
+ * fit, 1024x1024 does not).  Every call on one device shares that device's array, so
+ * calls that overlap on one GPU (other streams, other threads) race: issue
+ * them in order on one stream.  Sweep timing (hitx_timing_enable) must be
+ * off.  Results equal hitx_rk3_stage's bit for bit. */
+int hitx_rk3_stage_static(hitx_plan* plan, int32_t stage, const void* q0, const void* qs, void* out,
+                          const void* fx, const void* fy, double dt, void* stream);
+
 /* Biharmonic filter on the nodes where (active[ny][nx] != 0) xor invert --
  * invert = 1 takes the sensor's WENO mask and filters the compact nodes --
  * then enforce_physical_state, then the outflow guard if apply_guard.  out
