@@ -305,7 +305,9 @@ class BatchRunner:
     def tune(self, rb: int, rbp: int, line_form: int = 0, sensor_form: int = 0):
         """Launch shapes of every instance: rows per wave of the stage
         reductions and of the forcing projection, the line-sweep form (0 auto,
-        1 block per line, 2 wavefront per line) and the sensor form (0 auto,
+        1 block per line, 2 wavefront per line, 3 wavefront per line with the
+        y-lines read from the state through an LDS slab instead of a
+        transposed copy) and the sensor form (0 auto,
         1 fused tile, 2 split with the column-walk raw mask, 3 split with the
         tile raw mask)."""
         torch.cuda.synchronize()
@@ -326,7 +328,7 @@ class BatchRunner:
         on a copy of the batch's device state, restore the state and keep the
         fastest candidate.  Returns (best, {candidate: ms per step})."""
         if candidates is None:
-            candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2, 3) for rb in (4, 16, 32)
+            candidates = [(rb, rbp, lf, sf) for lf in (1, 2, 3) for sf in (1, 2, 3) for rb in (4, 16, 32)
                           for rbp in ((16, 64) if self.hit else (16,))]
         torch.cuda.synchronize()
         backup = self.ws.clone()

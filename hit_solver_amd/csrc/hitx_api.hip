@@ -262,6 +262,16 @@ hipError_t allow_line_lds(size_t bytes) {
   return e;
 }
 
+template <class T>
+hipError_t allow_slab_lds(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&b_line_wave_cols<T, true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&b_line_wave_cols<T, false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  return e;
+}
+
 size_t sensor_lds_bytes(int w, int esize) {
   const int H = w + 1, EX = kSTX + 2 * H, EY = kSTY + 2 * H;
   return (size_t)5 * EX * EY * esize + (size_t)(kSTX + 2 * w) * (kSTY + 2 * w) + (size_t)(kSTY + 2 * w) * kSTX +
@@ -370,8 +380,9 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
     if (!fold) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, nblk, w.scal);
   }
   if (fold) {
-    k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw);
-    k_dilate<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.raw, mask_out, w.maskT, qT);
+    k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw, qT);
+    const int ntiles = ((pl->d.nx + kDBCols - 1) / kDBCols) * ((pl->d.ny + kDBRows - 1) / kDBRows);
+    k_dilate_bits<T><<<(ntiles + kDBWaves - 1) / kDBWaves, 64 * kDBWaves, 0, s>>>(P, w.raw, mask_out, w.maskT);
   } else {
     k_sensor<T><<<g, kBlock, pl->sensor_lds, s>>>(P, q, enforce_first, w.scal, mask_out, w.maskT, qT);
   }
@@ -1122,8 +1133,9 @@ struct hitx_batch {
   std::vector<int> offs;  // OFF_KINDS * (n + 1)
   size_t line_lds = 0, sensor_lds = 0, proj_lds = 0, wave_slice = 16;
   bool line_small[2] = {true, true};  // every block-form line of the axis in one-chunk shape
-  int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave
-  bool wave_x = false, wave_y = false;
+  int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave, 3 wave with the slab y-sweep
+  bool wave_x = false, wave_y = false, slab_y = false;
+  size_t slab_lds = 0;  // dynamic LDS of the slab y-sweep (slab or slices, whichever is larger)
   int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split (column walk), 3 split (tiles)
   bool split_sensor = false, raw_tile = false;
   void* ws = nullptr;
@@ -1149,10 +1161,11 @@ int batch_blocks(const hitx_batch* b, const hitx_plan* pl, int kind) {
     case OFF_DROWS: return pl->d.drag_enabled && pl->d.n_drag_modes > 0 ? (2 * ny + kBlock - 1) / kBlock : 0;
     case OFF_FROWS: return pl->d.n_force_modes > 0 ? (ny * (pl->fkx + 1) + kBlock - 1) / kBlock : 0;
     case OFF_SENS: return ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
+    case OFF_DB: return (((nx + kDBCols - 1) / kDBCols) * ((ny + kDBRows - 1) / kDBRows) + kDBWaves - 1) / kDBWaves;
     case OFF_LX: return wx ? 0 : ny;
     case OFF_LY: return wy ? 0 : nx;
     case OFF_LXW: return wx ? (ny + kLineWaves - 1) / kLineWaves : 0;
-    case OFF_LYW: return wy ? (nx + kLineWaves - 1) / kLineWaves : 0;
+    case OFF_LYW: return wy ? (b->slab_y ? (nx + kSlabWaves - 1) / kSlabWaves : (nx + kLineWaves - 1) / kLineWaves) : 0;
     case OFF_T16: return ((nx + 15) / 16) * ((ny + 15) / 16);
     case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
     case OFF_RNG:
@@ -1174,9 +1187,10 @@ void batch_offsets(hitx_batch* b) {
   for (const hitx_plan* pl : b->plans) cells += pl->N;
   b->split_sensor = b->sensor_form >= 2 || (b->sensor_form == 0 && cells >= (1ll << 20));
   b->raw_tile = b->sensor_form == 3;
-  b->wave_x = b->line_form == 2 || (b->line_form == 0 && lx >= kWaveFormMinLines);
-  b->wave_y = b->line_form == 2 || (b->line_form == 0 && ly >= kWaveFormMinLines);
+  b->wave_x = b->line_form >= 2 || (b->line_form == 0 && lx >= kWaveFormMinLines);
+  b->wave_y = b->line_form >= 2 || (b->line_form == 0 && ly >= kWaveFormMinLines);
   if (b->line_form == 1) b->wave_x = b->wave_y = false;
+  b->slab_y = b->wave_y && b->line_form == 3;
   b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);
   for (int k = 0; k < OFF_KINDS; ++k) {
     int acc = 0;
@@ -1213,6 +1227,7 @@ size_t batch_offs_capacity(const hitx_batch* b) {
     cap += (size_t)pl->nblk_stage + walk_blocks(nx, ny, kTuneRbMin, 1) + ((nx + kSTX - 1) / kSTX) * ((ny + kSTY - 1) / kSTY);
     cap += (size_t)ny + nx + (ny + kLineWaves - 1) / kLineWaves + (nx + kLineWaves - 1) / kLineWaves;
     cap += (size_t)((nx + 15) / 16) * ((ny + 15) / 16);
+    cap += (size_t)(((nx + kDBCols - 1) / kDBCols) * ((ny + kDBRows - 1) / kDBRows) + kDBWaves - 1) / kDBWaves;
     if (pl->d.n_force_modes > 0)
       cap += (size_t)walk_blocks(nx, ny, kTuneRbpMin, 0) + (ny * (pl->fkx + 1) + kBlock - 1) / kBlock +
              2 * ((rng_chunks(pl->N) + kRngScatterThreads - 1) / kRngScatterThreads) +
@@ -1244,6 +1259,7 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   I.nblk_sr = pl->nblk_sr;
   I.nblk_proj = pl->nblk_proj;
   I.ntile = ((pl->d.nx + kSTX - 1) / kSTX) * ((pl->d.ny + kSTY - 1) / kSTY);
+  I.use_qT = !(b->wave_y && b->slab_y && line_wave_ok(pl->d.ny));
 }
 
 template <class T>
@@ -1296,11 +1312,19 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
                                                                                   DRAG && b->stage_walk ? 1 : 0);
   if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
   if (b->split_sensor) {
-    if (b->raw_tile)
+    // the tile raw mask writes the transposed state itself, so the dilation
+    // is bit work only; after the column-walk raw mask the byte dilation
+    // writes it (stages 0-2)
+    if (b->raw_tile) {
       b_rawtile<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
-    else
+      b_dilate_bits<T><<<b->total[OFF_DB], 64 * kDBWaves, 0, s>>>(tab, offs, n, st);
+    } else {
       b_rawmask<T><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
-    b_dilate<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+      if (st == 3)
+        b_dilate_bits<T><<<b->total[OFF_DB], 64 * kDBWaves, 0, s>>>(tab, offs, n, st);
+      else
+        b_dilate<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+    }
   } else {
     b_sensor<T><<<b->total[OFF_SENS], kBlock, b->sensor_lds, s>>>(tab, offs, n, st);
   }
@@ -1311,7 +1335,12 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const int sb = (int)b->wave_slice;
   for (int axis = 0; axis < 2; ++axis) {
     const int kb = axis == 0 ? OFF_LX : OFF_LY, kw = axis == 0 ? OFF_LXW : OFF_LYW;
-    if (b->total[kw] > 0) {
+    if (b->total[kw] > 0 && axis == 1 && b->slab_y) {
+      if (b->periodic)
+        b_line_wave_cols<T, true><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
+      else
+        b_line_wave_cols<T, false><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
+    } else if (b->total[kw] > 0) {
       if (b->periodic)
         b_line_wave<T, true><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
       else
@@ -1385,6 +1414,8 @@ int batch_set_lds(hitx_batch* b) {
   HIPCHK(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device));
   if (b->line_lds > (size_t)maxsh) return fail(HITX_EINVAL, "grid line too long for LDS");
   if (b->line_lds > 65536) HIPCHK(allow_line_lds<T>(b->line_lds));
+  if (b->slab_lds > (size_t)maxsh) return fail(HITX_EINVAL, "grid column too long for the slab y-sweep");
+  if (b->slab_lds > 65536) HIPCHK(allow_slab_lds<T>(b->slab_lds));
   return HITX_OK;
 }
 
@@ -1430,6 +1461,10 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     b->proj_lds = std::max(b->proj_lds, proj_lds_bytes(pl));
   }
   b->drag = make_params<double>(b->plans[0]).drag_on;
+  b->slab_lds = (size_t)kSlabWaves * b->wave_slice;
+  for (const hitx_plan* pl : b->plans)
+    if (line_wave_ok(pl->d.ny))
+      b->slab_lds = std::max(b->slab_lds, (size_t)4 * kSlabWaves * slab_cstride(pl->d.ny) * b->esize);
   int rc = b->esize == 8 ? batch_set_lds<double>(b) : batch_set_lds<float>(b);
   if (rc) {
     std::string keep = g_err;
@@ -1520,7 +1555,8 @@ int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, i
   if (!b) return fail(HITX_EINVAL, "null batch");
   if ((rb != 0 && (rb < kTuneRbMin || rb > kTuneRbMax)) || (rbp != 0 && (rbp < kTuneRbpMin || rbp > kTuneRbpMax)))
     return fail(HITX_EINVAL, "tile heights out of range (rb 0 or 2..64, rbp 0 or 8..256)");
-  if (line_form < 0 || line_form > 2) return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block) or 2 (wave)");
+  if (line_form < 0 || line_form > 3)
+    return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block), 2 (wave) or 3 (wave, slab y-sweep)");
   if (sensor_form < 0 || sensor_form > 3)
     return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused), 2 (split, column walk) or 3 (split, tiles)");
   b->line_form = line_form;

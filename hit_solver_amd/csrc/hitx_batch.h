@@ -40,13 +40,14 @@ struct Inst {
   RngJob R;            // forcing noise request: nx*ny normals of the rng stream into noise
   GPtr<double> drows, frows;  // per-row synthesis tables: drag low-pass, forcing
   int nblk_stage, nblk_sr, nblk_proj;
-  int ntile;  // 32x16 tiles of the grid
+  int ntile;   // 32x16 tiles of the grid
+  int use_qT;  // the y-sweep reads the transposed state (block form or wave form without the slab)
 };
 
 // offsets kinds (rows of the offsets table, each ninst+1 ints)
 enum {
   OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_LXW, OFF_LYW,
-  OFF_RNGS, OFF_KINDS
+  OFF_RNGS, OFF_DB, OFF_KINDS
 };
 
 constexpr int kMapLbBits = 20;  // local blocks per instance and kind < 2^20; instances < 2^12
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(kBlock) b_sensor(const Inst<T>* __restrict__ t
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
-  sensor_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.mask, I.maskT, st < 3 ? I.qT : nullptr, bx, by);
+  sensor_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.mask, I.maskT, st < 3 && I.use_qT ? I.qT : nullptr, bx, by);
 }
 
 // Split sensor: undilated mask by column walk, then dilation + transposes.
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(kBlock) b_rawtile(const Inst<T>* __restrict__ 
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
-  rawtile_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, bx, by);
+  rawtile_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, st < 3 && I.use_qT ? I.qT : nullptr, bx, by);
 }
 
 template <class T>
@@ -190,7 +191,24 @@ __global__ void __launch_bounds__(kBlock) b_dilate(const Inst<T>* __restrict__ t
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
-  dilate_body<T>(I.P, stage_in(I, st), st < 3, I.rawm, I.mask, I.maskT, st < 3 ? I.qT : nullptr, bx, by);
+  dilate_body<T>(I.P, stage_in(I, st), st < 3, I.rawm, I.mask, I.maskT, st < 3 && I.use_qT ? I.qT : nullptr, bx,
+                 by);
+}
+
+// Bit dilation: kDBWaves tiles of 64 rows x 32 columns per block (the raw
+// mask and, at stages 0-2 with the tile raw mask, the transposed state are
+// written by the raw-mask tiles).
+template <class T>
+__global__ void __launch_bounds__(64 * kDBWaves) b_dilate_bits(const Inst<T>* __restrict__ tab,
+                                                               const int* __restrict__ offs, int ninst, int st) {
+  __shared__ uint32_t hs[kDBWaves][kDBRows + 16];
+  HITX_POST_STEP_EXIT
+  HITX_BATCH_INST(OFF_DB)
+  if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  const int ntx = (I.P.nx + kDBCols - 1) / kDBCols, nty = (I.P.ny + kDBRows - 1) / kDBRows;
+  const int wv = threadIdx.x >> 6, t = lb * kDBWaves + wv;
+  if (t >= ntx * nty) return;
+  dilate_bits_tile<T>(I.P, I.rawm, I.mask, I.maskT, t % ntx, t / ntx, hs[wv]);
 }
 
 template <class T, bool PER, bool SMALL>
@@ -219,6 +237,30 @@ __global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
                           wsm + (size_t)w * slice_bytes);
   else
     line_wave_any<T, PER>(I.P, I.Ky, 1, 1, I.qT, I.maskT, I.scal, I.dfyT, line, nl, wsm + (size_t)w * slice_bytes);
+}
+
+// Slab form of the y-sweep: kSlabWaves adjacent columns of one instance per
+// block, read from the stage state (enforced on load) through the LDS slab
+// instead of a transposed copy; derivatives go to dfyT as in the other forms.
+template <class T, bool PER>
+__global__ void __launch_bounds__(64 * kSlabWaves) HITX_LINE_WAVE_ATTR(T, PER)
+    b_line_wave_cols(const Inst<T>* __restrict__ tab, const int* __restrict__ offs, int ninst, int st,
+                     int slice_bytes) {
+  extern __shared__ __align__(16) unsigned char wsm[];
+  HITX_BATCH_INST(OFF_LYW)
+  const int nx = I.P.nx, ny = I.P.ny;
+  const int x0 = lb * kSlabWaves, cs = slab_cstride(ny);
+  T* slab = reinterpret_cast<T*>(wsm);
+  slab_fill<T>(stage_in(I, st), nx, ny, x0, slab, cs);
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  const int line = x0 + w;
+  if (line >= nx) {  // no column: meet the sweep's one barrier, then leave
+    __syncthreads();
+    return;
+  }
+  line_wave_slab_any<T, PER>(I.P, I.Ky, slab + w * cs, kSlabWaves * cs, I.maskT, I.scal, I.dfyT, line, nx,
+                             wsm + (size_t)w * slice_bytes);
 }
 
 // update tiles are the sensor's 32x16 tiles (the same block offsets)

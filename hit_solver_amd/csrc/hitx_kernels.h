@@ -677,13 +677,15 @@ __device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __rest
 //     the later kernels of the stage.
 template <class T, bool FIN = false>
 __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
-                                             const double* __restrict__ scal, uint8_t* __restrict__ raw, int bx,
-                                             int by, const double* __restrict__ part = nullptr, int nblk = 0,
+                                             const double* __restrict__ scal, uint8_t* __restrict__ raw,
+                                             T* __restrict__ qT, int bx, int by,
+                                             const double* __restrict__ part = nullptr, int nblk = 0,
                                              int stride = 0, double* __restrict__ scal_out = nullptr,
                                              bool first = false) {
   constexpr int EX = kSTX + 2, EY = kSTY + 2, NE = EX * EY, NIT = (NE + kBlock - 1) / kBlock;
   constexpr int PF = 2;  // partial rows per thread loaded with the tile
   __shared__ T Lr[EX * EY], Lu[EX * EY], Lv[EX * EY], Lp[EX * EY], Le[EX * EY];
+  __shared__ T st[4 * kSTX * (kSTY + 1)];  // the tile's (enforced) state, transposed, when qT is wanted
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x0 = bx * kSTX, y0 = by * kSTY;
   const bool per = P.periodic;
@@ -727,6 +729,14 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
       Lv[t] = pr.v;
       Lp[t] = pr.p;
       Le[t] = dv(pr.p, pr.rho * P.gm1);
+      const int r = t / EX - 1, cc = t - (r + 1) * EX - 1;
+      if (qT != nullptr && r >= 0 && r < kSTY && cc >= 0 && cc < kSTX) {
+        const int o = cc * (kSTY + 1) + r;
+        st[o] = a;
+        st[kSTX * (kSTY + 1) + o] = b;
+        st[2 * kSTX * (kSTY + 1) + o] = c;
+        st[3 * kSTX * (kSTY + 1) + o] = d;
+      }
     }
   }
   if (FIN) {
@@ -757,6 +767,18 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
     scal = sc;
   }
   __syncthreads();
+  if (qT != nullptr) {  // enforced state, transposed for the y-lines: per column 16 consecutive y
+    for (int t = threadIdx.x; t < kSTX * kSTY; t += kBlock) {
+      const int c = t / kSTY, r = t - c * kSTY;
+      const int gy = y0 + r, gx = x0 + c;
+      if (gy < ny && gx < nx) {
+        const int o = c * (kSTY + 1) + r;
+        const size_t g = (size_t)gx * ny + gy;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qT[k * (size_t)N + g] = st[k * kSTX * (kSTY + 1) + o];
+      }
+    }
+  }
   const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
   const bool comp_on = scal[S_COMP_ON] != 0.0, shear_on = scal[S_SHEAR_ON] != 0.0;
   const double div_thr = scal[S_DIV_THR], shear_thr = scal[S_SHEAR_THR], thr = P.jump_thr;
@@ -1026,18 +1048,143 @@ __global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __r
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                         const double* __restrict__ part, int nblk, int stride,
-                                                        double* __restrict__ scal, uint8_t* __restrict__ raw) {
+                                                        double* __restrict__ scal, uint8_t* __restrict__ raw,
+                                                        T* __restrict__ qT) {
   const Tile2 t_ = tile_xcd();
-  rawtile_body<T, true>(P, q, enforce_first, nullptr, raw, t_.bx, t_.by, part, nblk, stride, scal,
+  rawtile_body<T, true>(P, q, enforce_first, nullptr, raw, qT, t_.bx, t_.by, part, nblk, stride, scal,
                         blockIdx.x == 0 && blockIdx.y == 0);
 }
 
+// ---------------------------------------------------------------- bit dilation
+// The (2w+1)^2 OR dilation of the undilated mask on bit words: one wavefront
+// per 64-row x 32-column tile, lane l on row y0 + l.  A row's 32 columns and
+// 8 more on each side become the bits of one word (interior rows read as
+// aligned 8- or 4-byte loads), the horizontal OR is 2w+1 shifts, the
+// vertical OR reads the 2w+1 rows' words from the wave's LDS slots, the mask
+// row is stored as bytes and each maskT column comes from one ballot over the
+// tile's rows.  Out-of-domain nodes wrap (periodic) or are false
+// (non-periodic), as in OOP/spatial_operator.py:169-176 and
+// OOP/problems/riemann_config3.py:149-162.  Undilated mask bytes are 0 or 1.
+constexpr int kDBRows = 64, kDBCols = 32, kDBWaves = 4;
+__device__ __forceinline__ uint32_t gather4(uint32_t d) {  // 4 bytes of 0/1 -> 4 bits
+  return (d & 1u) | ((d >> 7) & 2u) | ((d >> 14) & 4u) | ((d >> 21) & 8u);
+}
+__device__ __forceinline__ uint32_t spread4(uint32_t t) {  // 4 bits -> 4 bytes of 0/1
+  return (t & 1u) | ((t & 2u) << 7) | ((t & 4u) << 14) | ((t & 8u) << 21);
+}
+
+// Horizontally dilated bits of row gy, columns x0 .. x0+31 (bit c = column x0+c).
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_dilate(Params<T> P, const T* __restrict__ q, int enforce_first,
-                                                   const uint8_t* __restrict__ raw, uint8_t* __restrict__ mask,
-                                                   uint8_t* __restrict__ maskT, T* __restrict__ qT) {
-  const Tile2 t_ = tile_xcd();
-  dilate_body<T>(P, q, enforce_first, raw, mask, maskT, qT, t_.bx, t_.by);
+__device__ __forceinline__ uint32_t dilate_row_bits(const Params<T>& P, const uint8_t* __restrict__ raw, int gy,
+                                                    int x0) {
+  const int nx = P.nx, ny = P.ny, w = P.width;
+  if (P.periodic)
+    gy = wrapn(gy, ny);
+  else if (gy < 0 || gy >= ny)
+    return 0u;
+  const uint8_t* r = raw + (size_t)gy * nx;
+  uint64_t bits = 0;  // bit k: column x0 - 8 + k, k < 48
+  // aligned groups of 8 (or 4) columns never straddle the grid's end when nx
+  // is a multiple of 8 (or 4): each group is one load from its wrapped start,
+  // or zero outside a non-periodic grid
+  auto group = [&](int g0, bool& in) {
+    in = true;
+    if (P.periodic)
+      g0 = wrapn(g0, nx);
+    else
+      in = g0 >= 0 && g0 < nx;
+    return in ? g0 : 0;
+  };
+  if ((nx & 7) == 0) {
+    uint2 d[6];
+    bool in[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = *reinterpret_cast<const uint2*>(r + group(x0 - 8 + 8 * k, in[k]));
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (in[k]) bits |= (uint64_t)(gather4(d[k].x) | (gather4(d[k].y) << 4)) << (8 * k);
+  } else if ((nx & 3) == 0) {
+    uint32_t d[12];
+    bool in[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) d[k] = *reinterpret_cast<const uint32_t*>(r + group(x0 - 8 + 4 * k, in[k]));
+#pragma unroll
+    for (int k = 0; k < 12; ++k)
+      if (in[k]) bits |= (uint64_t)gather4(d[k]) << (4 * k);
+  } else {
+    for (int k = 0; k < 48; ++k) {
+      int gx = x0 - 8 + k;
+      bool in = true;
+      if (P.periodic)
+        gx = wrapn(gx, nx);
+      else
+        in = gx >= 0 && gx < nx;
+      if (in && r[gx] != 0) bits |= (uint64_t)1 << k;
+    }
+  }
+  uint32_t h = 0;
+  for (int s = -w; s <= w; ++s) h |= (uint32_t)(bits >> (8 + s));
+  return h;
+}
+
+// One tile (tx, ty) by the calling wavefront; hs: its kDBRows + 16 LDS words.
+template <class T>
+__device__ __forceinline__ void dilate_bits_tile(const Params<T>& P, const uint8_t* __restrict__ raw,
+                                                 uint8_t* __restrict__ mask, uint8_t* __restrict__ maskT, int tx,
+                                                 int ty, uint32_t* __restrict__ hs) {
+  const int nx = P.nx, ny = P.ny, w = P.width;
+  const int lane = threadIdx.x & 63;
+  const int x0 = tx * kDBCols, y0 = ty * kDBRows;
+  // slot i holds row y0 - w + i: own rows at w .. w+63, halo rows above and below
+  hs[w + lane] = dilate_row_bits(P, raw, y0 + lane, x0);
+  if (lane < 2 * w) {
+    const int slot = lane < w ? lane : kDBRows + lane;
+    const int gy = lane < w ? y0 - w + lane : y0 + kDBRows + (lane - w);
+    hs[slot] = dilate_row_bits(P, raw, gy, x0);
+  }
+  wave_sync();
+  uint32_t v = 0;
+  for (int d = 0; d <= 2 * w; ++d) v |= hs[lane + d];
+  const int ncol = nx - x0 < kDBCols ? nx - x0 : kDBCols, nrow = ny - y0 < kDBRows ? ny - y0 : kDBRows;
+  if (ncol < 32) v &= (1u << ncol) - 1u;
+  if (lane >= nrow) v = 0u;
+  if (lane < nrow) {
+    uint8_t* o = mask + (size_t)(y0 + lane) * nx + x0;
+    if (ncol == kDBCols && (nx & 3) == 0) {
+      uint32_t* o4 = reinterpret_cast<uint32_t*>(o);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) o4[k] = spread4((v >> (4 * k)) & 15u);
+    } else {
+      for (int k = 0; k < ncol; ++k) o[k] = (uint8_t)((v >> k) & 1u);
+    }
+  }
+  uint64_t col = 0;  // lane c < 32: column x0 + c over the tile's rows
+#pragma unroll
+  for (int c = 0; c < kDBCols; ++c) {
+    const uint64_t b = __ballot((v >> c) & 1u);
+    col = lane == c ? b : col;
+  }
+  if (lane < ncol) {
+    uint8_t* o = maskT + (size_t)(x0 + lane) * ny + y0;
+    if (nrow == kDBRows && (ny & 3) == 0) {
+      uint32_t* o4 = reinterpret_cast<uint32_t*>(o);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) o4[k] = spread4((uint32_t)(col >> (4 * k)) & 15u);
+    } else {
+      for (int k = 0; k < nrow; ++k) o[k] = (uint8_t)((col >> k) & 1u);
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(64 * kDBWaves) k_dilate_bits(Params<T> P, const uint8_t* __restrict__ raw,
+                                                               uint8_t* __restrict__ mask,
+                                                               uint8_t* __restrict__ maskT) {
+  __shared__ uint32_t hs[kDBWaves][kDBRows + 16];
+  const int ntx = (P.nx + kDBCols - 1) / kDBCols, nty = (P.ny + kDBRows - 1) / kDBRows;
+  const int wv = threadIdx.x >> 6, t = (int)blockIdx.x * kDBWaves + wv;
+  if (t >= ntx * nty) return;
+  dilate_bits_tile<T>(P, raw, mask, maskT, t % ntx, t / ntx, hs[wv]);
 }
 
 template <class T>

@@ -40,11 +40,28 @@ __host__ __device__ inline size_t line_wave_lds(int n, int esize) {
   return (size_t)(2 * line_row(n) + (n + 1 + 2 * kLineHalo)) * esize + (size_t)(n + 1) * 2 + (size_t)(n + 4) + 16;
 }
 
-template <class T, bool PER, int L>
+// Column slab (SLAB): a block of waves sweeps adjacent y-lines (grid
+// columns) of the row-major state.  The block first copies the columns from
+// HBM in row segments (coalesced) into an LDS slab, component c of column w
+// at slab[(c * waves + w) * cstride + slab_pos(i)]; each wave then reads its
+// column from the slab instead of HBM.  The wave slices overlay the slab, so
+// the waves meet at one block barrier after reading it (every wave of the
+// block, also one without a column, must reach that barrier once).
+template <int L>
+__host__ __device__ constexpr int slab_pos(int i) {
+  return i + line_pad(L) * (i / (L > 0 ? L : 1));
+}
+__host__ __device__ inline int slab_cstride(int n) {  // odd, so the block's copy-in spreads over the banks
+  const int L = n / 64;
+  const int last = n - 1 + line_pad(L) * ((n - 1) / (L > 0 ? L : 1));
+  return (last + 1) | 1;
+}
+
+template <class T, bool PER, int L, bool SLAB = false>
 __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K, int axis, int enforced_input,
                                           const T* __restrict__ ql, const uint8_t* __restrict__ ml,
                                           const double* __restrict__ scal, T* __restrict__ out, int line, int nl,
-                                          unsigned char* slice) {
+                                          unsigned char* slice, int slab_cs = 0) {
   const int n = 64 * L, S = n + 1 + 2 * kLineHalo, R = line_row(n);
   constexpr int PAD = line_pad(L);
   const int lane = threadIdx.x & 63;
@@ -65,7 +82,19 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
   const size_t g0 = (size_t)line * n + lane * L;
 #pragma unroll
   for (int j = 0; j < L; ++j) {
-    T a = ql[g0 + j], b = ql[NL + g0 + j], c = ql[2 * NL + g0 + j], d = ql[3 * NL + g0 + j];
+    T a, b, c, d;
+    if (SLAB) {  // the wave's column in the slab (ql), components slab_cs apart
+      const int o = slab_pos<L>(lane * L + j);
+      a = ql[o];
+      b = ql[slab_cs + o];
+      c = ql[2 * slab_cs + o];
+      d = ql[3 * slab_cs + o];
+    } else {
+      a = ql[g0 + j];
+      b = ql[NL + g0 + j];
+      c = ql[2 * NL + g0 + j];
+      d = ql[3 * NL + g0 + j];
+    }
     if (!enforced_input) enforce(P, a, b, c, d);
     const Prim<T> pr = prim(P, a, b, c, d);
     q[0][j] = a;
@@ -83,7 +112,12 @@ __device__ __forceinline__ void line_wave(const Params<T>& P, const LineK<T>& K,
       f[2][j] = pr.rho * (pr.v * pr.v) + pr.p;
       f[3][j] = (d + pr.p) * pr.v;
     }
-    Mk[lane * L + j] = ml[g0 + j];
+    if (!SLAB) Mk[lane * L + j] = ml[g0 + j];
+  }
+  if (SLAB) {
+    __syncthreads();  // every wave has read the slab: the slices may be written
+#pragma unroll
+    for (int j = 0; j < L; ++j) Mk[lane * L + j] = ml[g0 + j];
   }
   wave_sync();
   if (lane < 2) {  // mask halo: wrapped (periodic) or clamped copies
@@ -334,6 +368,60 @@ __device__ __forceinline__ void line_wave_any(const Params<T>& P, const LineK<T>
     case 6: line_wave<T, PER, 6>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
     case 7: line_wave<T, PER, 7>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
     default: line_wave<T, PER, 8>(P, K, axis, enforced_input, ql, ml, scal, out, line, nl, slice); break;
+  }
+}
+
+// Slab form of the y-sweep (one column per wave, kSlabWaves columns per
+// block); the caller has filled the slab and passed its wave's column.
+template <class T, bool PER>
+__device__ __forceinline__ void line_wave_slab_any(const Params<T>& P, const LineK<T>& K, const T* __restrict__ col,
+                                                   int cs, const uint8_t* __restrict__ ml,
+                                                   const double* __restrict__ scal, T* __restrict__ out, int line,
+                                                   int nl, unsigned char* slice) {
+  switch (K.n >> 6) {
+    case 1: line_wave<T, PER, 1, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    case 2: line_wave<T, PER, 2, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    case 3: line_wave<T, PER, 3, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    case 4: line_wave<T, PER, 4, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    case 5: line_wave<T, PER, 5, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    case 6: line_wave<T, PER, 6, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    case 7: line_wave<T, PER, 7, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+    default: line_wave<T, PER, 8, true>(P, K, 1, 0, col, ml, scal, out, line, nl, slice, cs); break;
+  }
+}
+
+constexpr int kSlabWaves = 8;  // columns (wavefronts) per block of the slab y-sweep
+
+// Copy columns x0 .. x0 + kSlabWaves - 1 of the row-major state (all rows,
+// four components) into the slab; columns beyond nx are left unwritten.
+template <class T>
+__device__ __forceinline__ void slab_fill(const T* __restrict__ q, int nx, int ny, int x0, T* __restrict__ slab,
+                                          int cs) {
+  const size_t N = (size_t)nx * ny;
+  const int L = ny >> 6, pad = line_pad(L);
+  const int total = ny * kSlabWaves;
+  constexpr int U = 4;  // rows in flight per thread
+  for (int base = (int)threadIdx.x; base < total; base += U * (int)blockDim.x) {
+    T v[U][4];
+    int pos[U], w[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = base + u * (int)blockDim.x;
+      const int y = idx / kSlabWaves;
+      w[u] = idx - y * kSlabWaves;
+      ok[u] = idx < total && x0 + w[u] < nx;
+      pos[u] = y + pad * (y / L);
+      const size_t g = (size_t)y * nx + x0 + w[u];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[u][c] = ok[u] ? q[c * N + g] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (ok[u]) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) slab[(c * kSlabWaves + w[u]) * cs + pos[u]] = v[u][c];
+      }
   }
 }
 

@@ -232,3 +232,36 @@ def test_hit_batch_walk_and_tile_drag_reductions_agree():
         one.run()
         for k in range(4):
             assert rel_l2(host(big.q[i])[k], host(one.q[0])[k]) <= 1e-12, (i, k)
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp64"])
+def test_slab_y_sweep_bitexact(dtype):
+    """Line form 3 reads the y-lines from the row-major state through an LDS
+    slab (enforced on load) instead of the transposed copy: the same
+    arithmetic, so the batch is bit-identical to line form 2.  Grids cover
+    y-lines of 1, 3, 4 and 7 wavefront chunks (L = ny/64) and column counts
+    that leave the last slab block partly empty; one instance's x-lines are
+    not a multiple of 64 (block form for x)."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannConfig6
+
+    kw = dict(cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0, viscosity=7.5e-4,
+              mach_control=True, large_scale_drag=0.10, cooling_time=5.0, tfinal=1.0, max_steps=3,
+              hyperviscosity_interval=2, dtype=dtype)
+    hit = [HIT2DConfig(nx=100, ny=64, target_mach=0.25, forcing_seed=1, **kw),
+           HIT2DConfig(nx=192, ny=192, target_mach=0.1, forcing_seed=2, **kw),
+           HIT2DConfig(nx=68, ny=256, target_mach=0.4, forcing_seed=3, **kw),
+           HIT2DConfig(nx=448, ny=448, target_mach=0.25, forcing_seed=4, **kw)]
+    rie = [RiemannConfig3(nx=100, ny=128, shear_threshold=None, progress_every=10**9, dtype=dtype),
+           RiemannConfig6(nx=192, ny=320, shear_threshold=None, progress_every=10**9, dtype=dtype)]
+    for cfgs in (hit, rie):
+        print(f"[batch] slab y-sweep vs wave form, {len(cfgs)} instances {dtype}", flush=True)
+        a, b = BatchRunner(cfgs, max_steps=3), BatchRunner(cfgs, max_steps=3)
+        a.tune(4, 16, 2, 3)
+        b.tune(4, 16, 3, 3)
+        a.run()
+        b.run()
+        for i in range(len(cfgs)):
+            np.testing.assert_array_equal(host(b.q[i]), host(a.q[i]), err_msg=f"instance {i}")
+        np.testing.assert_array_equal(b.control(), a.control())
