@@ -81,6 +81,14 @@ SIGNATURES = {
     "hitx_set_sensor_form": (c_i32, [c_vp, c_i32]),
     "hitx_normals_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
     "hitx_normals": (c_i32, [c_vp, c_vp, ctypes.c_int64, c_i32, c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_post_workspace_bytes": (ctypes.c_size_t, [c_i32, c_i32, c_i32]),
+    "hitx_post_fields": (c_i32, [c_i32, c_i32, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp,
+                                 c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_dft2": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, ctypes.c_size_t, c_vp]),
+    "hitx_shell_spectra": (c_i32, [c_i32, c_i32, c_dbl, c_dbl, c_vp, c_i32, c_vp, c_vp, c_vp, ctypes.c_size_t,
+                                   c_vp]),
+    "hitx_correlations": (c_i32, [c_i32, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t,
+                                  c_vp]),
     "hitx_batch_create": (c_i32, [ctypes.POINTER(HitxDesc), c_vp, c_i32, ctypes.POINTER(c_vp)]),
     "hitx_batch_destroy": (c_i32, [c_vp]),
     "hitx_batch_workspace_bytes": (ctypes.c_size_t, [c_vp]),

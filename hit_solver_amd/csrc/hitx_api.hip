@@ -15,6 +15,7 @@
 #include "hitx_diag.h"
 #include "hitx_init.h"
 #include "hitx_line.h"
+#include "hitx_post.h"
 
 using namespace hitx;
 
@@ -205,6 +206,7 @@ Params<T> make_params(const hitx_plan* pl) {
   P.twy = pl->d_tw ? pl->d_tw + (size_t)(2 * pl->kmx + 1) * d.nx * 2 : nullptr;
   P.drag_modes = pl->d_modes;
   P.force_modes = pl->d_modes ? pl->d_modes + 2 * d.n_drag_modes : nullptr;
+  P.force_bykx = pl->d_modes ? pl->d_modes + 2 * d.n_drag_modes + 2 * d.n_force_modes : nullptr;
   P.rb = pl->rb;
   P.rbp = pl->rbp;
   P.dkx = pl->dkx;
@@ -355,15 +357,21 @@ bool use_split_sensor(const hitx_plan* pl) {
   return (long long)pl->N >= kSplitSensorMinNodes;
 }
 
+// Every raw-mask tile re-reducing the stage partials costs tiles x rows:
+// above this many partial rows the finalize runs once in its own launch.
+constexpr int kFoldMaxRows = 1024;
+
 template <class T>
 int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, T* qT, const WS& w, hipStream_t s) {
   const Params<T> P = make_params<T>(pl);
-  // the split sensor's raw-mask tiles (and the drag rows) finalize the
-  // reduction themselves; the fused sensor needs the separate finalize
-  const bool fold = use_split_sensor(pl);
   const dim3 g((pl->d.nx + kSTX - 1) / kSTX, (pl->d.ny + kSTY - 1) / kSTY);
   const bool walk = P.drag_on && stage_walk_form((long long)g.x * g.y);
   const int nblk = walk ? pl->nblk_sr : (int)(g.x * g.y), stride = P.drag_on ? R_DRAG + kDragCols : R_DRAG;
+  // the split sensor's raw-mask tiles (and the drag rows) finalize the
+  // reduction themselves while the partial rows are few; otherwise, and for
+  // the fused sensor, the finalize has its own launch
+  const bool split = use_split_sensor(pl);
+  const bool fold = split && nblk <= kFoldMaxRows;
   if (P.drag_on) {
     if (walk)
       k_stage_reduce<T, true><<<pl->nblk_sr, kBlock, 0, s>>>(P, q, enforce_first, w.part);
@@ -379,8 +387,11 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
     k_stage_tile<T, false><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part);
     if (!fold) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, nblk, w.scal);
   }
-  if (fold) {
-    k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw, qT);
+  if (split) {
+    if (fold)
+      k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw, qT);
+    else
+      k_rawtile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw, qT);
     const int ntiles = ((pl->d.nx + kDBCols - 1) / kDBCols) * ((pl->d.ny + kDBRows - 1) / kDBRows);
     k_dilate_bits<T><<<(ntiles + kDBWaves - 1) / kDBWaves, 64 * kDBWaves, 0, s>>>(P, w.raw, mask_out, w.maskT);
   } else {
@@ -569,6 +580,19 @@ int hitx_plan_create(const hitx_desc* desc, hitx_plan** out) {
   }
   pl->kmx = km_x;
   pl->kmy = km_y;
+  // forcing modes grouped by |kx| (ascending mode index within a group), then
+  // the group offsets: the per-row synthesis table visits only its own modes
+  {
+    const int nf = desc->n_force_modes;
+    std::vector<int32_t> order, ofs(1, 0);
+    for (int k = 0; k <= pl->fkx; ++k) {
+      for (int m = 0; m < nf; ++m)
+        if (std::abs(pl->force_modes[2 * m]) == k) order.push_back(m);
+      ofs.push_back((int32_t)order.size());
+    }
+    all.insert(all.end(), order.begin(), order.end());
+    all.insert(all.end(), ofs.begin(), ofs.end());
+  }
   if (desc->n_drag_modes + desc->n_force_modes > 0) {
     const int nx = desc->nx, ny = desc->ny;
     std::vector<double> tw((size_t)(2 * km_x + 1) * nx * 2 + (size_t)(2 * km_y + 1) * ny * 2);
@@ -1056,6 +1080,178 @@ int hitx_viscous_rhs(hitx_plan* pl, const void* q, void* out, void* stream) {
                                                   nullptr, nullptr, nullptr, nullptr, nullptr, 0.0f,
                                                   static_cast<float*>(out));
   }
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+// ---------------------------------------------------------------- post-processing
+}  // extern "C"
+
+namespace {
+
+constexpr int kPostMaxN = 2048;  // line length of the LDS-staged transforms
+constexpr int kPostRedBlocks = 256;
+
+struct PostWS {
+  double *twx, *twy, *mid, *part, *means, *hist;
+};
+// Workspace of the post-processing calls on an nx x ny grid with `planes`
+// complex intermediate planes: the two twiddle tables, the planes, the mean
+// partials, the shell row partials.
+size_t post_layout(int nx, int ny, int planes, void* base, PostWS* w) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o = align256(o + bytes);
+    return r;
+  };
+  const size_t tx = take(2 * (size_t)nx * 8), ty = take(2 * (size_t)ny * 8);
+  const size_t mid = take((size_t)planes * nx * ny * 16);
+  const size_t part = take((size_t)kPostRedBlocks * PM_COUNT * 8), means = take(PM_COUNT * 8);
+  const size_t hist = take((size_t)ny * 2 * (std::min(nx, ny) / 2 + 1) * 8);
+  if (w && base) {
+    uint8_t* b = static_cast<uint8_t*>(base);
+    w->twx = reinterpret_cast<double*>(b + tx);
+    w->twy = reinterpret_cast<double*>(b + ty);
+    w->mid = reinterpret_cast<double*>(b + mid);
+    w->part = reinterpret_cast<double*>(b + part);
+    w->means = reinterpret_cast<double*>(b + means);
+    w->hist = reinterpret_cast<double*>(b + hist);
+  }
+  return o;
+}
+
+int post_check(int nx, int ny, int planes, void* ws, size_t ws_bytes) {
+  if (nx < 2 || ny < 2 || nx > kPostMaxN || ny > kPostMaxN)
+    return fail(HITX_EINVAL, "post-processing grids are 2..2048 nodes a side");
+  if (planes < 1) return fail(HITX_EINVAL, "at least one plane");
+  if (!ws || ws_bytes < post_layout(nx, ny, planes, nullptr, nullptr))
+    return fail(HITX_ENOMEM, "post-processing workspace too small: need " +
+                                 std::to_string(post_layout(nx, ny, planes, nullptr, nullptr)) + " bytes");
+  return HITX_OK;
+}
+
+// 2-D transform of count planes (in: real or complex, out: complex) through
+// the intermediate planes w.mid (count of them).
+int post_dft2(int nx, int ny, int count, const double* in, bool in_complex, bool inverse, double* out,
+              const PostWS& w, hipStream_t s) {
+  k_dft_table<<<(nx + kPostBlock - 1) / kPostBlock, kPostBlock, 0, s>>>(nx, w.twx);
+  k_dft_table<<<(ny + kPostBlock - 1) / kPostBlock, kPostBlock, 0, s>>>(ny, w.twy);
+  const int sign = inverse ? 1 : -1;
+  const double scale = inverse ? 1.0 / ((double)nx * (double)ny) : 1.0;
+  const size_t N = (size_t)nx * ny;
+  // rows (lines along x) of every plane: line r = plane * ny + y
+  if (in_complex)
+    k_dft_lines<false><<<count * ny, kDftThreads, 32 * (size_t)nx, s>>>(nx, count * ny, in, 2 * (long long)nx, 1,
+                                                                        w.mid, 2 * (long long)nx, 1, w.twx, sign, 1.0);
+  else
+    k_dft_lines<true><<<count * ny, kDftThreads, 32 * (size_t)nx, s>>>(nx, count * ny, in, nx, 1, w.mid,
+                                                                       2 * (long long)nx, 1, w.twx, sign, 1.0);
+  // columns (lines along y), plane by plane
+  for (int f = 0; f < count; ++f)
+    k_dft_lines<false><<<nx, kDftThreads, 32 * (size_t)ny, s>>>(ny, nx, w.mid + f * N * 2, 2, nx, out + f * N * 2, 2,
+                                                                nx, w.twy, sign, scale);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t hitx_post_workspace_bytes(int32_t nx, int32_t ny, int32_t planes) {
+  if (nx < 2 || ny < 2 || planes < 1) return 0;
+  return post_layout(nx, ny, planes, nullptr, nullptr);
+}
+
+int hitx_post_fields(int32_t nx, int32_t ny, double dx, double dy, const double* rho, const double* u,
+                     const double* v, const double* p, const double* omega, const double* theta, int32_t favre,
+                     double* out, void* ws, size_t ws_bytes, void* stream) {
+  int rc = post_check(nx, ny, 1, ws, ws_bytes);
+  if (rc) return rc;
+  if (!rho || !u || !v || !out) return fail(HITX_EINVAL, "null array");
+  if ((omega == nullptr) != (theta == nullptr)) return fail(HITX_EINVAL, "give both omega and theta, or neither");
+  if (!(dx > 0.0) || !(dy > 0.0)) return fail(HITX_EINVAL, "dx and dy must be positive");
+  PostWS w;
+  post_layout(nx, ny, 1, ws, &w);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int N = nx * ny;
+  if (!omega) {  // central differences into the omega' and theta' planes, fluctuations taken in place below
+    double* vo = out + 6 * (size_t)N;
+    double* th = out + 7 * (size_t)N;
+    k_post_divvort<<<std::min(1024, (N + kPostBlock - 1) / kPostBlock), kPostBlock, 0, s>>>(nx, ny, dx, dy, u, v, th,
+                                                                                            vo);
+    omega = vo;
+    theta = th;
+  }
+  const int nb = std::min(kPostRedBlocks, (N + kPostBlock - 1) / kPostBlock);
+  k_post_partials<<<nb, kPostBlock, 0, s>>>(N, rho, u, v, p, omega, theta, w.part);
+  k_post_means<<<1, kPostBlock, 0, s>>>(nb, N, w.part, w.means);
+  k_post_fields<<<std::min(1024, (N + kPostBlock - 1) / kPostBlock), kPostBlock, 0, s>>>(N, favre, rho, u, v, p, omega,
+                                                                                         theta, w.means, out);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+int hitx_dft2(int32_t nx, int32_t ny, int32_t count, const double* in, int32_t in_complex, int32_t inverse,
+              double* out, void* ws, size_t ws_bytes, void* stream) {
+  int rc = post_check(nx, ny, count, ws, ws_bytes);
+  if (rc) return rc;
+  if (!in || !out) return fail(HITX_EINVAL, "null array");
+  PostWS w;
+  post_layout(nx, ny, count, ws, &w);
+  return post_dft2(nx, ny, count, in, in_complex != 0, inverse != 0, out, w, static_cast<hipStream_t>(stream));
+}
+
+int hitx_shell_spectra(int32_t nx, int32_t ny, double dx, double dy, const double* hat, int32_t nshell,
+                       double* energy, double* enstrophy, void* ws, size_t ws_bytes, void* stream) {
+  int rc = post_check(nx, ny, 1, ws, ws_bytes);
+  if (rc) return rc;
+  if (!hat || !energy || !enstrophy) return fail(HITX_EINVAL, "null array");
+  if (nshell < 1 || nshell > std::min(nx, ny) / 2 + 1) return fail(HITX_EINVAL, "nshell must be 1..min(nx, ny)/2 + 1");
+  if (!(dx > 0.0) || !(dy > 0.0)) return fail(HITX_EINVAL, "dx and dy must be positive");
+  PostWS w;
+  post_layout(nx, ny, 1, ws, &w);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  k_shell_rows<<<ny, kPostBlock, 20 * (size_t)nx, s>>>(nx, ny, dx, dy, nshell, hat, w.hist);
+  k_shell_sum<<<(nshell + kPostBlock - 1) / kPostBlock, kPostBlock, 0, s>>>(ny, nshell, w.hist, energy, enstrophy);
+  LAUNCHCHK();
+  return HITX_OK;
+}
+
+int hitx_correlations(int32_t nx, int32_t ny, int32_t nmaps, const double* hat, int32_t nplanes,
+                      const int32_t* pairs_host, const int32_t* norm_host, double* out, void* ws, size_t ws_bytes,
+                      void* stream) {
+  if (nmaps < 1 || nmaps > kCorrMaxMaps) return fail(HITX_EINVAL, "1..16 correlation maps per call");
+  int rc = post_check(nx, ny, 3 * nmaps, ws, ws_bytes);
+  if (rc) return rc;
+  if (!hat || !out || !pairs_host || !norm_host) return fail(HITX_EINVAL, "null array");
+  CorrMaps M{};
+  M.n = nmaps;
+  for (int m = 0; m < nmaps; ++m) {
+    M.pair[m][0] = pairs_host[2 * m];
+    M.pair[m][1] = pairs_host[2 * m + 1];
+    M.mode[m] = norm_host[3 * m];
+    M.ref[m][0] = norm_host[3 * m + 1];
+    M.ref[m][1] = norm_host[3 * m + 2];
+    if (M.pair[m][0] < 0 || M.pair[m][0] >= nplanes || M.pair[m][1] < 0 || M.pair[m][1] >= nplanes)
+      return fail(HITX_EINVAL, "correlation pair names a plane beyond nplanes");
+    if (M.mode[m] < 0 || M.mode[m] > 2) return fail(HITX_EINVAL, "normalisation must be 0, 1 or 2");
+    if (M.mode[m] == 2 && (M.ref[m][0] < 0 || M.ref[m][0] >= nmaps || M.ref[m][1] < 0 || M.ref[m][1] >= nmaps))
+      return fail(HITX_EINVAL, "normalisation reference beyond the call's maps");
+  }
+  PostWS w;
+  post_layout(nx, ny, 3 * nmaps, ws, &w);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t N = (size_t)nx * ny;
+  double* prod = w.mid + 2 * (size_t)nmaps * N * 2;  // the last nmaps planes; the first 2 nmaps take the transform
+  double* inv = w.mid + (size_t)nmaps * N * 2;
+  const int nb = (int)std::min<size_t>(4096, (N * nmaps + kPostBlock - 1) / kPostBlock);
+  k_cross_spectra<<<nb, kPostBlock, 0, s>>>((int)N, M, hat, prod);
+  PostWS wt = w;  // intermediate planes of the inverse transform: the first nmaps
+  rc = post_dft2(nx, ny, nmaps, prod, true, true, inv, wt, s);
+  if (rc) return rc;
+  k_corr_finish<<<nb, kPostBlock, 0, s>>>((int)N, M, inv, out);
   LAUNCHCHK();
   return HITX_OK;
 }

@@ -68,6 +68,7 @@ struct Params {
   GPtr<const double> twy;             // [(2*kmy+1)][ny][2]
   GPtr<const int> drag_modes;         // [ndrag][2]
   GPtr<const int> force_modes;        // [nforce][2]
+  GPtr<const int> force_bykx;         // [nforce] mode indices grouped by |kx|, then [fkx+2] group offsets
   // separable mode sums (column walks and per-row tables)
   int rb, rbp;                        // rows per wave: stage reductions, forcing projection
   int dkx, dky;                       // max |kx|, |ky| of the drag modes (<= 2)

@@ -1055,6 +1055,17 @@ __global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __
                         blockIdx.x == 0 && blockIdx.y == 0);
 }
 
+// Raw-mask tiles that read the stage scalars written by the finalize kernel
+// (single plans with many partial rows, where re-reducing them in every tile
+// would cost more than the separate finalize).
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_rawtile(Params<T> P, const T* __restrict__ q, int enforce_first,
+                                                    const double* __restrict__ scal, uint8_t* __restrict__ raw,
+                                                    T* __restrict__ qT) {
+  const Tile2 t_ = tile_xcd();
+  rawtile_body<T>(P, q, enforce_first, scal, raw, qT, t_.bx, t_.by);
+}
+
 // ---------------------------------------------------------------- bit dilation
 // The (2w+1)^2 OR dilation of the undilated mask on bit words: one wavefront
 // per 64-row x 32-column tile, lane l on row y0 + l.  A row's 32 columns and
@@ -2200,10 +2211,12 @@ __device__ __forceinline__ void frows_body(const Params<T>& P, const double* __r
   if (t >= P.ny * kx1) return;
   const int y = t / kx1, k = t - y * kx1;
   double hr[2] = {0.0, 0.0}, hs[2] = {0.0, 0.0};
+  const int* grp = P.force_bykx;  // the modes with |kx| = k, in mode order
+  const int q0 = grp[nf + k], q1 = grp[nf + k + 1];
 #pragma unroll 4
-  for (int m = 0; m < nf; ++m) {
+  for (int qi = q0; qi < q1; ++qi) {
+    const int m = grp[qi];
     const int kx = P.force_modes[2 * m], ky = P.force_modes[2 * m + 1];
-    if (kx != k && kx != -k) continue;
     const double* ty = P.twy + ((size_t)(ky + P.kmy) * P.ny + y) * 2;
     const double cy = ty[0], sy = ty[1];
 #pragma unroll

@@ -329,6 +329,50 @@ int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
 #define HITX_CTL 32
 enum { HITX_BUF_STATE = 0, HITX_BUF_NOISE, HITX_BUF_CTL, HITX_BUF_PHI, HITX_BUF_RNG, HITX_BUF_FX, HITX_BUF_FY };
 
+/* ---------------------------------------------------------------- post-processing
+ * Shell spectra and two-point correlations of snapshot planes (double,
+ * [ny][nx], x fastest), on grids of 2..2048 nodes a side.  No plan: sizes
+ * are arguments; ws is caller device memory of hitx_post_workspace_bytes(nx,
+ * ny, planes) bytes, planes as each call states.  Complex planes are
+ * interleaved (re, im) doubles [ny][nx][2].
+ *
+ * hitx_post_fields (planes 1): out[8][ny][nx] = sqrt(rho) u', sqrt(rho) v',
+ *   u', v', rho', p', omega', theta' with u', v' the Reynolds (favre = 0) or
+ *   Favre fluctuations; p may be null (its plane is not written); omega and
+ *   theta both given, or both null: then the second-order central vorticity
+ *   and divergence of u, v on the periodic grid.  Replaces
+ *   HIT2DSpectra._fluctuations (OOP/postprocess/spectra.py:203-214),
+ *   TwoPointCorrelation2D._velocity_fluctuations, _divergence_and_vorticity
+ *   and the mean removals of compute (OOP/postprocess/
+ *   two_point_correlation.py:406-422, 472-485, 190-204).
+ * hitx_dft2 (planes count): numpy fft2 (inverse = 0) or ifft2 (inverse = 1,
+ *   scaled by 1/(nx ny)) of count planes, real (in_complex = 0) or complex
+ *   input, complex output (spectra.py:99-105; two_point_correlation.py:111,
+ *   131-133).
+ * hitx_shell_spectra (planes 1): from the transforms of the first four
+ *   hitx_post_fields planes ([4] complex planes), the energy 0.5 (|w_u|^2 +
+ *   |w_v|^2) / (nx ny)^2 and enstrophy 0.5 |kx v - ky u|^2 / (nx ny)^2 summed
+ *   over shells rint(|k| / dk), nshell = min(nx, ny)/2 + 1 shells
+ *   (spectra.py:97-126).
+ * hitx_correlations (planes 3 nmaps): for map m < nmaps <= 16, the real part
+ *   of ifft2(hat[a_m] conj(hat[b_m])) / (nx ny) with (a_m, b_m) =
+ *   pairs_host[2m..2m+1] planes of hat (nplanes of them), normalised by
+ *   norm_host[3m]: 0 none, 1 its own zero lag (autocorrelation), 2
+ *   sqrt(R(0) R'(0)) of the maps norm_host[3m+1], norm_host[3m+2] of this call
+ *   (cross-correlation); out[nmaps][ny][nx] (two_point_correlation.py:107-139).
+ */
+size_t hitx_post_workspace_bytes(int32_t nx, int32_t ny, int32_t planes);
+int hitx_post_fields(int32_t nx, int32_t ny, double dx, double dy, const double* rho, const double* u,
+                     const double* v, const double* p, const double* omega, const double* theta, int32_t favre,
+                     double* out, void* ws, size_t ws_bytes, void* stream);
+int hitx_dft2(int32_t nx, int32_t ny, int32_t count, const double* in, int32_t in_complex, int32_t inverse,
+              double* out, void* ws, size_t ws_bytes, void* stream);
+int hitx_shell_spectra(int32_t nx, int32_t ny, double dx, double dy, const double* hat, int32_t nshell,
+                       double* energy, double* enstrophy, void* ws, size_t ws_bytes, void* stream);
+int hitx_correlations(int32_t nx, int32_t ny, int32_t nmaps, const double* hat, int32_t nplanes,
+                      const int32_t* pairs_host, const int32_t* norm_host, double* out, void* ws,
+                      size_t ws_bytes, void* stream);
+
 typedef struct hitx_batch hitx_batch;
 int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hitx_batch** out);
 int hitx_batch_destroy(hitx_batch* batch);
