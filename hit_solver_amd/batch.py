@@ -323,12 +323,22 @@ class BatchRunner:
         check(lib().hitx_batch_stage_form(self.handle, int(form)))
         self._graph = None
 
+    def timing(self, on: bool):
+        """HIP events around every stage-update launch of eager steps."""
+        check(lib().hitx_batch_timing_enable(self.handle, int(on)))
+
+    def timing_read(self):
+        """(summed stage-update milliseconds, launches) since the last read."""
+        ms, n = ctypes.c_double(), ctypes.c_int32()
+        check(lib().hitx_batch_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def autotune(self, candidates=None, steps: int = 3):
         """Time a few steps per candidate (rb, rbp, line_form, sensor_form)
         on a copy of the batch's device state, restore the state and keep the
         fastest candidate.  Returns (best, {candidate: ms per step})."""
         if candidates is None:
-            candidates = [(rb, rbp, lf, sf) for lf in (1, 2, 3) for sf in (1, 2, 3) for rb in (4, 16, 32)
+            candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2, 3) for rb in (4, 16, 32)
                           for rbp in ((16, 64) if self.hit else (16,))]
         torch.cuda.synchronize()
         backup = self.ws.clone()

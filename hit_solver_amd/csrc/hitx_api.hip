@@ -1335,6 +1335,10 @@ struct hitx_batch {
   int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split (column walk), 3 split (tiles)
   bool split_sensor = false, raw_tile = false;
   void* ws = nullptr;
+  // event pairs around the stage-update launches while timing is on
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
 };
 
 namespace {
@@ -1493,6 +1497,20 @@ const int* boffs(const hitx_batch* b) {
   return reinterpret_cast<const int*>(static_cast<uint8_t*>(b->ws) + b->off_offs);
 }
 
+// Event on the batch's stream around each stage-update launch when timing is
+// on (eager steps only: events recorded inside a captured graph cannot be
+// read back).
+int batch_mark(hitx_batch* b, hipStream_t s) {
+  if (!b->timing) return HITX_OK;
+  if (b->ev_used == b->ev.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    b->ev.push_back(e);
+  }
+  HIPCHK(hipEventRecord(b->ev[b->ev_used++], s));
+  return HITX_OK;
+}
+
 template <class T, bool DRAG>
 int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const Inst<T>* tab = btab<T>(b);
@@ -1557,7 +1575,10 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
       }
     }
   }
+  int rc = batch_mark(b, s);
+  if (rc) return rc;
   b_update<T><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
+  if ((rc = batch_mark(b, s))) return rc;
   LAUNCHCHK();
   return HITX_OK;
 }
@@ -1714,6 +1735,7 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
 
 int hitx_batch_destroy(hitx_batch* b) {
   if (!b) return HITX_OK;
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
   for (hitx_plan* p : b->plans) hitx_plan_destroy(p);
   delete b;
   return HITX_OK;
@@ -1766,6 +1788,28 @@ int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, i
   batch_offsets(b);
   if (b->offs.size() > b->offs_cap) return fail(HITX_EINVAL, "block map larger than its bound");
   if (b->ws) return b->esize == 8 ? batch_bind_t<double>(b, b->ws, false) : batch_bind_t<float>(b, b->ws, false);
+  return HITX_OK;
+}
+
+int hitx_batch_timing_enable(hitx_batch* b, int32_t on) {
+  if (!b) return fail(HITX_EINVAL, "null batch");
+  b->timing = on != 0;
+  b->ev_used = 0;
+  return HITX_OK;
+}
+
+int hitx_batch_timing_read(hitx_batch* b, double* total_ms, int32_t* launches) {
+  if (!b || !total_ms || !launches) return fail(HITX_EINVAL, "null argument");
+  double t = 0.0;
+  for (size_t i = 0; i + 1 < b->ev_used; i += 2) {
+    HIPCHK(hipEventSynchronize(b->ev[i + 1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+    t += ms;
+  }
+  *total_ms = t;
+  *launches = (int32_t)(b->ev_used / 2);
+  b->ev_used = 0;
   return HITX_OK;
 }
 

@@ -397,6 +397,11 @@ int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_for
  * arithmetic independent of what else is in its batch.  Blocking like
  * hitx_batch_tune. */
 int hitx_batch_stage_form(hitx_batch* batch, int32_t form);
+/* Event timing of the stage-update launches of eager steps (no reference
+ * counterpart; measurement support): enable clears the record; read returns
+ * the summed milliseconds and the launch count since, and clears them. */
+int hitx_batch_timing_enable(hitx_batch* batch, int32_t on);
+int hitx_batch_timing_read(hitx_batch* batch, double* total_ms, int32_t* launches);
 /* Device pointer of one of instance i's buffers inside the bound workspace. */
 int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
 /* Fixed dt of every instance from its current state (Riemann loop). */
