@@ -1522,9 +1522,13 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
     b_stage_reduce<T, true><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
   else
     b_stage_tile<T, DRAG><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
-  b_stage_finalize<T, DRAG><<<dim3(n, DRAG ? 1 + kFinDragGroups : 1), kBlock, 0, s>>>(tab, st,
-                                                                                  DRAG && b->stage_walk ? 1 : 0);
-  if (DRAG) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
+  // the post-step sensor (st 3) needs only the sensor scalars: no drag
+  // coefficients and no drag row table (the next stage update is a step away
+  // and reduces its own)
+  const bool drag_now = DRAG && st < 3;
+  b_stage_finalize<T, DRAG><<<dim3(n, drag_now ? 1 + kFinDragGroups : 1), kBlock, 0, s>>>(tab, st,
+                                                                                     DRAG && b->stage_walk ? 1 : 0);
+  if (drag_now) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
   if (b->split_sensor) {
     // the tile raw mask writes the transposed state itself, so the dilation
     // is bit work only; after the column-walk raw mask the byte dilation

@@ -247,7 +247,9 @@ __global__ void __launch_bounds__(64 * kSlabWaves) HITX_LINE_WAVE_ATTR(T, PER)
     b_line_wave_cols(const Inst<T>* __restrict__ tab, const int* __restrict__ offs, int ninst, int st,
                      int slice_bytes) {
   extern __shared__ __align__(16) unsigned char wsm[];
-  HITX_BATCH_INST(OFF_LYW)
+  // XCD-aware: the blocks of neighbouring columns share the 128-byte lines
+  // of each row, so they run on one XCD (one L2)
+  HITX_BATCH_TILE(OFF_LYW)
   const int nx = I.P.nx, ny = I.P.ny;
   const int x0 = lb * kSlabWaves, cs = slab_cstride(ny);
   T* slab = reinterpret_cast<T*>(wsm);

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace hitx {
 
 constexpr int kBlock = 256;  // threads per block (4 wave64s)

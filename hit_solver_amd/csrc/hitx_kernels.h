@@ -121,6 +121,23 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
   const Walk W = walk_setup(nx, ny, P.periodic, P.rb, 1, bid);
   const int lane = threadIdx.x & 63;
   auto row_of = [&](int y) { return P.periodic ? (y < 0 ? y + ny : (y >= ny ? y - ny : y)) : clampi(y, 0, ny - 1); };
+  // maxima of |rho|, |p|, |e|, |u|+c, |v|+c in the state's type (widening
+  // to double is exact and monotone, so the maxima are the same)
+  T mx[5] = {T(0), T(0), T(0), T(0), T(0)};
+  // the drag end phase's column twiddles of |kx| = 1, 2, loaded before the
+  // walk so their latency is hidden behind it
+  double twc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (DRAG) {
+    const int xc = W.own ? W.x : 0;  // any column for lanes with zero sums
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+      if (k <= P.dkx) {
+        const double* t = P.twx + ((size_t)(k + P.kmx) * nx + xc) * 2;
+        twc[2 * k - 2] = t[0];
+        twc[2 * k - 1] = t[1];
+      }
+    }
+  }
   if (W.live) {
     auto prim_e = [&](T a, T b, T c, T d) {
       if (enforce_first) enforce(P, a, b, c, d);
@@ -143,12 +160,12 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       if (W.own) {
         const int x = W.x;
         const T cs = sqrt(dv(P.gamma * cu.p, cu.rho));
-        v[R_MAXR] = fmax(v[R_MAXR], (double)absT(cu.rho));
-        v[R_MAXP] = fmax(v[R_MAXP], (double)absT(cu.p));
+        mx[0] = fmax(mx[0], absT(cu.rho));
+        mx[1] = fmax(mx[1], absT(cu.p));
         const T e = dv(cu.p, cu.rho * P.gm1);
-        v[R_MAXE] = fmax(v[R_MAXE], (double)absT(e));
-        v[R_AX] = fmax(v[R_AX], (double)(absT(cu.u) + cs));
-        v[R_AY] = fmax(v[R_AY], (double)(absT(cu.v) + cs));
+        mx[2] = fmax(mx[2], absT(e));
+        mx[3] = fmax(mx[3], absT(cu.u) + cs);
+        mx[4] = fmax(mx[4], absT(cu.v) + cs);
         T ux, uy, vx, vy;
         if (P.periodic) {
           ux = dv(ur - ul, P.two_dx);
@@ -204,6 +221,8 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       }
     }
   }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[R_MAXR + k] = (double)mx[k];
   block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * NR);
   if (DRAG) {
     // Mode (kx, ky), field f: sum over the wave's columns of
@@ -220,17 +239,8 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
     for (int f = 0; f < 2; ++f)
 #pragma unroll
       for (int j = 0; j < 5; ++j) cs_[wid][f * 5 + j][lane] = W.own ? bs[f][j] : 0.0;
-    {
-      const int xc = W.own ? W.x : 0;  // any column for lanes with zero sums
 #pragma unroll
-      for (int k = 1; k <= 2; ++k) {
-        if (k <= P.dkx) {
-          const double* t = P.twx + ((size_t)(k + P.kmx) * nx + xc) * 2;
-          tw_[wid][2 * k - 2][lane] = t[0];
-          tw_[wid][2 * k - 1][lane] = t[1];
-        }
-      }
-    }
+    for (int k = 0; k < 4; ++k) tw_[wid][k][lane] = twc[k];
     __syncthreads();
     if (lane < 2 * P.ndrag) {
       const int m = lane >> 1, f = lane & 1;
@@ -467,6 +477,42 @@ __device__ __forceinline__ T rel_jump(T fl, T f, T fr, T eps, bool periodic, boo
   return npmax(j, cv);
 }
 
+// FP32 form of "rel_jump(...) > t": each quotient a/b (b > 0) compared as
+// a > t*b, without the reciprocals (three quarter-rate ops per call); it
+// differs from the quotient form only by rounding, which the FP32 mode's
+// error budget covers (FP64 keeps the quotients for bit parity).
+__device__ __forceinline__ bool jump_over(float fl, float f, float fr, float eps, float t, bool periodic, bool at_lo,
+                                          bool at_hi) {
+  const float af = fabsf(f), afl = fabsf(fl), afr = fabsf(fr);
+  bool o = fabsf((fr - 2.0f * f) + fl) > t * (((afr + 2.0f * af) + afl) + eps);
+  if (periodic || !at_hi) o = o || fabsf(fr - f) > t * ((afr + af) + eps);
+  if (periodic || !at_lo) o = o || fabsf(f - fl) > t * ((af + afl) + eps);
+  return o;
+}
+
+// Whether the relative jump of density, pressure or internal energy at a node
+// exceeds the threshold: a node's value and its x and y neighbours of each
+// field (OOP/spatial_operator.py:283-303, OOP/problems/riemann_config3.py:309-318).
+template <class T>
+struct Jn {
+  T xl, c, xr, yl, yr;  // left / centre / right along x, left / right along y
+};
+template <class T>
+__device__ __forceinline__ bool jumps_over(const Jn<T>& r, const Jn<T>& p, const Jn<T>& e, T er, T ep, T ee,
+                                           double thr, bool per, bool xlo, bool xhi, bool ylo, bool yhi) {
+  if constexpr (std::is_same<T, float>::value) {
+    const float t = (float)thr;
+    return jump_over(r.xl, r.c, r.xr, er, t, per, xlo, xhi) || jump_over(r.yl, r.c, r.yr, er, t, per, ylo, yhi) ||
+           jump_over(p.xl, p.c, p.xr, ep, t, per, xlo, xhi) || jump_over(p.yl, p.c, p.yr, ep, t, per, ylo, yhi) ||
+           jump_over(e.xl, e.c, e.xr, ee, t, per, xlo, xhi) || jump_over(e.yl, e.c, e.yr, ee, t, per, ylo, yhi);
+  } else {
+    const T jr = npmax(rel_jump(r.xl, r.c, r.xr, er, per, xlo, xhi), rel_jump(r.yl, r.c, r.yr, er, per, ylo, yhi));
+    const T jp = npmax(rel_jump(p.xl, p.c, p.xr, ep, per, xlo, xhi), rel_jump(p.yl, p.c, p.yr, ep, per, ylo, yhi));
+    const T je = npmax(rel_jump(e.xl, e.c, e.xr, ee, per, xlo, xhi), rel_jump(e.yl, e.c, e.yr, ee, per, ylo, yhi));
+    return (double)jr > thr || (double)jp > thr || (double)je > thr;
+  }
+}
+
 // Fused sensor on a 32x16 output tile: primitives of (enforced) q on the tile
 // plus a (w+1) halo in LDS, undilated mask on the tile plus a w halo, the
 // separable (2w+1)^2 OR dilation, mask and maskT stores, and (when qT is not
@@ -530,13 +576,9 @@ __device__ __forceinline__ void sensor_body(const Params<T>& P, const T* __restr
       if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
       const bool xlo = !per && gx == 0, xhi = !per && gx == nx - 1;
       const bool ylo = !per && gy == 0, yhi = !per && gy == ny - 1;
-      const T jr = npmax(rel_jump(Lr[ixl], Lr[ic], Lr[ixr], er, per, xlo, xhi),
-                         rel_jump(Lr[iyl], Lr[ic], Lr[iyr], er, per, ylo, yhi));
-      const T jp = npmax(rel_jump(Lp[ixl], Lp[ic], Lp[ixr], ep, per, xlo, xhi),
-                         rel_jump(Lp[iyl], Lp[ic], Lp[iyr], ep, per, ylo, yhi));
-      const T je = npmax(rel_jump(Le[ixl], Le[ic], Le[ixr], ee, per, xlo, xhi),
-                         rel_jump(Le[iyl], Le[ic], Le[iyr], ee, per, ylo, yhi));
-      if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+      const Jn<T> jr{Lr[ixl], Lr[ic], Lr[ixr], Lr[iyl], Lr[iyr]}, jp{Lp[ixl], Lp[ic], Lp[ixr], Lp[iyl], Lp[iyr]};
+      const Jn<T> je{Le[ixl], Le[ic], Le[ixr], Le[iyl], Le[iyr]};
+      if (jumps_over(jr, jp, je, er, ep, ee, thr, per, xlo, xhi, ylo, yhi)) m = true;
       if (!per && P.bguard > 0) {
         const int g = P.bguard;
         if (gy < g || gy >= ny - g || gx < g || gx >= nx - g) m = true;
@@ -648,10 +690,8 @@ __device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __rest
       if (comp_on && (double)div < div_thr) m = true;
       if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
       const bool ylo = !per && y == 0, yhi = !per && y == ny - 1;
-      const T jr = npmax(rel_jump(rl, cu.rho, rr, er, per, xlo, xhi), rel_jump(pv.rho, cu.rho, nn.rho, er, per, ylo, yhi));
-      const T jp = npmax(rel_jump(pl, cu.p, pr, ep, per, xlo, xhi), rel_jump(pv.p, cu.p, nn.p, ep, per, ylo, yhi));
-      const T je = npmax(rel_jump(el, cu.e, erg, ee, per, xlo, xhi), rel_jump(pv.e, cu.e, nn.e, ee, per, ylo, yhi));
-      if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+      const Jn<T> jr{rl, cu.rho, rr, pv.rho, nn.rho}, jp{pl, cu.p, pr, pv.p, nn.p}, je{el, cu.e, erg, pv.e, nn.e};
+      if (jumps_over(jr, jp, je, er, ep, ee, thr, per, xlo, xhi, ylo, yhi)) m = true;
       if (!per && P.bguard > 0) {
         const int g = P.bguard;
         if (y < g || y >= ny - g || x < g || x >= nx - g) m = true;
@@ -799,13 +839,9 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
     if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
     const bool xlo = !per && gx == 0, xhi = !per && gx == nx - 1;
     const bool ylo = !per && gy == 0, yhi = !per && gy == ny - 1;
-    const T jr = npmax(rel_jump(Lr[ixl], Lr[ic], Lr[ixr], er, per, xlo, xhi),
-                       rel_jump(Lr[iyl], Lr[ic], Lr[iyr], er, per, ylo, yhi));
-    const T jp = npmax(rel_jump(Lp[ixl], Lp[ic], Lp[ixr], ep, per, xlo, xhi),
-                       rel_jump(Lp[iyl], Lp[ic], Lp[iyr], ep, per, ylo, yhi));
-    const T je = npmax(rel_jump(Le[ixl], Le[ic], Le[ixr], ee, per, xlo, xhi),
-                       rel_jump(Le[iyl], Le[ic], Le[iyr], ee, per, ylo, yhi));
-    if ((double)jr > thr || (double)jp > thr || (double)je > thr) m = true;
+    const Jn<T> jr{Lr[ixl], Lr[ic], Lr[ixr], Lr[iyl], Lr[iyr]}, jp{Lp[ixl], Lp[ic], Lp[ixr], Lp[iyl], Lp[iyr]};
+    const Jn<T> je{Le[ixl], Le[ic], Le[ixr], Le[iyl], Le[iyr]};
+    if (jumps_over(jr, jp, je, er, ep, ee, thr, per, xlo, xhi, ylo, yhi)) m = true;
     if (!per && P.bguard > 0) {
       const int g = P.bguard;
       if (gy < g || gy >= ny - g || gx < g || gx >= nx - g) m = true;
