@@ -1,6 +1,8 @@
-"""Batched runs: many HIT or Riemann instances of different grid sizes advance
-together through ``hitx_batch_step`` (one launch per kernel kind per stage for
-all instances, per-instance scalar control on the device).
+"""Batched runs: many HIT and Riemann instances of different grid sizes
+advance together through ``hitx_batch_step`` (one launch per kernel kind per
+stage for all instances -- two for the line sweeps when periodic and
+non-periodic instances are mixed -- with per-instance scalar control on the
+device).
 
 Each instance follows the reference's step loop for its own configuration
 (OOP/hit2d.py:764-918 or OOP/problems/riemann_config3.py:681-749).  The
@@ -117,8 +119,10 @@ def _lib_kpar():
 
 
 class BatchRunner:
-    """All instances share periodicity, dtype, scheme, sensor width and the drag
-    on/off setting (one kernel instantiation per batch).
+    """All instances share the dtype (one kernel instantiation per batch);
+    forced HIT and Riemann instances, schemes, sensor widths and drag settings
+    may be mixed.  Each instance runs the same reductions and sweeps as in a
+    batch of its own kind, so mixing does not change its results.
 
     max_steps: None, one cap for every instance, or a list (one per instance).
     init: True, or a list of booleans: instances left uninitialised are to be
@@ -132,16 +136,17 @@ class BatchRunner:
         self.configs = list(configs)
         n = len(self.configs)
         self.stream = stream
-        self.hit = isinstance(self.configs[0], HIT2DConfig)
+        self.is_hit = [isinstance(c, HIT2DConfig) for c in self.configs]
+        self.hit = all(self.is_hit)  # every instance forced HIT
         caps = list(max_steps) if isinstance(max_steps, (list, tuple)) else [max_steps] * n
         todo = list(init) if isinstance(init, (list, tuple)) else [bool(init)] * n
         if len(caps) != n or len(todo) != n:
             raise ValueError("max_steps and init lists need one entry per instance")
         specs, kpars, self.domains = [], [], []
-        for c, cap in zip(self.configs, caps):
-            if self.hit != isinstance(c, HIT2DConfig):
-                raise ValueError("a batch holds either HIT or Riemann instances")
-            if self.hit:
+        for c, cap, is_hit in zip(self.configs, caps, self.is_hit):
+            if not is_hit and not isinstance(c, RiemannConfig3):
+                raise ValueError(f"batch instances are HIT2DConfig or RiemannConfig3, not {type(c).__name__}")
+            if is_hit:
                 if cap is not None:
                     c = dataclasses.replace(c, max_steps=int(cap))
                 spec, k, domain, _ = hit_spec_kpar(c)
@@ -170,11 +175,12 @@ class BatchRunner:
         check(lib().hitx_batch_bind(h, ctypes.c_void_p(self.ws.data_ptr()), nbytes))
         self.q = [self._view(i, BUF_STATE, sp.dtype, (4, sp.ny, sp.nx)) for i, sp in enumerate(specs)]
         self.ctl_views = [self._view(i, BUF_CTL, torch.float64, (32,)) for i in range(n)]
-        self.noise = [self._view(i, BUF_NOISE, torch.float64, (sp.ny, sp.nx)) for i, sp in enumerate(specs)] \
-            if self.hit else None
-        self.rng_state = [self._view(i, BUF_RNG, torch.int64, (4,)) for i in range(n)] if self.hit else None
-        self.phi = [self._view(i, BUF_PHI, torch.float64, (max(1, len(sp.force_modes) // 2), 2)) for i, sp in
-                    enumerate(specs)] if self.hit else None
+        # forcing buffers of the HIT instances (None for Riemann instances)
+        self.noise = [self._view(i, BUF_NOISE, torch.float64, (sp.ny, sp.nx)) if h else None
+                      for i, (sp, h) in enumerate(zip(specs, self.is_hit))]
+        self.rng_state = [self._view(i, BUF_RNG, torch.int64, (4,)) if h else None for i, h in enumerate(self.is_hit)]
+        self.phi = [self._view(i, BUF_PHI, torch.float64, (max(1, len(sp.force_modes) // 2), 2)) if h else None
+                    for i, (sp, h) in enumerate(zip(specs, self.is_hit))]
         with self._ctx():
             for i in range(n):
                 if todo[i]:
@@ -196,14 +202,14 @@ class BatchRunner:
     def _init_instance(self, i):
         """Initial state (enforce, + outflow guard), control scalars and noise
         stream of instance i, as the single-instance drivers set them up."""
-        c, sp, k = self.configs[i], self.specs[i], self.kpars[i]
-        if self.hit:
+        c, sp, k, hit = self.configs[i], self.specs[i], self.kpars[i], self.is_hit[i]
+        if hit:
             domain = self.domains[i]
             q0 = initialize_hit_2d(c, domain, _make_equation(c), np.random.default_rng(c.forcing_seed),
                                    dtype=sp.dtype)
         else:
             q0 = c.initial_state()
-        get_plan(sp).enforce(q0, apply_guard=not self.hit, out=self.q[i])
+        get_plan(sp).enforce(q0, apply_guard=not hit, out=self.q[i])
         ctl = torch.zeros(32, dtype=torch.float64)
         ctl[C_ACTIVE] = 1.0
         ctl[C_TFINAL] = c.tfinal
@@ -211,7 +217,7 @@ class BatchRunner:
         ctl[C_POWER] = k[11]
         ctl[C_ALPHA] = 1.0
         self.ctl_views[i].copy_(ctl)
-        if self.hit:
+        if hit:
             w = pcg64_words(np.random.default_rng(c.forcing_seed + 1).bit_generator)
             self.rng_state[i].copy_(torch.from_numpy(w.view(np.int64).copy()))
 
@@ -221,7 +227,7 @@ class BatchRunner:
         with self._ctx():
             self.q[i].copy_(other.q[j])
             self.ctl_views[i].copy_(other.ctl_views[j])
-            if self.hit:
+            if self.is_hit[i]:
                 self.phi[i].copy_(other.phi[j])
                 self.rng_state[i].copy_(other.rng_state[j])
 
@@ -232,7 +238,7 @@ class BatchRunner:
         built."""
         with self._ctx():
             out = {"q": self.q[j].clone(), "ctl": self.ctl_views[j].clone()}
-            if self.hit:
+            if self.is_hit[j]:
                 out["phi"] = self.phi[j].clone()
                 out["rng"] = self.rng_state[j].clone()
         return out
@@ -243,7 +249,7 @@ class BatchRunner:
         with self._ctx():
             self.q[i].copy_(saved["q"])
             self.ctl_views[i].copy_(saved["ctl"])
-            if self.hit:
+            if self.is_hit[i]:
                 self.phi[i].copy_(saved["phi"])
                 self.rng_state[i].copy_(saved["rng"])
 
@@ -339,7 +345,7 @@ class BatchRunner:
         fastest candidate.  Returns (best, {candidate: ms per step})."""
         if candidates is None:
             candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2, 3) for rb in (4, 16, 32)
-                          for rbp in ((16, 64) if self.hit else (16,))]
+                          for rbp in ((16, 64) if any(self.is_hit) else (16,))]
         torch.cuda.synchronize()
         backup = self.ws.clone()
         times = {}

@@ -1314,7 +1314,9 @@ int hitx_timing_read(hitx_plan* pl, double* total_ms, int32_t* launches) {
 
 // ============================================================== batches
 struct hitx_batch {
-  int n = 0, esize = 8, periodic = 1, drag = 0, device = 0;
+  int n = 0, esize = 8, drag = 0, device = 0;
+  bool any_per = false, any_nonper = false;  // periodic (HIT) and non-periodic (Riemann) instances present
+  bool any_nodrag = false;                   // instances without the drag source (with drag: b->drag)
   bool stage_walk = false;  // drag stage reductions by column walk (many tiles) instead of on tiles
   int stage_form = 0;       // 0 auto (column walk above kStageTileMaxTiles tiles), 1 tiles, 2 column walk
   std::vector<hitx_plan*> plans;
@@ -1518,10 +1520,13 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   const int n = b->n;
   // (batches keep the one-block-per-instance finalize: with tens of tile
   // blocks per CU, a reduction in every tile block costs more than the launch)
-  if (DRAG && b->stage_walk)
-    b_stage_reduce<T, true><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
-  else
-    b_stage_tile<T, DRAG><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+  // drag instances by column walk or on tiles (b->stage_walk); instances
+  // without drag always on tiles, so a batch with and without drag gives each
+  // instance the reduction it has in a batch of its own kind
+  const bool walk = DRAG && b->stage_walk;
+  if (walk) b_stage_reduce<T, true><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);
+  if (!walk || b->any_nodrag)
+    b_stage_tile<T, DRAG><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st, walk ? 1 : 0);
   // the post-step sensor (st 3) needs only the sensor scalars: no drag
   // coefficients and no drag row table (the next stage update is a step away
   // and reduces its own)
@@ -1551,31 +1556,36 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
     return HITX_OK;
   }
   const int sb = (int)b->wave_slice;
+  // one launch per line form and periodicity present in the batch; each
+  // skips the instances of the other periodicity
   for (int axis = 0; axis < 2; ++axis) {
     const int kb = axis == 0 ? OFF_LX : OFF_LY, kw = axis == 0 ? OFF_LXW : OFF_LYW;
-    if (b->total[kw] > 0 && axis == 1 && b->slab_y) {
-      if (b->periodic)
-        b_line_wave_cols<T, true><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
-      else
-        b_line_wave_cols<T, false><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
-    } else if (b->total[kw] > 0) {
-      if (b->periodic)
-        b_line_wave<T, true><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
-      else
-        b_line_wave<T, false><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
-    }
-    if (b->total[kb] > 0) {
-      const bool sm = b->line_small[axis];
-      if (b->periodic) {
-        if (sm)
-          b_line<T, true, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+    for (int per = 1; per >= 0; --per) {
+      if (!(per ? b->any_per : b->any_nonper)) continue;
+      if (b->total[kw] > 0 && axis == 1 && b->slab_y) {
+        if (per)
+          b_line_wave_cols<T, true><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
         else
-          b_line<T, true, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
-      } else {
-        if (sm)
-          b_line<T, false, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+          b_line_wave_cols<T, false><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
+      } else if (b->total[kw] > 0) {
+        if (per)
+          b_line_wave<T, true><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
         else
-          b_line<T, false, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+          b_line_wave<T, false><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
+      }
+      if (b->total[kb] > 0) {
+        const bool sm = b->line_small[axis];
+        if (per) {
+          if (sm)
+            b_line<T, true, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+          else
+            b_line<T, true, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+        } else {
+          if (sm)
+            b_line<T, false, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+          else
+            b_line<T, false, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+        }
       }
     }
   }
@@ -1596,7 +1606,7 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
   // the batch-wide hyperviscosity word (slot after the block-map starts)
   int* hvany = const_cast<int*>(offs) + (size_t)OFF_KINDS * (n + 1) + OFF_KINDS;
   b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab, hvany);
-  if (b->periodic && b->total[OFF_PROJ] > 0) {
+  if (b->total[OFF_PROJ] > 0) {  // some instance is forced
     b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
     b_rng_scan<T><<<n, kRngScan, 0, s>>>(tab);
     b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, s>>>(tab, offs, n);
@@ -1651,15 +1661,10 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
   for (int i = 0; i < n; ++i)  // the block map holds local block indices below 2^kMapLbBits
     if ((long long)((descs[i].nx + 15) / 16) * ((descs[i].ny + 15) / 16) >= (1ll << kMapLbBits))
       return fail(HITX_EINVAL, "batch instance " + std::to_string(i) + " too large (16x16 tiles must be < 2^20)");
-  for (int i = 1; i < n; ++i) {
-    const hitx_desc &a = descs[0], &c = descs[i];
-    if (a.periodic != c.periodic || a.dtype != c.dtype || a.scheme != c.scheme || a.sensor_width != c.sensor_width ||
-        (a.drag_enabled != 0) != (c.drag_enabled != 0))
-      return fail(HITX_EINVAL, "batch instances must share periodic, dtype, scheme, sensor_width and drag on/off");
-  }
+  for (int i = 1; i < n; ++i)  // one kernel instantiation per element type; everything else is per instance
+    if (descs[i].dtype != descs[0].dtype) return fail(HITX_EINVAL, "batch instances must share the dtype");
   hitx_batch* b = new hitx_batch();
   b->n = n;
-  b->periodic = descs[0].periodic;
   b->esize = descs[0].dtype == HITX_F64 ? 8 : 4;
   b->kpar.assign(kpar, kpar + (size_t)n * K_COUNT);
   (void)hipGetDevice(&b->device);
@@ -1681,7 +1686,13 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     b->sensor_lds = std::max(b->sensor_lds, pl->sensor_lds);
     b->proj_lds = std::max(b->proj_lds, proj_lds_bytes(pl));
   }
-  b->drag = make_params<double>(b->plans[0]).drag_on;
+  for (const hitx_plan* pl : b->plans) {
+    const bool dr = make_params<double>(pl).drag_on;
+    b->drag |= dr ? 1 : 0;
+    b->any_nodrag |= !dr;
+    b->any_per |= pl->d.periodic != 0;
+    b->any_nonper |= pl->d.periodic == 0;
+  }
   b->slab_lds = (size_t)kSlabWaves * b->wave_slice;
   for (const hitx_plan* pl : b->plans)
     if (line_wave_ok(pl->d.ny))

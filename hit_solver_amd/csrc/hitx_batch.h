@@ -95,6 +95,7 @@ __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restri
   HITX_POST_STEP_EXIT
   HITX_BATCH_INST(OFF_SR)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  if (DRAG && !I.P.drag_on) return;  // instances without drag reduce on tiles (b_stage_tile)
   stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb);
 }
 
@@ -103,10 +104,11 @@ __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restri
 // batch instance and its single plan reduce in the same order).
 template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_tile(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
-                                                       int ninst, int st) {
+                                                       int ninst, int st, int walk) {
   HITX_POST_STEP_EXIT
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  if (walk && I.P.drag_on) return;  // reduced by the column walk (b_stage_reduce)
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
   stage_tile_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, bx, by, lb);
@@ -122,7 +124,7 @@ template <class T, bool DRAG>
 __global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __restrict__ tab, int st, int walk) {
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || (st == 3 && I.ctl[C_HVNOW] == 0.0)) return;
-  const int nblk = walk ? I.nblk_sr : I.ntile;
+  const int nblk = (walk && I.P.drag_on) ? I.nblk_sr : I.ntile;  // instances without drag reduce on tiles
   constexpr int NR = DRAG ? (R_DRAG + kDragCols) : R_DRAG;
   __shared__ double r[DRAG ? (R_DRAG > kFinDragCols ? R_DRAG : kFinDragCols) : R_DRAG];
   if (blockIdx.y == 0) {
@@ -133,7 +135,7 @@ __global__ void __launch_bounds__(kBlock) b_stage_finalize(const Inst<T>* __rest
   }
   if (DRAG) {
     const int c0 = ((int)blockIdx.y - 1) * kFinDragCols;
-    if (c0 >= 4 * I.P.ndrag) return;
+    if (!I.P.drag_on || c0 >= 4 * I.P.ndrag) return;
     reduce_partial_cols<kFinDragCols, 0>(I.part, nblk, NR, R_DRAG + c0, r);
     __syncthreads();
     const double N = (double)I.P.nx * (double)I.P.ny;
@@ -215,6 +217,7 @@ template <class T, bool PER, bool SMALL>
 __global__ void __launch_bounds__(kLineBlock, 4) b_line(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                      int ninst, int axis, int st) {
   HITX_BATCH_INST(axis == 0 ? OFF_LX : OFF_LY)
+  if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch
   if (axis == 0)
     line_body<T, PER, SMALL>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, lb, I.P.ny);
   else
@@ -228,6 +231,7 @@ __global__ void __launch_bounds__(64 * kLineWaves) HITX_LINE_WAVE_ATTR(T, PER)
                 int slice_bytes) {
   extern __shared__ __align__(16) unsigned char wsm[];
   HITX_BATCH_INST(axis == 0 ? OFF_LXW : OFF_LYW)
+  if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch
   const int w = threadIdx.x >> 6;
   const int line = lb * kLineWaves + w;
   const int nl = axis == 0 ? I.P.ny : I.P.nx;
@@ -250,6 +254,7 @@ __global__ void __launch_bounds__(64 * kSlabWaves) HITX_LINE_WAVE_ATTR(T, PER)
   // XCD-aware: the blocks of neighbouring columns share the 128-byte lines
   // of each row, so they run on one XCD (one L2)
   HITX_BATCH_TILE(OFF_LYW)
+  if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch (whole block, before any barrier)
   const int nx = I.P.nx, ny = I.P.ny;
   const int x0 = lb * kSlabWaves, cs = slab_cstride(ny);
   T* slab = reinterpret_cast<T*>(wsm);

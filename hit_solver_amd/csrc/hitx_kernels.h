@@ -995,7 +995,7 @@ __device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __r
     }
   }
   block_reduce_write<R_DRAG, 0x1Full>(v, part + (size_t)bid * NR);
-  if (DRAG) drag_tile_project(P, q, part + (size_t)bid * NR + R_DRAG, x0, y0);
+  if (DRAG && P.drag_on) drag_tile_project(P, q, part + (size_t)bid * NR + R_DRAG, x0, y0);
 }
 
 // 2. 32x16 tile: the (2w+1)^2 OR dilation of raw (out-of-domain nodes of the

@@ -2,9 +2,9 @@
 GPU, admitted by HBM budget, with device-resident state.
 
 Requests arrive over time (a Poisson stream in the benchmark).  Admitted
-solves are grouped by kernel class (periodicity, dtype, scheme, sensor width,
-drag), one batch per class, each on its own HIP stream so the classes' work
-overlaps on the device.  The engine advances every batch by a round of steps
+solves are grouped by element type -- forced HIT and the Riemann problems
+share one batch per dtype, each batch on its own HIP stream so the two
+classes' work overlaps on the device.  The engine advances every batch by a round of steps
 (a captured HIP graph), reads the control scalars once per round, retires the
 solves that finished and admits waiting ones as HBM frees up.  When a class's
 membership changes its batch is rebuilt and the running solves move over
@@ -92,13 +92,9 @@ def request_config(r: Request):
 
 
 def class_key(cfg):
-    """Kernel class: what one batch's instances must share."""
-    hit = isinstance(cfg, HIT2DConfig)
-    if hit:
-        spec = hit_spec_kpar(cfg)[0]
-    else:
-        spec = riemann_spec_kpar(cfg)[0]
-    return (hit, spec.dtype, spec.scheme if not hit else "hybrid", spec.sensor_width, bool(spec.drag_modes))
+    """Kernel class: what one batch's instances must share (the element
+    type; HIT and Riemann solves, schemes and drag settings mix)."""
+    return (cfg.torch_dtype,)
 
 
 _FOOTPRINT: dict = {}

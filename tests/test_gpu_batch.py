@@ -265,3 +265,46 @@ def test_slab_y_sweep_bitexact(dtype):
         for i in range(len(cfgs)):
             np.testing.assert_array_equal(host(b.q[i]), host(a.q[i]), err_msg=f"instance {i}")
         np.testing.assert_array_equal(b.control(), a.control())
+
+
+@pytest.mark.parametrize("dtype", ["fp64", "fp32"])
+@pytest.mark.parametrize("stage_form,line_form", [(1, 2), (2, 2), (2, 1)])
+def test_mixed_hit_riemann_batch_bitexact(dtype, stage_form, line_form):
+    """Forced HIT with and without the drag source, with different sensor
+    widths, and the three Riemann problems (hybrid and WENO-only schemes) in
+    one batch: every instance is bit-identical to the same instance in a batch
+    of its own kind.  The line sweeps launch once per periodicity and skip
+    the other's instances; instances without drag reduce the stage on tiles
+    whatever form the drag instances use."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+    from hit_solver_amd.riemann import RiemannConfig3, RiemannConfig3Offset08, RiemannConfig6
+
+    kw = dict(cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0, viscosity=7.5e-4,
+              mach_control=True, cooling_time=5.0, tfinal=1.0, hyperviscosity_interval=2, dtype=dtype)
+    drag = [HIT2DConfig(nx=64, ny=64, target_mach=0.25, forcing_seed=1, large_scale_drag=0.10, **kw),
+            HIT2DConfig(nx=128, ny=96, target_mach=0.4, forcing_seed=2, large_scale_drag=0.10, **kw)]
+    nodrag = [HIT2DConfig(nx=48, ny=64, target_mach=0.1, forcing_seed=3, large_scale_drag=0.0, sensor_width=2, **kw)]
+    rie = [RiemannConfig3(nx=64, ny=48, shear_threshold=None, progress_every=10**9, dtype=dtype),
+           RiemannConfig6(nx=48, ny=64, shear_threshold=None, progress_every=10**9, dtype=dtype),
+           RiemannConfig3Offset08(nx=56, ny=56, shear_threshold=None, progress_every=10**9, dtype=dtype,
+                                  scheme="weno")]
+    groups = [drag, nodrag, rie]
+    order = [(0, 0), (2, 0), (1, 0), (2, 1), (0, 1), (2, 2)]  # (group, index) of each mixed-batch slot
+    steps = 6
+
+    def run(cfgs):
+        b = BatchRunner(cfgs, max_steps=steps)
+        b.tune(4, 16, line_form, 3)
+        b.set_stage_form(stage_form)
+        b.run()
+        return b
+
+    print(f"[batch] mixed HIT + Riemann {dtype}, stage form {stage_form}, line form {line_form}", flush=True)
+    mixed = run([groups[g][i] for g, i in order])
+    alone = [run(g) for g in groups]
+    ctl = mixed.control()
+    for slot, (g, i) in enumerate(order):
+        np.testing.assert_array_equal(host(mixed.q[slot]), host(alone[g].q[i]), err_msg=f"slot {slot} = {g},{i}")
+        np.testing.assert_array_equal(ctl[slot], alone[g].control()[i])
+        assert ctl[slot, 4] == steps

@@ -35,10 +35,12 @@ def test_poisson_stream_statistics():
 
 
 def test_class_keys_group_kernel_compatible_requests():
+    """One class per element type: HIT and the Riemann problems share a batch
+    (the batch runs each instance as in a batch of its own kind)."""
     from hit_solver_amd.serving import Request, class_key, request_config
 
     k = lambda kind, dt: class_key(request_config(Request(0, kind, 128, dt, 100, 0.0)))  # noqa: E731
-    assert k("c3", "fp64") == k("c6", "fp64") == k("c3_08", "fp64")
+    assert k("c3", "fp64") == k("c6", "fp64") == k("c3_08", "fp64") == k("hit", "fp64")
     assert k("c3", "fp64") != k("c3", "fp32")
-    assert k("hit", "fp32") != k("c3", "fp32")
+    assert k("hit", "fp32") == k("c3", "fp32")
     assert math.isfinite(request_config(Request(0, "c6", 256, "fp32", 300, 0.0)).tfinal)
