@@ -236,8 +236,9 @@ def serving(count, rate, steps):
                     "stream": f"{count} requests, kinds uniform over c3/c3_08/c6/hit, n uniform in 128..1024 step "
                               f"64, fp32/fp64 uniform, {steps[0]}-{steps[1]} steps each, {cells / 1e9:.2f} G "
                               "cell-updates in total",
-                    "form": "continuous batching: per-class batches on their own streams, rounds of 10 steps as "
-                            "HIP graphs, HBM-budget admission (80% of free device memory)"})
+                    "form": "continuous batching: one batch per dtype (HIT and Riemann solves mixed) on its own "
+                            "stream, rounds of 10 steps as HIP graphs, HBM-budget admission (80% of free device "
+                            "memory); phases = host wall seconds by phase"})
     for k in ("p50_s", "p99_s", "mcell_updates_per_s"):
         if summary.get(k) is not None:
             summary[k] = round(summary[k], 4)

@@ -162,6 +162,10 @@ class ServingEngine:
         self.groups: dict = {}
         self.used = 0
         self.log: list = []
+        # host wall time by phase: rebuilds (capture included), rounds (graph
+        # replay to the synchronisation), control reads, idle waits
+        self.times = {"rebuilds": 0, "rebuild_s": 0.0, "capture_s": 0.0, "round_s": 0.0, "control_s": 0.0,
+                      "idle_s": 0.0}
 
     # ---------------------------------------------------------------- batches
     def _rebuild(self, key, g: _Group):
@@ -169,6 +173,7 @@ class ServingEngine:
         solves' state (state, control scalars, OU coefficients, noise stream)
         is staged in small copies and the old batch workspace is freed before
         the new one is allocated, so a rebuild never holds two workspaces."""
+        t_start = time.perf_counter()
         old, old_slots = g.runner, g.slots
         g.runner, g.slots = None, []
         carry = {}
@@ -195,9 +200,13 @@ class ServingEngine:
             # allows, drag stage sums by column walk whatever the class's size
             r.tune(0, 0, 2, 0)
             r.set_stage_form(2)
+            t_cap = time.perf_counter()
             r.capture(self.round_steps)
+            self.times["capture_s"] += time.perf_counter() - t_cap
             g.runner, g.slots = r, list(g.members)
         g.dirty = False
+        self.times["rebuild_s"] += time.perf_counter() - t_start  # staging, build, initial states, capture
+        self.times["rebuilds"] += 1
 
     # ---------------------------------------------------------------- loop
     def run(self, requests, time_limit: float | None = None):
@@ -239,13 +248,18 @@ class ServingEngine:
             live = [g for g in self.groups.values() if g.runner is not None]
             if not live:
                 if pending:
+                    t_idle = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                    self.times["idle_s"] += time.perf_counter() - t_idle
                 continue
+            t_round = time.perf_counter()
             for g in live:  # one round per class, each on its own stream
                 g.runner.replay(1)
             torch.cuda.synchronize()
+            self.times["round_s"] += time.perf_counter() - t_round
             rounds += 1
             t = now()
+            t_ctl = time.perf_counter()
             for key, g in self.groups.items():
                 if g.runner is None:
                     continue
@@ -268,6 +282,7 @@ class ServingEngine:
             for key, g in self.groups.items():
                 if g.dirty and not g.members:
                     g.runner, g.slots, g.dirty = None, [], False
+            self.times["control_s"] += time.perf_counter() - t_ctl
         wall = now()
         res = [results[r.rid] for r in requests]
         lat = np.array([x.latency for x in res if np.isfinite(x.finished)])
@@ -278,5 +293,6 @@ class ServingEngine:
             "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
             "mcell_updates_per_s": cell_updates / wall / 1e6 if wall > 0 else None,
             "hbm_budget_bytes": self.budget,
+            "phases": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in self.times.items()},
         }
         return res, summary
