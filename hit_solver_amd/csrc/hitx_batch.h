@@ -1,8 +1,10 @@
-// Batched launches: many instances of possibly different grid sizes advance
-// together, one launch per kernel kind per stage for all of them.  Each
-// instance has a descriptor (Inst) in device memory; a kernel finds its
-// instance by binary search of the block index in a per-kind table of block
-// offsets and runs the same per-block body as the single-instance kernels.
+// Batched launches: many instances of possibly different grid sizes and kinds
+// (forced HIT and Riemann, one element type) advance together, one launch
+// per kernel kind per stage for all of them (the line sweeps: one per
+// periodicity present).  Each instance has a descriptor (Inst) in device
+// memory; a block finds its instance and local block index in a per-kind
+// block map (one 32-bit entry per block, built on the host) and runs the same
+// per-block body as the single-instance kernels.
 // Per-instance scalar control (dt, Mach control, the forcing's OU recurrence
 // and power rescale, step counting) runs in small device kernels, so a whole
 // step needs no host synchronisation and can be captured in a HIP graph.

@@ -304,9 +304,11 @@ int hitx_timing_enable(hitx_plan* plan, int32_t on);
 int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
 
 /* ---------------------------------------------------------------- batches
- * Many instances (grids of different sizes, same periodic/dtype/scheme/
- * sensor width/drag setting) advance together: one launch per kernel kind
- * per stage covers all of them, and per-instance scalar control (dt, Mach
+ * Many instances (grids of different sizes; forced HIT and Riemann, schemes,
+ * sensor widths and drag settings mixed; one dtype) advance together: one
+ * launch per kernel kind per stage covers all of them (the line sweeps: one
+ * per periodicity present), each instance runs the reductions and sweeps it
+ * runs in a batch of its own kind, and per-instance scalar control (dt, Mach
  * control, the forcing's OU recurrence and power rescale, step counting)
  * runs on the device, so hitx_batch_step needs no host synchronisation and
  * can be captured in a HIP graph.  No reference counterpart: the reference
