@@ -388,10 +388,15 @@ int run_sensor(hitx_plan* pl, const T* q, int enforce_first, uint8_t* mask_out, 
     if (!fold) k_stage_finalize<T, false><<<1, kBlock, 0, s>>>(P, w.part, nblk, w.scal);
   }
   if (split) {
-    if (fold)
-      k_rawtile_fin<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw, qT);
+    const bool per = pl->d.periodic != 0;
+    if (fold && per)
+      k_rawtile_fin<T, true><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw, qT);
+    else if (fold)
+      k_rawtile_fin<T, false><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.part, nblk, stride, w.scal, w.raw, qT);
+    else if (per)
+      k_rawtile<T, true><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw, qT);
     else
-      k_rawtile<T><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw, qT);
+      k_rawtile<T, false><<<g, kBlock, 0, s>>>(P, q, enforce_first, w.scal, w.raw, qT);
     const int ntiles = ((pl->d.nx + kDBCols - 1) / kDBCols) * ((pl->d.ny + kDBRows - 1) / kDBRows);
     k_dilate_bits<T><<<(ntiles + kDBWaves - 1) / kDBWaves, 64 * kDBWaves, 0, s>>>(P, w.raw, mask_out, w.maskT);
   } else {
@@ -1539,7 +1544,8 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
     // is bit work only; after the column-walk raw mask the byte dilation
     // writes it (stages 0-2)
     if (b->raw_tile) {
-      b_rawtile<T><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+      if (b->any_per) b_rawtile<T, true><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+      if (b->any_nonper) b_rawtile<T, false><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
       b_dilate_bits<T><<<b->total[OFF_DB], 64 * kDBWaves, 0, s>>>(tab, offs, n, st);
     } else {
       b_rawmask<T><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);

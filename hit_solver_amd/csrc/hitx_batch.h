@@ -176,15 +176,16 @@ __global__ void __launch_bounds__(kBlock) b_rawmask(const Inst<T>* __restrict__ 
   rawmask_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, lb);
 }
 
-template <class T>
+template <class T, bool PER>
 __global__ void __launch_bounds__(kBlock) b_rawtile(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                     int ninst, int st) {
   HITX_POST_STEP_EXIT
   HITX_BATCH_TILE(OFF_SENS)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
+  if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
-  rawtile_body<T>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, st < 3 && I.use_qT ? I.qT : nullptr, bx, by);
+  rawtile_body<T, PER>(I.P, stage_in(I, st), st < 3, I.scal, I.rawm, st < 3 && I.use_qT ? I.qT : nullptr, bx, by);
 }
 
 template <class T>

@@ -125,6 +125,29 @@ __device__ __forceinline__ float dv<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
 
+// A divisor used many times: dv(a, d) with the FP32 reciprocal formed once
+// (the same bits as dv, which forms it per call).
+template <class T>
+struct Divisor {
+  T d, r;
+};
+template <class T>
+__device__ __forceinline__ Divisor<T> divisor(T d) {
+  return {d, d};
+}
+template <>
+__device__ __forceinline__ Divisor<float> divisor<float>(float d) {
+  return {d, __builtin_amdgcn_rcpf(d)};
+}
+template <class T>
+__device__ __forceinline__ T dv(T a, const Divisor<T>& d) {
+  return a / d.d;
+}
+template <>
+__device__ __forceinline__ float dv<float>(float a, const Divisor<float>& d) {
+  return a * d.r;
+}
+
 // a*b + c: fused for float (one rounding; the FP32 mode is held to its error
 // budget), two IEEE operations for double (the library is built without
 // contraction, so the FP64 mode keeps the reference's rounding).  Call sites

@@ -118,9 +118,11 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
 #pragma unroll
     for (int j = 0; j < 5; ++j) bs[f][j] = 0.0;
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const Walk W = walk_setup(nx, ny, P.periodic, P.rb, 1, bid);
+  const bool per = DRAG || P.periodic;  // the drag source exists on periodic grids only
+  const Walk W = walk_setup(nx, ny, per, P.rb, 1, bid);
   const int lane = threadIdx.x & 63;
-  auto row_of = [&](int y) { return P.periodic ? (y < 0 ? y + ny : (y >= ny ? y - ny : y)) : clampi(y, 0, ny - 1); };
+  auto row_of = [&](int y) { return per ? (y < 0 ? y + ny : (y >= ny ? y - ny : y)) : clampi(y, 0, ny - 1); };
+  const Divisor<T> d2x = divisor(P.two_dx), d2y = divisor(P.two_dy);
   // maxima of |rho|, |p|, |e|, |u|+c, |v|+c in the state's type (widening
   // to double is exact and monotone, so the maxima are the same)
   T mx[5] = {T(0), T(0), T(0), T(0), T(0)};
@@ -143,20 +145,29 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       if (enforce_first) enforce(P, a, b, c, d);
       return prim(P, a, b, c, d);
     };
-    Prim<T> pv = prim_at(P, q, N, row_of(W.ya - 1) * nx + W.xl, enforce_first);
-    // the state as given of the current and the next row stays in registers
-    // (cooling and the drag sums read it; no reload)
-    T c0, c1, c2, c3, m0, m1, m2, m3;
-    load4(q, N, W.ya * nx + W.xl, c0, c1, c2, c3);
-    load4(q, N, row_of(W.ya + 1) * nx + W.xl, m0, m1, m2, m3);
-    Prim<T> cu = prim_e(c0, c1, c2, c3);
-    Prim<T> nn = prim_e(m0, m1, m2, m3);
-    for (int y = W.ya; y < W.yb; ++y) {
-      // next row's loads in flight while this row is reduced
-      T n0 = T(0), n1 = T(0), n2 = T(0), n3 = T(0);
-      if (y + 1 < W.yb) load4(q, N, row_of(y + 2) * nx + W.xl, n0, n1, n2, n3);
+    // Rows rotate through three register sets (primitives of rows y-1, y,
+    // y+1 and the state as given of rows y, y+1, y+2; cooling and the drag
+    // sums read the state as given, no reload).  The loop body handles three
+    // rows with the sets' roles permuted, so no row is copied between sets.
+    struct Raw {
+      T a, b, c, d;
+    };
+    Prim<T> P0 = prim_at(P, q, N, row_of(W.ya - 1) * nx + W.xl, enforce_first), P1, P2;
+    Raw R0, R1, R2;
+    load4(q, N, W.ya * nx + W.xl, R1.a, R1.b, R1.c, R1.d);
+    load4(q, N, row_of(W.ya + 1) * nx + W.xl, R2.a, R2.b, R2.c, R2.d);
+    P1 = prim_e(R1.a, R1.b, R1.c, R1.d);
+    P2 = prim_e(R2.a, R2.b, R2.c, R2.d);
+    // row y with pv, cu, nn the primitives of rows y-1, y, y+1, cr the state
+    // as given of row y; the loads of row y+2 go to nr (in flight during the
+    // row) and its primitives to pn (the set row y-1 held)
+    auto row = [&](int y, Prim<T>& pv, const Prim<T>& cu, const Prim<T>& nn, const Raw& cr, Raw& nr,
+                   Prim<T>& pn) {
+      const bool more = y + 1 < W.yb;
+      if (more) load4(q, N, row_of(y + 2) * nx + W.xl, nr.a, nr.b, nr.c, nr.d);
       const T ul = lane_step(cu.u, -1), ur = lane_step(cu.u, 1);
       const T vl = lane_step(cu.v, -1), vr = lane_step(cu.v, 1);
+      const T c0 = cr.a, c1 = cr.b, c2 = cr.c, c3 = cr.d;
       if (W.own) {
         const int x = W.x;
         const T cs = sqrt(dv(P.gamma * cu.p, cu.rho));
@@ -167,11 +178,11 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
         mx[3] = fmax(mx[3], absT(cu.u) + cs);
         mx[4] = fmax(mx[4], absT(cu.v) + cs);
         T ux, uy, vx, vy;
-        if (P.periodic) {
-          ux = dv(ur - ul, P.two_dx);
-          vx = dv(vr - vl, P.two_dx);
-          uy = dv(nn.u - pv.u, P.two_dy);
-          vy = dv(nn.v - pv.v, P.two_dy);
+        if (per) {
+          ux = dv(ur - ul, d2x);
+          vx = dv(vr - vl, d2x);
+          uy = dv(nn.u - pv.u, d2y);
+          vy = dv(nn.v - pv.v, d2y);
         } else {  // OOP/problems/riemann_config3.py:268-275 (clamped neighbours)
           const bool xin = (x > 0 && x < nx - 1), yin = (y > 0 && y < ny - 1);
           ux = dv(ur - ul, xin ? P.two_dx : P.dx);
@@ -179,11 +190,13 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
           uy = dv(nn.u - pv.u, yin ? P.two_dy : P.dy);
           vy = dv(nn.v - pv.v, yin ? P.two_dy : P.dy);
         }
+        // sums of squares and products accumulate with fused multiply-adds
+        // (summation order is free here; the pointwise algebra above is not)
         const T div = ux + vy;
-        v[R_DIV2] += (double)div * (double)div;
+        v[R_DIV2] = fma((double)div, (double)div, v[R_DIV2]);
         if (P.shear_on) {
           const T w = vx - uy;
-          v[R_VORT2] += (double)w * (double)w;
+          v[R_VORT2] = fma((double)w, (double)w, v[R_VORT2]);
         }
         if (P.cool_on || DRAG) {
           const T a = c0, b = c1, c = c2, d = c3;
@@ -197,28 +210,23 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
               if (k <= P.dky) {
                 const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + y) * 2;
                 const double cy = ty[0], sy = ty[1];
-                bs[0][2 * k - 1] += fb * cy;
-                bs[0][2 * k] += fb * sy;
-                bs[1][2 * k - 1] += fc * cy;
-                bs[1][2 * k] += fc * sy;
+                bs[0][2 * k - 1] = fma(fb, cy, bs[0][2 * k - 1]);
+                bs[0][2 * k] = fma(fb, sy, bs[0][2 * k]);
+                bs[1][2 * k - 1] = fma(fc, cy, bs[1][2 * k - 1]);
+                bs[1][2 * k] = fma(fc, sy, bs[1][2 * k]);
               }
             }
           }
         }
       }
-      pv = cu;
-      cu = nn;
-      c0 = m0;
-      c1 = m1;
-      c2 = m2;
-      c3 = m3;
-      if (y + 1 < W.yb) {
-        m0 = n0;
-        m1 = n1;
-        m2 = n2;
-        m3 = n3;
-        nn = prim_e(n0, n1, n2, n3);
-      }
+      if (more) pn = prim_e(nr.a, nr.b, nr.c, nr.d);
+    };
+    for (int y = W.ya; y < W.yb; y += 3) {
+      row(y, P0, P1, P2, R1, R0, P0);
+      if (y + 1 >= W.yb) break;
+      row(y + 1, P1, P2, P0, R2, R1, P1);
+      if (y + 2 >= W.yb) break;
+      row(y + 2, P2, P0, P1, R0, R2, P2);
     }
   }
 #pragma unroll
@@ -715,7 +723,7 @@ __device__ __forceinline__ void rawmask_body(const Params<T>& P, const T* __rest
 //     every block and a separate finalize agree to the bit) into the stage
 //     scalars it then uses; block `first` also writes them to scal_out for
 //     the later kernels of the stage.
-template <class T, bool FIN = false>
+template <class T, bool PER, bool FIN = false>
 __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
                                              const double* __restrict__ scal, uint8_t* __restrict__ raw,
                                              T* __restrict__ qT, int bx, int by,
@@ -728,7 +736,7 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
   __shared__ T st[4 * kSTX * (kSTY + 1)];  // the tile's (enforced) state, transposed, when qT is wanted
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x0 = bx * kSTX, y0 = by * kSTY;
-  const bool per = P.periodic;
+  constexpr bool per = PER;  // periodic grids take no clamps and no end tests
   double pf[PF][R_DRAG];
   if (FIN) {
 #pragma unroll
@@ -821,6 +829,7 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
   }
   const T er = (T)scal[S_EPS_R], ep = (T)scal[S_EPS_P], ee = (T)scal[S_EPS_E];
   const bool comp_on = scal[S_COMP_ON] != 0.0, shear_on = scal[S_SHEAR_ON] != 0.0;
+  const Divisor<T> d2x = divisor(P.two_dx), d2y = divisor(P.two_dy);
   const double div_thr = scal[S_DIV_THR], shear_thr = scal[S_SHEAR_THR], thr = P.jump_thr;
   for (int t = threadIdx.x; t < kSTX * kSTY; t += kBlock) {
     const int r = t / kSTX, c = t - r * kSTX;
@@ -830,10 +839,18 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
     const int ic = (r + 1) * EX + (c + 1);
     const int ixl = ic - 1, ixr = ic + 1, iyl = ic - EX, iyr = ic + EX;
     const bool xin = per || (gx > 0 && gx < nx - 1), yin = per || (gy > 0 && gy < ny - 1);
-    const T ux = dv(Lu[ixr] - Lu[ixl], xin ? P.two_dx : P.dx);
-    const T vx = dv(Lv[ixr] - Lv[ixl], xin ? P.two_dx : P.dx);
-    const T uy = dv(Lu[iyr] - Lu[iyl], yin ? P.two_dy : P.dy);
-    const T vy = dv(Lv[iyr] - Lv[iyl], yin ? P.two_dy : P.dy);
+    T ux, vx, uy, vy;
+    if (per) {
+      ux = dv(Lu[ixr] - Lu[ixl], d2x);
+      vx = dv(Lv[ixr] - Lv[ixl], d2x);
+      uy = dv(Lu[iyr] - Lu[iyl], d2y);
+      vy = dv(Lv[iyr] - Lv[iyl], d2y);
+    } else {
+      ux = dv(Lu[ixr] - Lu[ixl], xin ? P.two_dx : P.dx);
+      vx = dv(Lv[ixr] - Lv[ixl], xin ? P.two_dx : P.dx);
+      uy = dv(Lu[iyr] - Lu[iyl], yin ? P.two_dy : P.dy);
+      vy = dv(Lv[iyr] - Lv[iyl], yin ? P.two_dy : P.dy);
+    }
     const T div = ux + vy;
     if (comp_on && (double)div < div_thr) m = true;
     if (shear_on && (double)absT(vx - uy) > shear_thr) m = true;
@@ -1081,25 +1098,25 @@ __global__ void __launch_bounds__(kBlock) k_stage_tile(Params<T> P, const T* __r
   stage_tile_body<T, DRAG>(P, q, enforce_first, part, t_.bx, t_.by, t_.by * (int)gridDim.x + t_.bx);
 }
 
-template <class T>
+template <class T, bool PER>
 __global__ void __launch_bounds__(kBlock) k_rawtile_fin(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                         const double* __restrict__ part, int nblk, int stride,
                                                         double* __restrict__ scal, uint8_t* __restrict__ raw,
                                                         T* __restrict__ qT) {
   const Tile2 t_ = tile_xcd();
-  rawtile_body<T, true>(P, q, enforce_first, nullptr, raw, qT, t_.bx, t_.by, part, nblk, stride, scal,
-                        blockIdx.x == 0 && blockIdx.y == 0);
+  rawtile_body<T, PER, true>(P, q, enforce_first, nullptr, raw, qT, t_.bx, t_.by, part, nblk, stride, scal,
+                             blockIdx.x == 0 && blockIdx.y == 0);
 }
 
 // Raw-mask tiles that read the stage scalars written by the finalize kernel
 // (single plans with many partial rows, where re-reducing them in every tile
 // would cost more than the separate finalize).
-template <class T>
+template <class T, bool PER>
 __global__ void __launch_bounds__(kBlock) k_rawtile(Params<T> P, const T* __restrict__ q, int enforce_first,
                                                     const double* __restrict__ scal, uint8_t* __restrict__ raw,
                                                     T* __restrict__ qT) {
   const Tile2 t_ = tile_xcd();
-  rawtile_body<T>(P, q, enforce_first, scal, raw, qT, t_.bx, t_.by);
+  rawtile_body<T, PER>(P, q, enforce_first, scal, raw, qT, t_.bx, t_.by);
 }
 
 // ---------------------------------------------------------------- bit dilation
