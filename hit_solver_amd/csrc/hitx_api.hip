@@ -1375,6 +1375,7 @@ int batch_blocks(const hitx_batch* b, const hitx_plan* pl, int kind) {
     case OFF_LYW: return wy ? (b->slab_y ? (nx + kSlabWaves - 1) / kSlabWaves : (nx + kLineWaves - 1) / kLineWaves) : 0;
     case OFF_T16: return ((nx + 15) / 16) * ((ny + 15) / 16);
     case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
+    case OFF_FS: return pl->d.n_force_modes > 0 ? walk_blocks(nx, ny, kFsRows, 0) : 0;
     case OFF_RNG:
       return pl->d.n_force_modes > 0 ? (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads : 0;
     case OFF_RNGS:
@@ -1438,7 +1439,7 @@ size_t batch_offs_capacity(const hitx_batch* b) {
     if (pl->d.n_force_modes > 0)
       cap += (size_t)walk_blocks(nx, ny, kTuneRbpMin, 0) + (ny * (pl->fkx + 1) + kBlock - 1) / kBlock +
              2 * ((rng_chunks(pl->N) + kRngScatterThreads - 1) / kRngScatterThreads) +
-             (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads;
+             (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads + walk_blocks(nx, ny, kFsRows, 0);
     cap += (size_t)(2 * ny + kBlock - 1) / kBlock;
   }
   return cap;
@@ -1597,7 +1598,8 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
   }
   int rc = batch_mark(b, s);
   if (rc) return rc;
-  b_update<T><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
+  if (b->any_per) b_update<T, true><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
+  if (b->any_nonper) b_update<T, false><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
   if ((rc = batch_mark(b, s))) return rc;
   LAUNCHCHK();
   return HITX_OK;
@@ -1620,7 +1622,7 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
     b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, s>>>(tab, offs, n);
     b_fou<T><<<n, kBlock, 0, s>>>(tab);
     b_frows<T><<<b->total[OFF_FROWS], kBlock, 0, s>>>(tab, offs, n);
-    b_fsynth<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, offs, n);
+    b_fsynth<T><<<b->total[OFF_FS], kBlock, 0, s>>>(tab, offs, n);
     b_falpha<T><<<n, kBlock, 0, s>>>(tab);
   }
   LAUNCHCHK();

@@ -49,8 +49,11 @@ struct Inst {
 // offsets kinds (rows of the offsets table, each ninst+1 ints)
 enum {
   OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_LXW, OFF_LYW,
-  OFF_RNGS, OFF_DB, OFF_KINDS
+  OFF_RNGS, OFF_DB, OFF_FS, OFF_KINDS
 };
+
+// rows per wavefront of the batch forcing synthesis (column walk)
+constexpr int kFsRows = 16;
 
 constexpr int kMapLbBits = 20;  // local blocks per instance and kind < 2^20; instances < 2^12
 
@@ -275,16 +278,17 @@ __global__ void __launch_bounds__(64 * kSlabWaves) HITX_LINE_WAVE_ATTR(T, PER)
 
 // update tiles are the sensor's 32x16 tiles (the same block offsets)
 static_assert(kUTX == kSTX && kUTY == kSTY, "update and sensor tiles differ");
-template <class T>
+template <class T, bool PER>
 __global__ void __launch_bounds__(kUBlock) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                     int ninst, int st) {
   HITX_BATCH_TILE(OFF_SENS)
+  if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch
   const int ntx = (I.P.nx + kUTX - 1) / kUTX;
   const int by = lb / ntx, bx = lb - by * ntx;
   // the last stage writes Q in place when no hyperviscosity pass follows and
   // there is no outflow guard (each node then reads Q only at itself)
   const GPtr<T> out = (st == 2 && I.ctl[C_HVNOW] == 0.0 && I.P.guard == 0) ? I.Q : stage_out(I, st);
-  update_body<T>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
+  update_body<T, PER>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
                  out, bx, by, (T)I.ctl[C_SCALE], (T)I.ctl[C_ALPHA]);
 }
 
@@ -488,34 +492,73 @@ __global__ void __launch_bounds__(kBlock) b_frows(const Inst<T>* __restrict__ ta
 // applied where the stage update reads them), with the sums the rescale and
 // the statistics need: sum(fx^2+fy^2), sum(rho*(fx*u+fy*v)) at Q, sum(fx^2),
 // sum(fy^2), sum(fx*fy).
+// Column walk: a wavefront owns 64 columns and kFsRows rows.  Each lane keeps
+// its column's twiddles cos/sin(k x), k = 1..fkx, in registers; the wave's
+// rows of the synthesis table (2 fields x (2 fkx + 1) doubles per row) are
+// copied to its LDS slice once and read as broadcasts, so a node costs the
+// two inverse-DFT sums as fused multiply-adds (the summation order of
+// fsynth_at, fused: within a few ulp of the single-plan kernel).
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                    int ninst) {
-  HITX_BATCH_INST(OFF_RED)
+  constexpr int kRow = 2 * (2 * kFK + 1);  // table entries of a row at most
+  __shared__ double hs[kWaves][kFsRows * kRow];
+  HITX_BATCH_INST(OFF_FS)
   if (I.kpar[K_FORCE] == 0.0) return;
   const Params<T>& P = I.P;
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const double invN = 1.0 / ((double)nx * (double)ny);
+  const Walk W = walk_setup(nx, ny, 1, kFsRows, 0, lb);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int fk = P.fkx, FR = 2 * fk + 1;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  // a wavefront takes 64 columns of one row at a time, so the row's table
-  // entries are wave-uniform (scalar loads)
-  const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  const int nseg = (nx + 63) >> 6, npairs = ny * nseg;
-  for (int rs = lb * kWaves + wid; rs < npairs; rs += I.nblk_stage * kWaves) {
-    const int y = rs / nseg, x = (rs - y * nseg) * 64 + lane;
-    if (x >= nx) continue;
-    const int idx = y * nx + x;
-    double ax, ay;
-    fsynth_at(P, I.frows, y, x, ax, ay);
-    const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
-    I.fx[idx] = fxv;
-    I.fy[idx] = fyv;
-    const Prim<T> pr = prim_at<T>(P, I.Q, N, idx, 0);
-    acc[0] += (double)(fxv * fxv + fyv * fyv);
-    acc[1] += (double)(pr.rho * (fxv * pr.u + fyv * pr.v));
-    acc[2] += (double)(fxv * fxv);
-    acc[3] += (double)(fyv * fyv);
-    acc[4] += (double)(fxv * fyv);
+  if (W.live) {
+    double* H0 = hs[wid];
+    {
+      const double* src = I.frows + (size_t)W.ya * 2 * FR;
+      const int cnt = (W.yb - W.ya) * 2 * FR;
+      for (int i = lane; i < cnt; i += 64) H0[i] = src[i];
+    }
+    double tc[kFK], ts[kFK];
+    {
+      const int xc = W.own ? W.x : 0;  // any column for lanes past the grid
+#pragma unroll
+      for (int k = 1; k <= kFK; ++k) {  // past fkx: the row of fkx, never used
+        const double* tx = P.twx + ((size_t)(min(k, fk) + P.kmx) * nx + xc) * 2;
+        tc[k - 1] = tx[0];
+        ts[k - 1] = tx[1];
+      }
+    }
+    // the slice is the wave's own: its stores complete before any lane reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int y = W.ya; y < W.yb; ++y) {
+      const double* H = H0 + (y - W.ya) * 2 * FR;
+      double ax = H[0], ay = H[FR];
+#pragma unroll
+      for (int k = 1; k <= kFK; ++k) {
+        if (k <= fk) {
+          const double c = tc[k - 1], sn = ts[k - 1];
+          ax = fma(H[2 * k - 1], c, ax);
+          ax = fma(-H[2 * k], sn, ax);
+          ay = fma(H[FR + 2 * k - 1], c, ay);
+          ay = fma(-H[FR + 2 * k], sn, ay);
+        }
+      }
+      if (W.own) {
+        const int idx = y * nx + W.x;
+        const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
+        I.fx[idx] = fxv;
+        I.fy[idx] = fyv;
+        const Prim<T> pr = prim_at<T>(P, I.Q, N, idx, 0);
+        acc[0] += (double)(fxv * fxv + fyv * fyv);
+        acc[1] += (double)(pr.rho * (fxv * pr.u + fyv * pr.v));
+        acc[2] += (double)(fxv * fxv);
+        acc[3] += (double)(fyv * fyv);
+        acc[4] += (double)(fxv * fyv);
+      }
+    }
   }
   block_reduce_write<5, 0>(acc, I.part + (size_t)lb * 5);
 }
@@ -529,7 +572,7 @@ __global__ void __launch_bounds__(kBlock) b_falpha(const Inst<T>* __restrict__ t
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
   __shared__ double r[5];
-  reduce_partials<5, 0>(I.part, I.nblk_stage, r);
+  reduce_partials<5, 0>(I.part, walk_blocks(I.P.nx, I.P.ny, kFsRows, 0), r);
   __syncthreads();
   if (threadIdx.x != 0) return;
   double* c = I.ctl;

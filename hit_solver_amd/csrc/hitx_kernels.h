@@ -1836,7 +1836,7 @@ enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
 // interior source, OOP/problems/riemann_config3.py:419-429).
 // One kUTX x kUTY tile per block: the y-derivative (stored transposed,
 // [c][x][y]) is staged through LDS so both derivative reads are coalesced.
-template <class T>
+template <class T, bool PER_ONLY = false>  // PER_ONLY: a periodic grid (no outflow guard), known at compile time
 __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
                                             const T* __restrict__ qs, const T* __restrict__ dfx,
                                             const T* __restrict__ dfyT, const T* __restrict__ fx,
@@ -1851,7 +1851,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   const int x = x0 + tx, y = y0 + ty;
   const bool in = x < nx && y < ny;
   int cx = x, cy = y;
-  if (P.guard > 0) {
+  if (!PER_ONLY && P.guard > 0) {
     cx = clampi(x, P.guard, nx - 1 - P.guard);
     cy = clampi(y, P.guard, ny - 1 - P.guard);
   }
@@ -1938,7 +1938,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
     for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = vv[c];
     return;
   }
-  const bool own = cx == x && cy == y;  // guard nodes take their interior source's values
+  const bool own = PER_ONLY || (cx == x && cy == y);  // guard nodes take their interior source's values
   T Lv[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
