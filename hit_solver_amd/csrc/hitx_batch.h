@@ -533,7 +533,13 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // the state of the current row for the power sums, the next row's loads
+    // in flight while this row is synthesised
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+    if (W.own) load4<T>(I.Q, N, W.ya * nx + W.x, a0, a1, a2, a3);
     for (int y = W.ya; y < W.yb; ++y) {
+      T b0 = T(0), b1 = T(0), b2 = T(0), b3 = T(0);
+      if (W.own && y + 1 < W.yb) load4<T>(I.Q, N, (y + 1) * nx + W.x, b0, b1, b2, b3);
       const double* H = H0 + (y - W.ya) * 2 * FR;
       double ax = H[0], ay = H[FR];
 #pragma unroll
@@ -551,13 +557,17 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
         const T fxv = (T)(ax * invN), fyv = (T)(ay * invN);
         I.fx[idx] = fxv;
         I.fy[idx] = fyv;
-        const Prim<T> pr = prim_at<T>(P, I.Q, N, idx, 0);
+        const Prim<T> pr = prim(P, a0, a1, a2, a3);
         acc[0] += (double)(fxv * fxv + fyv * fyv);
         acc[1] += (double)(pr.rho * (fxv * pr.u + fyv * pr.v));
         acc[2] += (double)(fxv * fxv);
         acc[3] += (double)(fyv * fyv);
         acc[4] += (double)(fxv * fyv);
       }
+      a0 = b0;
+      a1 = b1;
+      a2 = b2;
+      a3 = b3;
     }
   }
   block_reduce_write<5, 0>(acc, I.part + (size_t)lb * 5);

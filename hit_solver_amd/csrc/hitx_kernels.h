@@ -2159,6 +2159,7 @@ __global__ void __launch_bounds__(kBlock) k_enforce(Params<T> P, const T* __rest
 
 // ================================================================ forcing
 constexpr int kFK = 12;  // largest |kx|, |ky| index of a forcing mode
+constexpr int kProjChunk = 16;  // rows of row twiddles a projection wave stages in LDS at a time
 
 // fft2(noise) at the shell modes (OOP/forcing.py:78-79): per block and mode
 // the partial sum of noise*exp(-i(kx x + ky y)), re and im.  Column walk
@@ -2184,23 +2185,41 @@ __device__ __forceinline__ void fproject_body(const Params<T>& P, const double* 
   if (W.live) {
     // rows in groups of 8 with the group's noise loads in flight together;
     // every k up to kFK without a branch: the sums past fky read the table
-    // row of fky and are never stored
-    const int fk = P.fky;
-    for (int y0 = W.ya; y0 < W.yb; y0 += 8) {
-      double nv[8];
+    // row of fky and are never stored.  The rows' twiddles cos/sin(k y),
+    // k = 1..fky, are copied 16 rows at a time into the wave's LDS slice (the
+    // end phase's area, free during the walk: 2 fky x 16 <= 64 (2 fky + 1)
+    // doubles) and read as broadcasts (no scalar loads).
+    const int fk = P.fky, fs = fk > 0 ? fk : 1;
+    double* tw = dsh + (size_t)(threadIdx.x >> 6) * (2 * fk + 1) * 64;
+    for (int c0 = W.ya; c0 < W.yb; c0 += kProjChunk) {
+      const int rows = min(kProjChunk, W.yb - c0);
+      __builtin_amdgcn_wave_barrier();  // the previous chunk's reads are done (in order within the wave)
+      for (int i = lane; i < rows * fk; i += 64) {
+        const int r = i / fk, k = i - r * fk + 1;
+        const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + c0 + r) * 2;
+        tw[(r * fs + k - 1) * 2] = ty[0];
+        tw[(r * fs + k - 1) * 2 + 1] = ty[1];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int y0 = c0; y0 < c0 + rows; y0 += 8) {
+        double nv[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) nv[j] = (W.own && y0 + j < W.yb) ? noise[(size_t)(y0 + j) * nx + W.x] : 0.0;
+        for (int j = 0; j < 8; ++j) nv[j] = (W.own && y0 + j < W.yb) ? noise[(size_t)(y0 + j) * nx + W.x] : 0.0;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int y = y0 + j;
-        if (y < W.yb) {
-          const double n = nv[j];
-          ac[0] += n;
+        for (int j = 0; j < 8; ++j) {
+          const int y = y0 + j;
+          if (y < c0 + rows) {
+            const double n = nv[j];
+            const double* t = tw + (y - c0) * fs * 2;
+            ac[0] += n;
 #pragma unroll
-          for (int k = 1; k <= kFK; ++k) {
-            const double* ty = P.twy + ((size_t)(min(k, fk) + P.kmy) * ny + y) * 2;
-            ac[k] += n * ty[0];
-            as[k] += n * ty[1];
+            for (int k = 1; k <= kFK; ++k) {
+              const int kk = min(k, fs);
+              ac[k] += n * t[2 * (kk - 1)];
+              as[k] += n * t[2 * (kk - 1) + 1];
+            }
           }
         }
       }
