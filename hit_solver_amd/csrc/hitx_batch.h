@@ -427,8 +427,8 @@ __global__ void __launch_bounds__(kRngAttemptThreads) b_rng_attempts(const Inst<
                                                                      const int* __restrict__ offs, int ninst) {
   HITX_BATCH_INST(OFF_RNG)
   if (I.kpar[K_FORCE] == 0.0) return;
-  __shared__ __align__(16) uint8_t lds[kRngAttemptThreads * kRngChunk];
-  rng_attempt_body(I.R, lb * kRngAttemptThreads, lds + threadIdx.x * kRngChunk);
+  __shared__ __align__(16) uint8_t lds[kRngAttemptThreads * 3 * kRngChunk];
+  rng_attempt_body(I.R, lb * kRngAttemptThreads, lds + threadIdx.x * 3 * kRngChunk);
 }
 
 template <class T>

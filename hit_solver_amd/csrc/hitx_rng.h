@@ -173,6 +173,7 @@ __host__ __device__ inline int rng_chunks(long long n) { return (rng_words(n) + 
 // is c0 (every thread of the block calls it).  One thread advances the state
 // to the block's first word; each thread then jumps by 32 threadIdx.x words
 // with the table.  lds: kRngChunk bytes per thread.
+// lds: the thread's 3 * kRngChunk bytes (codes, chain spills, chain counts).
 __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_t* lds) {
   __shared__ uint64_t sb[2];
   const int c = c0 + threadIdx.x;
@@ -198,15 +199,27 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
     dst[0] = src[0];
     dst[1] = src[1];
   }
-  for (int e = 0; e < kRngEntries; ++e) {
-    int p = e, cnt = 0;
-    while (p < kRngChunk) {
-      const uint8_t code = lds[p];
-      cnt += code >> 7;
-      p += code & 0x7f;
+  // Chain ends from every position by one backward pass (the chain from p
+  // continues at p + len(p)): sp[p] = words past the chunk where it leaves,
+  // ct[p] = values it yields on the way; the map's entries are those of
+  // positions 0..7.  (Walking each entry's chain on its own costs about
+  // eight times the LDS round trips.)
+  uint8_t* sp = lds + kRngChunk;
+  uint8_t* ct = lds + 2 * kRngChunk;
+  for (int p = kRngChunk - 1; p >= 0; --p) {
+    const uint8_t code = lds[p];
+    const int nx = p + (code & 0x7f), y = code >> 7;
+    if (nx >= kRngChunk) {
+      sp[p] = (uint8_t)min(nx - kRngChunk, 0xff);
+      ct[p] = (uint8_t)y;
+    } else {
+      sp[p] = sp[nx];
+      ct[p] = (uint8_t)(ct[nx] + y);
     }
-    const int spill = p - kRngChunk;
-    J.map[(size_t)c * kRngEntries + e] = (uint16_t)((spill < kRngEntries ? spill : 0xff) | (cnt << 8));
+  }
+  for (int e = 0; e < kRngEntries; ++e) {
+    const int spill = sp[e];
+    J.map[(size_t)c * kRngEntries + e] = (uint16_t)((spill < kRngEntries ? spill : 0xff) | (ct[e] << 8));
   }
 }
 
@@ -397,8 +410,8 @@ constexpr int kRngAttemptThreads = 256;
 
 static_assert(kRngAttemptThreads <= 256, "jump table covers 256 threads");
 __global__ void __launch_bounds__(kRngAttemptThreads) k_rng_attempts(RngJob J) {
-  __shared__ __align__(16) uint8_t lds[kRngAttemptThreads * kRngChunk];
-  rng_attempt_body(J, blockIdx.x * kRngAttemptThreads, lds + threadIdx.x * kRngChunk);
+  __shared__ __align__(16) uint8_t lds[kRngAttemptThreads * 3 * kRngChunk];
+  rng_attempt_body(J, blockIdx.x * kRngAttemptThreads, lds + threadIdx.x * 3 * kRngChunk);
 }
 __global__ void __launch_bounds__(kRngScan) k_rng_scan(RngJob J, int force_serial) { rng_scan_body(J, force_serial); }
 __global__ void __launch_bounds__(kRngScatterThreads) k_rng_scatter(RngJob J) {
