@@ -1346,6 +1346,11 @@ struct hitx_batch {
   bool timing = false;
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
+  // the forcing chain's own stream (non-blocking, default priority) and the
+  // fork/join events: it runs beside stage 0's sensor and sweeps, which do
+  // not read the forcing field; created on the first forced step
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 namespace {
@@ -1520,7 +1525,7 @@ int batch_mark(hitx_batch* b, hipStream_t s) {
 }
 
 template <class T, bool DRAG>
-int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
+int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before_update = nullptr) {
   const Inst<T>* tab = btab<T>(b);
   const int* offs = boffs(b);
   const int n = b->n;
@@ -1596,6 +1601,7 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s) {
       }
     }
   }
+  if (before_update) HIPCHK(hipStreamWaitEvent(s, before_update, 0));  // the forcing field is ready
   int rc = batch_mark(b, s);
   if (rc) return rc;
   if (b->any_per) b_update<T, true><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
@@ -1614,20 +1620,35 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
   // the batch-wide hyperviscosity word (slot after the block-map starts)
   int* hvany = const_cast<int*>(offs) + (size_t)OFF_KINDS * (n + 1) + OFF_KINDS;
   b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab, hvany);
-  if (b->total[OFF_PROJ] > 0) {  // some instance is forced
-    b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, s>>>(tab, offs, n);
-    b_rng_scan<T><<<n, kRngScan, 0, s>>>(tab);
-    b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, s>>>(tab, offs, n);
-    b_rng_finish<T><<<n, 64, 0, s>>>(tab);
-    b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, s>>>(tab, offs, n);
-    b_fou<T><<<n, kBlock, 0, s>>>(tab);
-    b_frows<T><<<b->total[OFF_FROWS], kBlock, 0, s>>>(tab, offs, n);
-    b_fsynth<T><<<b->total[OFF_FS], kBlock, 0, s>>>(tab, offs, n);
-    b_falpha<T><<<n, kBlock, 0, s>>>(tab);
-  }
   LAUNCHCHK();
+  // The forcing chain (noise, projection, OU update, synthesis, rescale)
+  // reads the control scalars b_ctl_begin wrote and writes only forcing
+  // buffers, its partials included (fpart); stage 0's sensor and sweeps read
+  // neither.  So the chain runs on the side stream beside them, and stage 0's
+  // update waits for it.  Captured into a graph, the fork and join become the
+  // graph's edges.
+  hipEvent_t joined = nullptr;
+  if (b->total[OFF_PROJ] > 0) {  // some instance is forced
+    if (!b->side) return fail(HITX_EINVAL, "batch not bound to a workspace");
+    hipStream_t f = b->side;
+    HIPCHK(hipEventRecord(b->fork_ev, s));
+    HIPCHK(hipStreamWaitEvent(f, b->fork_ev, 0));
+    b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, f>>>(tab, offs, n);
+    b_rng_scan<T><<<n, kRngScan, 0, f>>>(tab);
+    b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, f>>>(tab, offs, n);
+    b_rng_finish<T><<<n, 64, 0, f>>>(tab);
+    b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, f>>>(tab, offs, n);
+    b_fou<T><<<n, kBlock, 0, f>>>(tab);
+    b_frows<T><<<b->total[OFF_FROWS], kBlock, 0, f>>>(tab, offs, n);
+    b_fsynth<T><<<b->total[OFF_FS], kBlock, 0, f>>>(tab, offs, n);
+    b_falpha<T><<<n, kBlock, 0, f>>>(tab);
+    LAUNCHCHK();
+    HIPCHK(hipEventRecord(b->join_ev, f));
+    joined = b->join_ev;
+  }
   for (int st = 0; st < 4; ++st) {
-    int rc = b->drag ? batch_stage_sequence<T, true>(b, st, s) : batch_stage_sequence<T, false>(b, st, s);
+    const hipEvent_t w = st == 0 ? joined : nullptr;
+    int rc = b->drag ? batch_stage_sequence<T, true>(b, st, s, w) : batch_stage_sequence<T, false>(b, st, s, w);
     if (rc) return rc;
   }
   int guard_any = 0;  // outflow guards copy S -> Q after every step
@@ -1737,7 +1758,9 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     u.phi = take(2 * nf * sizeof(double));
     u.fhat = take(4 * nf * sizeof(double));
     u.noise = take(pl->d.n_force_modes > 0 ? N * sizeof(double) : 8);
-    u.fpart = take((size_t)std::max(pl->nblk_proj, pj_max) * 2 * nf * sizeof(double));
+    // forcing partials: the projection's (2 per mode and block), then the synthesis's (5 per block)
+    u.fpart = take(std::max((size_t)std::max(pl->nblk_proj, pj_max) * 2 * nf, (size_t)walk_blocks(pl->d.nx, pl->d.ny, kFsRows, 0) * 5) *
+                   sizeof(double));
     u.kxy = take(2 * nf * sizeof(double));
     u.rng = take(4 * sizeof(uint64_t));
     u.rngws = take(pl->d.n_force_modes > 0 ? rng_layout(N).total : 8);
@@ -1759,6 +1782,9 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
 int hitx_batch_destroy(hitx_batch* b) {
   if (!b) return HITX_OK;
   for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+  if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+  if (b->side) (void)hipStreamDestroy(b->side);
   for (hitx_plan* p : b->plans) hitx_plan_destroy(p);
   delete b;
   return HITX_OK;
@@ -1770,6 +1796,11 @@ int hitx_batch_bind(hitx_batch* b, void* ws, size_t ws_bytes) {
   if (!b || !ws) return fail(HITX_EINVAL, "null argument");
   if (ws_bytes < b->ws_bytes) return fail(HITX_ENOMEM, "batch workspace too small");
   b->ws = ws;
+  if (!b->side) {  // the forcing chain's stream and fork/join events (outside any graph capture)
+    HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
+  }
   return b->esize == 8 ? batch_bind_t<double>(b, ws, true) : batch_bind_t<float>(b, ws, true);
 }
 

@@ -570,7 +570,9 @@ __global__ void __launch_bounds__(kBlock) b_fsynth(const Inst<T>* __restrict__ t
       a3 = b3;
     }
   }
-  block_reduce_write<5, 0>(acc, I.part + (size_t)lb * 5);
+  // partials in fpart (free after the OU update): the stage reduction may
+  // use part meanwhile
+  block_reduce_write<5, 0>(acc, I.fpart + (size_t)lb * 5);
 }
 
 // One block per instance: rms normalisation and the power rescale with the
@@ -582,7 +584,7 @@ __global__ void __launch_bounds__(kBlock) b_falpha(const Inst<T>* __restrict__ t
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
   __shared__ double r[5];
-  reduce_partials<5, 0>(I.part, walk_blocks(I.P.nx, I.P.ny, kFsRows, 0), r);
+  reduce_partials<5, 0>(I.fpart, walk_blocks(I.P.nx, I.P.ny, kFsRows, 0), r);
   __syncthreads();
   if (threadIdx.x != 0) return;
   double* c = I.ctl;
