@@ -1351,6 +1351,11 @@ struct hitx_batch {
   // not read the forcing field; created on the first forced step
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // per stage: the drag row table and the x-sweep on their own stream, beside
+  // the raw mask, dilation and y-sweep (fork after the finalize, the x-sweep
+  // after the dilation, join before the update)
+  hipStream_t side2 = nullptr;
+  hipEvent_t sfork_ev = nullptr, smask_ev = nullptr, sjoin_ev = nullptr;
 };
 
 namespace {
@@ -1544,7 +1549,14 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before
   const bool drag_now = DRAG && st < 3;
   b_stage_finalize<T, DRAG><<<dim3(n, drag_now ? 1 + kFinDragGroups : 1), kBlock, 0, s>>>(tab, st,
                                                                                      DRAG && b->stage_walk ? 1 : 0);
-  if (drag_now) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, s>>>(tab, offs, n, st);
+  // stages 0-2: fork the second stream (drag row table, then the x-sweep)
+  hipStream_t s2 = b->side2;
+  const bool fork = st < 3 && s2 != nullptr;
+  if (fork) {
+    HIPCHK(hipEventRecord(b->sfork_ev, s));
+    HIPCHK(hipStreamWaitEvent(s2, b->sfork_ev, 0));
+  }
+  if (drag_now) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, fork ? s2 : s>>>(tab, offs, n, st);
   if (b->split_sensor) {
     // the tile raw mask writes the transposed state itself, so the dilation
     // is bit work only; after the column-walk raw mask the byte dilation
@@ -1568,38 +1580,48 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before
     return HITX_OK;
   }
   const int sb = (int)b->wave_slice;
+  // the x-sweep on the second stream once the mask is ready, the y-sweep here
+  if (fork) {
+    HIPCHK(hipEventRecord(b->smask_ev, s));
+    HIPCHK(hipStreamWaitEvent(s2, b->smask_ev, 0));
+  }
   // one launch per line form and periodicity present in the batch; each
   // skips the instances of the other periodicity
   for (int axis = 0; axis < 2; ++axis) {
     const int kb = axis == 0 ? OFF_LX : OFF_LY, kw = axis == 0 ? OFF_LXW : OFF_LYW;
+    const hipStream_t ls = (fork && axis == 0) ? s2 : s;
     for (int per = 1; per >= 0; --per) {
       if (!(per ? b->any_per : b->any_nonper)) continue;
       if (b->total[kw] > 0 && axis == 1 && b->slab_y) {
         if (per)
-          b_line_wave_cols<T, true><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
+          b_line_wave_cols<T, true><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, ls>>>(tab, offs, n, st, sb);
         else
-          b_line_wave_cols<T, false><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, s>>>(tab, offs, n, st, sb);
+          b_line_wave_cols<T, false><<<b->total[kw], 64 * kSlabWaves, b->slab_lds, ls>>>(tab, offs, n, st, sb);
       } else if (b->total[kw] > 0) {
         if (per)
-          b_line_wave<T, true><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
+          b_line_wave<T, true><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, ls>>>(tab, offs, n, axis, st, sb);
         else
-          b_line_wave<T, false><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, axis, st, sb);
+          b_line_wave<T, false><<<b->total[kw], 64 * kLineWaves, kLineWaves * sb, ls>>>(tab, offs, n, axis, st, sb);
       }
       if (b->total[kb] > 0) {
         const bool sm = b->line_small[axis];
         if (per) {
           if (sm)
-            b_line<T, true, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+            b_line<T, true, true><<<b->total[kb], kLineBlock, b->line_lds, ls>>>(tab, offs, n, axis, st);
           else
-            b_line<T, true, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+            b_line<T, true, false><<<b->total[kb], kLineBlock, b->line_lds, ls>>>(tab, offs, n, axis, st);
         } else {
           if (sm)
-            b_line<T, false, true><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+            b_line<T, false, true><<<b->total[kb], kLineBlock, b->line_lds, ls>>>(tab, offs, n, axis, st);
           else
-            b_line<T, false, false><<<b->total[kb], kLineBlock, b->line_lds, s>>>(tab, offs, n, axis, st);
+            b_line<T, false, false><<<b->total[kb], kLineBlock, b->line_lds, ls>>>(tab, offs, n, axis, st);
         }
       }
     }
+  }
+  if (fork) {  // the x-derivatives and the drag row table are ready
+    HIPCHK(hipEventRecord(b->sjoin_ev, s2));
+    HIPCHK(hipStreamWaitEvent(s, b->sjoin_ev, 0));
   }
   if (before_update) HIPCHK(hipStreamWaitEvent(s, before_update, 0));  // the forcing field is ready
   int rc = batch_mark(b, s);
@@ -1785,6 +1807,9 @@ int hitx_batch_destroy(hitx_batch* b) {
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   if (b->join_ev) (void)hipEventDestroy(b->join_ev);
   if (b->side) (void)hipStreamDestroy(b->side);
+  for (hipEvent_t e : {b->sfork_ev, b->smask_ev, b->sjoin_ev})
+    if (e) (void)hipEventDestroy(e);
+  if (b->side2) (void)hipStreamDestroy(b->side2);
   for (hitx_plan* p : b->plans) hitx_plan_destroy(p);
   delete b;
   return HITX_OK;
@@ -1796,10 +1821,14 @@ int hitx_batch_bind(hitx_batch* b, void* ws, size_t ws_bytes) {
   if (!b || !ws) return fail(HITX_EINVAL, "null argument");
   if (ws_bytes < b->ws_bytes) return fail(HITX_ENOMEM, "batch workspace too small");
   b->ws = ws;
-  if (!b->side) {  // the forcing chain's stream and fork/join events (outside any graph capture)
+  if (!b->side) {  // the side streams and their fork/join events (outside any graph capture)
     HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
+    HIPCHK(hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&b->sfork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->smask_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->sjoin_ev, hipEventDisableTiming));
   }
   return b->esize == 8 ? batch_bind_t<double>(b, ws, true) : batch_bind_t<float>(b, ws, true);
 }
