@@ -1,14 +1,14 @@
 """Fixed workloads for profiling under rocprofv3 (no timing of its own).
 
   python3 -m hit_solver_amd.profile_workloads batch --steps 8 --tiles 32 64 2 2
-  python3 -m hit_solver_amd.profile_workloads riemann --steps 20
+  python3 -m hit_solver_amd.profile_workloads riemann --steps 20 [--n 1024]
   python3 -m hit_solver_amd.profile_workloads hit --steps 10
 
 batch: 64 forced-HIT instances, 16 each at 128^2, 256^2, 384^2, 512^2, seeds
 0-63, Mach (0.1, 0.25, 0.4)[seed % 3], production settings, FP32, with the
 given launch shapes (rb, rbp, line form, sensor form) instead of the
 autotuner, so every profiled step uses the same kernels.
-riemann: Riemann configuration 3 at 512^2 FP64, the single-plan driver.
+riemann: Riemann configuration 3 at 512^2 (or --n) FP64, the single-plan driver.
 hit: production forced HIT 512^2 FP64, one instance on device control.
 """
 
@@ -25,6 +25,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dtype", default=None)
+    ap.add_argument("--n", type=int, default=512, help="grid size of the riemann workload")
     ap.add_argument("--tiles", type=int, nargs=4, default=(32, 64, 2, 2), metavar=("RB", "RBP", "LINE", "SENSOR"))
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="3 plain device copies of this many MiB first (scale for FETCH_SIZE/WRITE_SIZE)")
@@ -46,7 +47,7 @@ def main(argv=None):
     elif a.workload == "riemann":
         from .riemann import RiemannConfig3, RiemannSolver
 
-        s = RiemannSolver(RiemannConfig3(nx=512, ny=512, shear_threshold=None, progress_every=10**9,
+        s = RiemannSolver(RiemannConfig3(nx=a.n, ny=a.n, shear_threshold=None, progress_every=10**9,
                                          dtype=a.dtype or "fp64"))
         step = lambda: s.step(validate=False)  # noqa: E731
     else:
