@@ -148,6 +148,12 @@ __device__ __forceinline__ float dv<float>(float a, const Divisor<float>& d) {
   return a * d.r;
 }
 
+// Square root in the pointwise algebra: IEEE for double; for float the
+// hardware square root (<= 1 ulp, no correction sequence), as the FP32 mode
+// is held to its error budget.
+__device__ __forceinline__ double sqrt_alg(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_alg(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // a*b + c: fused for float (one rounding; the FP32 mode is held to its error
 // budget), two IEEE operations for double (the library is built without
 // contraction, so the FP64 mode keeps the reference's rounding).  Call sites

@@ -170,7 +170,7 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
       const T c0 = cr.a, c1 = cr.b, c2 = cr.c, c3 = cr.d;
       if (W.own) {
         const int x = W.x;
-        const T cs = sqrt(dv(P.gamma * cu.p, cu.rho));
+        const T cs = sqrt_alg(dv(P.gamma * cu.p, cu.rho));
         mx[0] = fmax(mx[0], absT(cu.rho));
         mx[1] = fmax(mx[1], absT(cu.p));
         const T e = dv(cu.p, cu.rho * P.gm1);
@@ -435,9 +435,9 @@ __device__ __forceinline__ void step_reduce_body(const Params<T>& P, const T* __
   const int N = P.nx * P.ny, stride = nblk * kBlock;
   auto node = [&](T a, T b, T cc, T d) {
     const Prim<T> c = prim(P, a, b, cc, d);
-    const T cs = sqrt((P.gamma * c.p) / c.rho);
-    T sr = (absT(c.u) + cs) / P.dx;
-    sr = sr + (absT(c.v) + cs) / P.dy;
+    const T cs = sqrt_alg(dv(P.gamma * c.p, c.rho));
+    T sr = dv(absT(c.u) + cs, P.dx);
+    sr = sr + dv(absT(c.v) + cs, P.dy);
     v[0] = fmax(v[0], (double)sr);
     v[1] += (double)(c.u * c.u + c.v * c.v);
     v[2] += (double)cs;
@@ -984,7 +984,7 @@ __device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __r
     Lu[t] = pr.u;
     Lv[t] = pr.v;
     if (own) {
-      const T cs = sqrt(dv(P.gamma * pr.p, pr.rho));
+      const T cs = sqrt_alg(dv(P.gamma * pr.p, pr.rho));
       v[R_MAXR] = fmax(v[R_MAXR], (double)absT(pr.rho));
       v[R_MAXP] = fmax(v[R_MAXP], (double)absT(pr.p));
       const T e = dv(pr.p, pr.rho * P.gm1);
