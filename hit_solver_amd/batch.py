@@ -25,7 +25,7 @@ from ._lib import check, lib
 from .forcing import shell_modes
 from .hit2d import HIT2DConfig, _make_equation, create_grid, drag_modes, initialize_hit_2d
 from .riemann import RiemannConfig3, hyperviscosity_enabled_for_config
-from .rng import pcg64_words
+from .rng import pcg64_words, pcg64_words_rewind
 from .runtime import PlanSpec, get_plan, make_desc
 from .spatial_operator import PeriodicHyperviscosity2D
 
@@ -33,6 +33,7 @@ NAN = float("nan")
 # control-scalar slots (include/hitx.h)
 C_ACTIVE, C_TIME, C_TFINAL, C_DT, C_STEP, C_MAXSTEPS = 0, 1, 2, 3, 4, 5
 C_HVAPPS, C_POWER, C_ALPHA, C_NSTEPS = 7, 11, 12, 26
+C_NPRE, C_NPREW = 27, 28  # next step's noise drawn during the last step; the PCG64 words it consumed
 C_FXX, C_FYY, C_FXY, C_PIN, C_PBEFORE, C_ATGT, C_MACH, C_PDES = 18, 19, 20, 21, 22, 23, 24, 25
 BUF_STATE, BUF_NOISE, BUF_CTL, BUF_PHI, BUF_RNG = 0, 1, 2, 3, 4
 
@@ -224,23 +225,28 @@ class BatchRunner:
     def copy_from(self, i, other: "BatchRunner", j):
         """Continue other's instance j as this batch's instance i (same
         configuration): state, control scalars, OU coefficients, noise stream."""
-        with self._ctx():
-            self.q[i].copy_(other.q[j])
-            self.ctl_views[i].copy_(other.ctl_views[j])
-            if self.is_hit[i]:
-                self.phi[i].copy_(other.phi[j])
-                self.rng_state[i].copy_(other.rng_state[j])
+        self.import_instance(i, other.export_instance(j))
 
     def export_instance(self, j):
         """Copies of what continues instance j elsewhere (state, control
         scalars, OU coefficients, noise stream): a few arrays of the state's
         size, so the batch workspace can be freed before the next batch is
-        built."""
+        built.  A step draws the next step's noise while it runs; that draw
+        is not carried over: the noise stream is moved back by the words it
+        consumed, so the next batch draws the same values again."""
         with self._ctx():
             out = {"q": self.q[j].clone(), "ctl": self.ctl_views[j].clone()}
             if self.is_hit[j]:
                 out["phi"] = self.phi[j].clone()
                 out["rng"] = self.rng_state[j].clone()
+        if self.is_hit[j]:
+            self._sync()
+            ctl = out["ctl"].cpu().numpy()
+            if ctl[C_NPRE] != 0.0:
+                w = pcg64_words_rewind(out["rng"].cpu().numpy(), int(ctl[C_NPREW]))
+                out["rng"].copy_(torch.from_numpy(w.view(np.int64).copy()))
+                out["ctl"][C_NPRE] = 0.0
+                out["ctl"][C_NPREW] = 0.0
         return out
 
     def import_instance(self, i, saved):

@@ -1346,11 +1346,12 @@ struct hitx_batch {
   bool timing = false;
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
-  // the forcing chain's own stream (non-blocking, default priority) and the
-  // fork/join events: it runs beside stage 0's sensor and sweeps, which do
-  // not read the forcing field; created on the first forced step
+  // the forcing chain's own stream (non-blocking, default priority) and its
+  // events: the chain runs beside stage 0's sensor and sweeps, which do not
+  // read the forcing field, the next step's noise draw beside stages 1-2;
+  // created when the batch is bound
   hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, pre_ev = nullptr, prejoin_ev = nullptr;
   // per stage: the drag row table and the x-sweep on their own stream, beside
   // the raw mask, dilation and y-sweep (fork after the finalize, the x-sweep
   // after the dilation, join before the update)
@@ -1633,6 +1634,21 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before
   return HITX_OK;
 }
 
+// Noise draw and its projection for every forced instance on stream f:
+// pre = 1 draws the next step's (and marks the instances), pre = 0 only for
+// instances not marked.
+template <class T>
+void batch_draw(hitx_batch* b, hipStream_t f, int pre) {
+  const Inst<T>* tab = btab<T>(b);
+  const int* offs = boffs(b);
+  const int n = b->n;
+  b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, f>>>(tab, offs, n, pre);
+  b_rng_scan<T><<<n, kRngScan, 0, f>>>(tab, pre);
+  b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, f>>>(tab, offs, n, pre);
+  b_rng_finish<T><<<n, 64, 0, f>>>(tab, pre);
+  b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, f>>>(tab, offs, n, pre);
+}
+
 template <class T>
 int batch_step_t(hitx_batch* b, hipStream_t s) {
   const Inst<T>* tab = btab<T>(b);
@@ -1643,11 +1659,12 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
   int* hvany = const_cast<int*>(offs) + (size_t)OFF_KINDS * (n + 1) + OFF_KINDS;
   b_ctl_begin<T><<<n, kBlock, 0, s>>>(tab, hvany);
   LAUNCHCHK();
-  // The forcing chain (noise, projection, OU update, synthesis, rescale)
-  // reads the control scalars b_ctl_begin wrote and writes only forcing
+  // The forcing chain (OU update, synthesis, rescale; the noise draw and its
+  // projection only for instances whose noise the previous step did not
+  // draw) reads the control scalars b_ctl_begin wrote and writes only forcing
   // buffers, its partials included (fpart); stage 0's sensor and sweeps read
   // neither.  So the chain runs on the side stream beside them, and stage 0's
-  // update waits for it.  Captured into a graph, the fork and join become the
+  // update waits for it.  Captured into a graph, the fork and joins become the
   // graph's edges.
   hipEvent_t joined = nullptr;
   if (b->total[OFF_PROJ] > 0) {  // some instance is forced
@@ -1655,11 +1672,7 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
     hipStream_t f = b->side;
     HIPCHK(hipEventRecord(b->fork_ev, s));
     HIPCHK(hipStreamWaitEvent(f, b->fork_ev, 0));
-    b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, f>>>(tab, offs, n);
-    b_rng_scan<T><<<n, kRngScan, 0, f>>>(tab);
-    b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, f>>>(tab, offs, n);
-    b_rng_finish<T><<<n, 64, 0, f>>>(tab);
-    b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, f>>>(tab, offs, n);
+    batch_draw<T>(b, f, 0);  // instances whose noise was not drawn during the previous step
     b_fou<T><<<n, kBlock, 0, f>>>(tab);
     b_frows<T><<<b->total[OFF_FROWS], kBlock, 0, f>>>(tab, offs, n);
     b_fsynth<T><<<b->total[OFF_FS], kBlock, 0, f>>>(tab, offs, n);
@@ -1672,7 +1685,19 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
     const hipEvent_t w = st == 0 ? joined : nullptr;
     int rc = b->drag ? batch_stage_sequence<T, true>(b, st, s, w) : batch_stage_sequence<T, false>(b, st, s, w);
     if (rc) return rc;
+    if (st == 0 && joined) {
+      // the next step's noise and projection on the side stream beside
+      // stages 1-2 (the noise and the partials are free once the rescale has
+      // read the synthesis sums; the stages read only fx, fy); joined before
+      // the step's control end
+      HIPCHK(hipEventRecord(b->pre_ev, s));
+      HIPCHK(hipStreamWaitEvent(b->side, b->pre_ev, 0));
+      batch_draw<T>(b, b->side, 1);
+      LAUNCHCHK();
+      HIPCHK(hipEventRecord(b->prejoin_ev, b->side));
+    }
   }
+  if (joined) HIPCHK(hipStreamWaitEvent(s, b->prejoin_ev, 0));
   int guard_any = 0;  // outflow guards copy S -> Q after every step
   for (const hitx_plan* pl : b->plans) guard_any |= (!pl->d.periodic && pl->d.guard_cells > 0) ? 1 : 0;
   b_finish<T><<<b->total[OFF_T16], kBlock, 0, s>>>(tab, offs, n, guard_any);
@@ -1804,8 +1829,8 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
 int hitx_batch_destroy(hitx_batch* b) {
   if (!b) return HITX_OK;
   for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
-  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
-  if (b->join_ev) (void)hipEventDestroy(b->join_ev);
+  for (hipEvent_t e : {b->fork_ev, b->join_ev, b->pre_ev, b->prejoin_ev})
+    if (e) (void)hipEventDestroy(e);
   if (b->side) (void)hipStreamDestroy(b->side);
   for (hipEvent_t e : {b->sfork_ev, b->smask_ev, b->sjoin_ev})
     if (e) (void)hipEventDestroy(e);
@@ -1825,6 +1850,8 @@ int hitx_batch_bind(hitx_batch* b, void* ws, size_t ws_bytes) {
     HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->pre_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->prejoin_ev, hipEventDisableTiming));
     HIPCHK(hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&b->sfork_ev, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->smask_ev, hipEventDisableTiming));

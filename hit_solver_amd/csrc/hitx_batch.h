@@ -21,6 +21,8 @@ enum {
   C_ACTIVE = 0, C_TIME, C_TFINAL, C_DT, C_STEP, C_MAXSTEPS, C_HVNOW, C_HVAPPS,
   C_MAXSR, C_SUV, C_SC, C_POWER, C_ALPHA, C_DECAY, C_INCR, C_SCALE,
   C_SFF, C_SP, C_FXX, C_FYY, C_FXY, C_PIN, C_PBEFORE, C_ATGT, C_MACH, C_PDES, C_NSTEPS,
+  C_NPRE,   // 1: the noise and projection partials of the next step are drawn (prefetched)
+  C_NPREW,  // PCG64 words that prefetched draw consumed (a copy of the instance rewinds by them)
   C_COUNT = 32
 };
 // per-instance run parameters (double, constant); NaN encodes "None"
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(kBlock) b_stage_reduce(const Inst<T>* __restri
   HITX_BATCH_INST(OFF_SR)
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   if (DRAG && !I.P.drag_on) return;  // instances without drag reduce on tiles (b_stage_tile)
-  stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, lb);
+  stage_reduce_body<T, DRAG>(I.P, stage_in(I, st), st < 3 ? 2 : 0, I.part, lb);
 }
 
 // The stage reduction (with the drag projection when DRAG) runs on the
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(kBlock) b_stage_tile(const Inst<T>* __restrict
   if (walk && I.P.drag_on) return;  // reduced by the column walk (b_stage_reduce)
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
   const int by = lb / ntx, bx = lb - by * ntx;
-  stage_tile_body<T, DRAG>(I.P, stage_in(I, st), st < 3, I.part, bx, by, lb);
+  stage_tile_body<T, DRAG>(I.P, stage_in(I, st), st < 3 ? 2 : 0, I.part, bx, by, lb);
 }
 
 // Grid (instances, 1 + kFinDragGroups with drag): block y = 0 reduces the
@@ -412,46 +414,62 @@ __global__ void b_ctl_end(const Inst<T>* __restrict__ tab, int ninst, int* __res
 }
 
 // ---------------------------------------------------------------- forcing
+// (after b_rng_finish: with pre = 1 the mark is set and this projection is
+// part of the prefetch, so only pre = 0 reads it)
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_fproject(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
-                                                     int ninst) {
+                                                     int ninst, int pre) {
   HITX_BATCH_INST(OFF_PROJ)
-  if (I.kpar[K_FORCE] == 0.0) return;
+  if (rng_skip(I.ctl, I.kpar, pre)) return;
   fproject_body<T>(I.P, I.noise, I.fpart, lb);
 }
 
 // The instance's next nx*ny standard normals (OOP/forcing.py:77), in place
-// of the host draw: four launches, see hitx_rng.h.
+// of the host draw: four launches, see hitx_rng.h; then their projection.
+// The draw for step n+1 runs during step n (pre = 1, after stage 0's update,
+// beside stages 1-2) and marks the instance (C_NPRE); at the start of a step
+// the same launches with pre = 0 draw only for instances not marked (the
+// first step, a copied instance), and the OU update clears the mark.
+// The noise depends only on the PCG64 state, so the values are those of a
+// draw at the start of the step.
+__device__ __forceinline__ bool rng_skip(const double* ctl, const double* kpar, int pre) {
+  return kpar[K_FORCE] == 0.0 || (pre == 0 && ctl[C_NPRE] != 0.0);
+}
 template <class T>
 __global__ void __launch_bounds__(kRngAttemptThreads) b_rng_attempts(const Inst<T>* __restrict__ tab,
-                                                                     const int* __restrict__ offs, int ninst) {
+                                                                     const int* __restrict__ offs, int ninst,
+                                                                     int pre) {
   HITX_BATCH_INST(OFF_RNG)
-  if (I.kpar[K_FORCE] == 0.0) return;
+  if (rng_skip(I.ctl, I.kpar, pre)) return;
   __shared__ __align__(16) uint8_t lds[kRngAttemptThreads * 3 * kRngChunk];
   rng_attempt_body(I.R, lb * kRngAttemptThreads, lds + threadIdx.x * 3 * kRngChunk);
 }
 
 template <class T>
-__global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict__ tab) {
+__global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict__ tab, int pre) {
   const Inst<T>& I = tab[blockIdx.x];
-  if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
+  if (I.ctl[C_ACTIVE] == 0.0 || rng_skip(I.ctl, I.kpar, pre)) return;
   rng_scan_body(I.R, 0);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kRngScatterThreads) b_rng_scatter(const Inst<T>* __restrict__ tab,
-                                                                    const int* __restrict__ offs, int ninst) {
+                                                                    const int* __restrict__ offs, int ninst, int pre) {
   __shared__ double lds[kRngScatterThreads / 64][64 * kRngChunk];
   HITX_BATCH_INST(OFF_RNGS)
-  if (I.kpar[K_FORCE] == 0.0) return;
+  if (rng_skip(I.ctl, I.kpar, pre)) return;
   rng_scatter_body(I.R, lb * kRngScatterThreads + threadIdx.x, lds[threadIdx.x >> 6]);
 }
 
 template <class T>
-__global__ void b_rng_finish(const Inst<T>* __restrict__ tab) {
+__global__ void b_rng_finish(const Inst<T>* __restrict__ tab, int pre) {
   const Inst<T>& I = tab[blockIdx.x];
-  if (threadIdx.x != 0 || I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
-  rng_finish_body(I.R);
+  if (threadIdx.x != 0 || I.ctl[C_ACTIVE] == 0.0 || rng_skip(I.ctl, I.kpar, pre)) return;
+  const unsigned long long words = rng_finish_body(I.R);
+  if (pre) {
+    I.ctl[C_NPREW] = (double)words;
+    I.ctl[C_NPRE] = 1.0;
+  }
 }
 
 // One block per instance: sum the projection partials, OU recurrence of the
@@ -462,11 +480,29 @@ __global__ void __launch_bounds__(kBlock) b_fou(const Inst<T>* __restrict__ tab)
   if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
   const int nm = I.P.nforce;
   const double decay = I.ctl[C_DECAY], incr = I.ctl[C_INCR];
+  const int nb = I.nblk_proj;
   for (int m = threadIdx.x; m < nm; m += blockDim.x) {
+    // the partials in block order, eight blocks' loads in flight at a time
     double re = 0.0, im = 0.0;
-    for (int b = 0; b < I.nblk_proj; ++b) {
-      re += I.fpart[((size_t)b * nm + m) * 2];
-      im += I.fpart[((size_t)b * nm + m) * 2 + 1];
+    const double* fp = I.fpart + (size_t)m * 2;
+    const size_t stride = (size_t)nm * 2;
+    int b = 0;
+    for (; b + 8 <= nb; b += 8) {
+      double r8[8], i8[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        r8[j] = fp[(size_t)(b + j) * stride];
+        i8[j] = fp[(size_t)(b + j) * stride + 1];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        re += r8[j];
+        im += i8[j];
+      }
+    }
+    for (; b < nb; ++b) {
+      re += fp[(size_t)b * stride];
+      im += fp[(size_t)b * stride + 1];
     }
     const double pr = decay * I.phi[2 * m] + incr * re;
     const double pi = decay * I.phi[2 * m + 1] + incr * im;
@@ -478,6 +514,7 @@ __global__ void __launch_bounds__(kBlock) b_fou(const Inst<T>* __restrict__ tab)
     I.fhat[2 * nm + 2 * m] = 0.0 * pr - (-kx) * pi;  // (-1j*kx) * phi
     I.fhat[2 * nm + 2 * m + 1] = 0.0 * pi + (-kx) * pr;
   }
+  if (threadIdx.x == 0) I.ctl[C_NPRE] = 0.0;  // the drawn noise is used
 }
 
 template <class T>

@@ -105,9 +105,21 @@ __device__ __forceinline__ double wave_reduce(double a) {
 // Column walk (halo 1, P.rb rows per wave).  The drag projection is
 // separable: each lane sums rho*u and rho*v times cos/sin(ky*y) for
 // ky = 0..dky over its rows, and multiplies by exp(-i*kx*x) once at the end.
+// enforce_first: 0 the state as given, 1 enforce first (OOP/spatial_operator.py
+// rhs: the sensor sees enforce(q)), 2 enforce first, the state being an RK
+// stage state that the kernel which wrote it already enforced.  With 2 the
+// FP32 reductions take the state as given (enforcing an enforced state again
+// moves a node by at most an ulp): one set of primitives per node and the
+// cooling pressure shared.  FP64 enforces first as the reference does.
+template <class T>
+__device__ __forceinline__ int reduce_enforce_first(int enforce_first) {
+  return (sizeof(T) == 4 && enforce_first == 2) ? 0 : (enforce_first != 0 ? 1 : 0);
+}
+
 template <class T, bool DRAG>
 __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* __restrict__ q,
-                                                  int enforce_first, double* __restrict__ part, int bid) {
+                                                  int enforce_first_in, double* __restrict__ part, int bid) {
+  const int enforce_first = reduce_enforce_first<T>(enforce_first_in);
   constexpr int NR = DRAG ? (R_DRAG + kDragCols) : R_DRAG;
   double v[R_DRAG];
 #pragma unroll
@@ -200,7 +212,7 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
         }
         if (P.cool_on || DRAG) {
           const T a = c0, b = c1, c = c2, d = c3;
-          if (P.cool_on) v[R_PSUM] += (double)prim(P, a, b, c, d).p;
+          if (P.cool_on) v[R_PSUM] += (double)(enforce_first ? prim(P, a, b, c, d).p : cu.p);
           if (DRAG) {
             const double fb = (double)b, fc = (double)c;
             bs[0][0] += fb;
@@ -953,8 +965,9 @@ __device__ __forceinline__ void drag_tile_project(const Params<T>& P, const T* _
 }
 
 template <class T, bool DRAG>
-__device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first,
+__device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __restrict__ q, int enforce_first_in,
                                                 double* __restrict__ part, int bx, int by, int bid) {
+  const int enforce_first = reduce_enforce_first<T>(enforce_first_in);
   constexpr int NR = DRAG ? R_DRAG + kDragCols : R_DRAG;  // partial row: the scalars, then the drag projection
   constexpr int EX = kSTX + 2, EY = kSTY + 2;
   __shared__ T Lu[EX * EY], Lv[EX * EY];
@@ -978,9 +991,10 @@ __device__ __forceinline__ void stage_tile_body(const Params<T>& P, const T* __r
     T a, b, cc, d;
     load4(q, N, gy * nx + gx, a, b, cc, d);
     const bool own = r >= 1 && r <= kSTY && c >= 1 && c <= kSTX && uy < ny && ux < nx;
-    if (own && P.cool_on) v[R_PSUM] += (double)prim(P, a, b, cc, d).p;
+    if (own && P.cool_on && enforce_first) v[R_PSUM] += (double)prim(P, a, b, cc, d).p;
     if (enforce_first) enforce(P, a, b, cc, d);
     const Prim<T> pr = prim(P, a, b, cc, d);
+    if (own && P.cool_on && !enforce_first) v[R_PSUM] += (double)pr.p;
     Lu[t] = pr.u;
     Lv[t] = pr.v;
     if (own) {
@@ -1754,38 +1768,59 @@ __global__ void __launch_bounds__(kLineBlock, 4)
 }
 
 // ================================================================ stage update
-// Periodic Navier-Stokes terms at one node (OOP/equations.py:232-267): nested
-// second-order central differences on the 13-point diamond.  U, V, Tt are
-// the node primitives u, v, p/(rho R) of a tile with a 2-node halo in LDS
-// (row stride kVS); (ly, lx) is the node's position in that tile.
+// Periodic Navier-Stokes terms (OOP/equations.py:232-267) in the reference's
+// flux form: the stresses tau_xx, tau_yy, tau_xy and the energy fluxes
+// u tau_xx + v tau_xy + k dT/dx, u tau_xy + v tau_yy + k dT/dy are central
+// differences of the primitives at each node, and the viscous terms their
+// central differences.  A tile forms the five flux values once per node of
+// its one-node halo ring (from the primitives u, v, p/(rho R) of its
+// two-node halo in LDS), then each node differences its neighbours' values:
+// the same expressions, evaluated once instead of at every neighbour.
 // Update tiles are kUTX x kUTY nodes, one thread each: a row of the tile is
 // one 128-byte line of an FP32 plane (two of an FP64 plane).
 constexpr int kUTX = 32, kUTY = 16, kUBlock = kUTX * kUTY;
-constexpr int kVW = kUTX + 4, kVH = kUTY + 4;  // viscous tile with its 2-node halo
+constexpr int kVW = kUTX + 4, kVH = kUTY + 4;  // primitives tile with its 2-node halo
 constexpr int kVS = kVW + 1;
+constexpr int kFW = kUTX + 2, kFH = kUTY + 2;  // flux tile with its 1-node halo
+constexpr int kFS = kFW + 1;
 template <class T>
-__device__ __forceinline__ void viscous_lds(const Params<T>& P, const T* U, const T* V, const T* Tt, int ly, int lx,
-                                            T out[4]) {
-  auto at = [&](const T* a, int dy, int dx) { return a[(ly + dy) * kVS + lx + dx]; };
-  auto ux = [&](int a, int b) { return dv(at(U, a, b + 1) - at(U, a, b - 1), P.two_dx); };
-  auto uy = [&](int a, int b) { return dv(at(U, a + 1, b) - at(U, a - 1, b), P.two_dy); };
-  auto vx = [&](int a, int b) { return dv(at(V, a, b + 1) - at(V, a, b - 1), P.two_dx); };
-  auto vy = [&](int a, int b) { return dv(at(V, a + 1, b) - at(V, a - 1, b), P.two_dy); };
-  auto txx = [&](int a, int b) { return madd(-P.tt_mu, ux(a, b) + vy(a, b), P.two_mu * ux(a, b)); };
-  auto tyy = [&](int a, int b) { return madd(-P.tt_mu, ux(a, b) + vy(a, b), P.two_mu * vy(a, b)); };
-  auto txy = [&](int a, int b) { return P.mu * (uy(a, b) + vx(a, b)); };
-  auto ex = [&](int a, int b) {
-    const T tx = dv(at(Tt, a, b + 1) - at(Tt, a, b - 1), P.two_dx);
-    return madd(P.kcond, tx, madd(at(V, a, b), txy(a, b), at(U, a, b) * txx(a, b)));
-  };
-  auto ey = [&](int a, int b) {
-    const T ty = dv(at(Tt, a + 1, b) - at(Tt, a - 1, b), P.two_dy);
-    return madd(P.kcond, ty, madd(at(V, a, b), tyy(a, b), at(U, a, b) * txy(a, b)));
-  };
+struct ViscLds {
+  T U[kVH * kVS], V[kVH * kVS], Tt[kVH * kVS];                    // primitives (2-node halo)
+  T xx[kFH * kFS], yy[kFH * kFS], xy[kFH * kFS], ex[kFH * kFS], ey[kFH * kFS];  // fluxes (1-node halo)
+};
+// The fluxes at the kFW x kFH halo-ring nodes (the primitives in LDS); the
+// caller synchronises before and after.
+template <class T>
+__device__ __forceinline__ void viscous_fluxes(const Params<T>& P, ViscLds<T>& L) {
+  for (int i = threadIdx.x; i < kFW * kFH; i += kUBlock) {
+    const int fr = i / kFW, fc = i - fr * kFW;
+    const int b = (fr + 1) * kVS + (fc + 1);  // the node in the primitives tile
+    const T ux = dv(L.U[b + 1] - L.U[b - 1], P.two_dx);
+    const T uy = dv(L.U[b + kVS] - L.U[b - kVS], P.two_dy);
+    const T vx = dv(L.V[b + 1] - L.V[b - 1], P.two_dx);
+    const T vy = dv(L.V[b + kVS] - L.V[b - kVS], P.two_dy);
+    const T txx = madd(-P.tt_mu, ux + vy, P.two_mu * ux);
+    const T tyy = madd(-P.tt_mu, ux + vy, P.two_mu * vy);
+    const T txy = P.mu * (uy + vx);
+    const T tx = dv(L.Tt[b + 1] - L.Tt[b - 1], P.two_dx);
+    const T ty = dv(L.Tt[b + kVS] - L.Tt[b - kVS], P.two_dy);
+    const T u = L.U[b], v = L.V[b];
+    const int f = fr * kFS + fc;
+    L.xx[f] = txx;
+    L.yy[f] = tyy;
+    L.xy[f] = txy;
+    L.ex[f] = madd(P.kcond, tx, madd(v, txy, u * txx));
+    L.ey[f] = madd(P.kcond, ty, madd(v, tyy, u * txy));
+  }
+}
+// Viscous terms at the tile node (ty, tx) from the fluxes in LDS.
+template <class T>
+__device__ __forceinline__ void viscous_div(const Params<T>& P, const ViscLds<T>& L, int ty, int tx, T out[4]) {
+  const int c = (ty + 1) * kFS + (tx + 1);
   out[0] = T(0);
-  out[1] = dv(txx(0, 1) - txx(0, -1), P.two_dx) + dv(txy(1, 0) - txy(-1, 0), P.two_dy);
-  out[2] = dv(txy(0, 1) - txy(0, -1), P.two_dx) + dv(tyy(1, 0) - tyy(-1, 0), P.two_dy);
-  out[3] = dv(ex(0, 1) - ex(0, -1), P.two_dx) + dv(ey(1, 0) - ey(-1, 0), P.two_dy);
+  out[1] = dv(L.xx[c + 1] - L.xx[c - 1], P.two_dx) + dv(L.xy[c + kFS] - L.xy[c - kFS], P.two_dy);
+  out[2] = dv(L.xy[c + 1] - L.xy[c - 1], P.two_dx) + dv(L.yy[c + kFS] - L.yy[c - kFS], P.two_dy);
+  out[3] = dv(L.ex[c + 1] - L.ex[c - 1], P.two_dx) + dv(L.ey[c + kFS] - L.ey[c - kFS], P.two_dy);
 }
 
 // Primitives u, v, p/(rho R) of the kVW x kVH tile around a kUTX x kUTY
@@ -1843,7 +1878,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
                                             const T* __restrict__ fy, const double* __restrict__ scal,
                                             const double* __restrict__ drows, T dt, T* __restrict__ out, int bx,
                                             int by, T fsc = T(1), T fal = T(1)) {
-  __shared__ T sU[kVH * kVS], sV[kVH * kVS], sT[kVH * kVS];
+  __shared__ ViscLds<T> sVis;
   __shared__ T sD[4][kUTX][kUTY + 1];
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x0 = bx * kUTX, y0 = by * kUTY;
@@ -1924,16 +1959,20 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       dsrc[3] = madd(st.v, d2, st.u * d1);
     }
   }
-  if (visc) viscous_tile_store(P, vt, sU, sV, sT);
+  if (visc) viscous_tile_store(P, vt, sVis.U, sVis.V, sVis.Tt);
   if (mode != MODE_VISC) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) sD[c][threadIdx.x / kUTY][threadIdx.x % kUTY] = dsh[c];
   }
   __syncthreads();
+  if (visc) {
+    viscous_fluxes(P, sVis);
+    __syncthreads();
+  }
   if (!in) return;
   if (mode == MODE_VISC) {  // standalone viscous_rhs of qs
     T vv[4];
-    viscous_lds(P, sU, sV, sT, ty + 2, tx + 2, vv);
+    viscous_div(P, sVis, ty, tx, vv);
 #pragma unroll
     for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = vv[c];
     return;
@@ -1952,7 +1991,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   }
   if (P.visc_on) {
     T vv[4];
-    viscous_lds(P, sU, sV, sT, ty + 2, tx + 2, vv);
+    viscous_div(P, sVis, ty, tx, vv);
 #pragma unroll
     for (int c = 0; c < 4; ++c) Lv[c] = Lv[c] + vv[c];
   }

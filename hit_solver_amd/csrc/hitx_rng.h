@@ -389,21 +389,26 @@ __device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double*
 }
 
 // Last step (one thread): advance the state, or regenerate serially.
-__device__ __forceinline__ void rng_finish_body(const RngJob& J) {
+// Returns the number of 64-bit words the request consumed.
+__device__ __forceinline__ unsigned long long rng_finish_body(const RngJob& J) {
   Pcg g = pcg_load(J.state);
+  unsigned long long words = 0;
   if (J.stat[0] == 0) {
-    g.s = pcg_advance(g.s, g.inc, (unsigned long long)(unsigned)J.stat[1]);
+    words = (unsigned long long)(unsigned)J.stat[1];
+    g.s = pcg_advance(g.s, g.inc, words);
   } else {
     for (long long k = 0; k < J.n;) {
       const uint64_t w = g.next();
       const ZigAttempt a = zig_attempt<false>(w, g);
       for (int j = 1; j < a.len; ++j) g.next();
+      words += (unsigned long long)a.len;
       if (a.out) J.out[k++] = a.v;
     }
   }
   J.state[0] = (uint64_t)(g.s >> 64);
   J.state[1] = (uint64_t)g.s;
   J.stat[0] = 0;
+  return words;
 }
 
 constexpr int kRngAttemptThreads = 256;

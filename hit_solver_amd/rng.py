@@ -24,6 +24,36 @@ def pcg64_words(bit_generator) -> np.ndarray:
     return np.array([s >> 64, s & _MASK64, inc >> 64, inc & _MASK64], dtype=np.uint64)
 
 
+_MASK128 = (1 << 128) - 1
+# the PCG64 LCG multiplier (O'Neill's 128-bit default, as NumPy's PCG64 uses)
+_PCG_MULT = 0x2360ED051FC65DA44385DF649FCCF645
+
+
+def pcg64_advance(state: int, inc: int, delta: int) -> int:
+    """State of the PCG64 LCG s -> s*M + inc after `delta` steps (delta taken
+    mod 2^128, so a negative delta steps back), by the square-and-multiply
+    jump of O'Neill's PCG paper (Brown's arbitrary-stride LCG skip)."""
+    delta &= _MASK128
+    cm, cp, am, ap = _PCG_MULT, inc & _MASK128, 1, 0
+    while delta:
+        if delta & 1:
+            am = (am * cm) & _MASK128
+            ap = (ap * cm + cp) & _MASK128
+        cp = ((cm + 1) * cp) & _MASK128
+        cm = (cm * cm) & _MASK128
+        delta >>= 1
+    return (am * state + ap) & _MASK128
+
+
+def pcg64_words_rewind(words, k: int) -> np.ndarray:
+    """4-word PCG64 state (state high, state low, increment high, increment
+    low) moved back by k 64-bit outputs."""
+    w = np.asarray(words).view(np.uint64)
+    s, inc = (int(w[0]) << 64) | int(w[1]), (int(w[2]) << 64) | int(w[3])
+    s = pcg64_advance(s, inc, -int(k))
+    return np.array([s >> 64, s & _MASK64, w[2], w[3]], dtype=np.uint64)
+
+
 class DeviceNormalStream:
     """The stream of ``numpy.random.default_rng(seed).standard_normal`` with
     its state on the device.  ``normals(shape)`` returns the next values as a

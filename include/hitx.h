@@ -322,11 +322,15 @@ int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
  *   16 correlation time, 17 forcing on, 18 tfinal, 19 max steps.
  * Control scalars (HITX_BUF_CTL, doubles): 0 active, 1 time, 2 tfinal, 3 dt, 4 step,
  *   7 hyperviscosity applications, 11 target power, 12 alpha, 18-21 Fxx Fyy Fxy P_in,
- *   22 power before rescale, 23 alpha target, 24 Mach, 25 desired power, 26 n steps (fixed dt).
+ *   22 power before rescale, 23 alpha target, 24 Mach, 25 desired power, 26 n steps (fixed dt),
+ *   27 the next step's noise is drawn, 28 PCG64 words that draw consumed.
  * The caller writes each instance's initial state (HITX_BUF_STATE), control
  * scalars and, for forced HIT, the PCG64 state of its noise stream
- * (HITX_BUF_RNG, 4 uint64 words as for hitx_normals); each step draws the
- * instance's next nx*ny normals into HITX_BUF_NOISE on the device. */
+ * (HITX_BUF_RNG, 4 uint64 words as for hitx_normals); a step draws the
+ * instance's normals of the next step into HITX_BUF_NOISE on the device while
+ * it runs (slot 27 set), and a step that finds slot 27 clear draws its own
+ * first.  To continue an instance elsewhere with slot 27 set, move its PCG64
+ * state back by slot 28's words and clear slot 27. */
 #define HITX_KPAR 24
 #define HITX_CTL 32
 enum { HITX_BUF_STATE = 0, HITX_BUF_NOISE, HITX_BUF_CTL, HITX_BUF_PHI, HITX_BUF_RNG, HITX_BUF_FX, HITX_BUF_FY };
