@@ -319,12 +319,24 @@ class BatchRunner:
         reductions and of the forcing projection, the line-sweep form (0 auto,
         1 block per line, 2 wavefront per line, 3 wavefront per line with the
         y-lines read from the state through an LDS slab instead of a
-        transposed copy) and the sensor form (0 auto,
+        transposed copy, 4 wavefront per line with the x-sweep and the stage
+        update of each row panel in one launch where the batch allows it:
+        FP32, periodic grids, both line lengths in the wave form; otherwise
+        as 2) and the sensor form (0 auto,
         1 fused tile, 2 split with the column-walk raw mask, 3 split with the
         tile raw mask)."""
         torch.cuda.synchronize()
         check(lib().hitx_batch_tune(self.handle, int(rb), int(rbp), int(line_form), int(sensor_form)))
         self._graph = None  # grid sizes changed
+
+    def forms(self):
+        """Launch forms in effect: dict of booleans (x-sweep fused into the
+        update, x- and y-lines in the wave form, slab y-sweep, split sensor,
+        drag stage sums by column walk)."""
+        out = (ctypes.c_int32 * 6)()
+        check(lib().hitx_batch_forms(self.handle, ctypes.cast(out, ctypes.c_void_p)))
+        keys = ("xfused", "wave_x", "wave_y", "slab_y", "split_sensor", "stage_walk")
+        return {k: bool(v) for k, v in zip(keys, out)}
 
     def set_stage_form(self, form: int):
         """Form of the drag stage reductions: 0 auto (tiles while the batch has
@@ -350,7 +362,7 @@ class BatchRunner:
         on a copy of the batch's device state, restore the state and keep the
         fastest candidate.  Returns (best, {candidate: ms per step})."""
         if candidates is None:
-            candidates = [(rb, rbp, lf, sf) for lf in (1, 2) for sf in (1, 2, 3) for rb in (4, 16, 32)
+            candidates = [(rb, rbp, lf, sf) for lf in (1, 2, 4) for sf in (1, 2, 3) for rb in (4, 16, 32)
                           for rbp in ((16, 64) if any(self.is_hit) else (16,))]
         torch.cuda.synchronize()
         backup = self.ws.clone()

@@ -1336,7 +1336,10 @@ struct hitx_batch {
   std::vector<int> offs;  // OFF_KINDS * (n + 1)
   size_t line_lds = 0, sensor_lds = 0, proj_lds = 0, wave_slice = 16;
   bool line_small[2] = {true, true};  // every block-form line of the axis in one-chunk shape
-  int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave, 3 wave with the slab y-sweep
+  int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave, 3 wave with the slab y-sweep,
+                       // 4 wave with the x-sweep fused into the stage update (where the batch allows it)
+  bool xfused = false;  // stages run the y-sweep, then b_xupdate (x-sweep + update per row panel)
+  size_t xu_lds = 0;    // dynamic LDS of b_xupdate
   bool wave_x = false, wave_y = false, slab_y = false;
   size_t slab_lds = 0;  // dynamic LDS of the slab y-sweep (slab or slices, whichever is larger)
   int sensor_form = 0;  // 0 auto (split for >= 1M nodes), 1 fused tile, 2 split (column walk), 3 split (tiles)
@@ -1382,7 +1385,8 @@ int batch_blocks(const hitx_batch* b, const hitx_plan* pl, int kind) {
     case OFF_DB: return (((nx + kDBCols - 1) / kDBCols) * ((ny + kDBRows - 1) / kDBRows) + kDBWaves - 1) / kDBWaves;
     case OFF_LX: return wx ? 0 : ny;
     case OFF_LY: return wy ? 0 : nx;
-    case OFF_LXW: return wx ? (ny + kLineWaves - 1) / kLineWaves : 0;
+    case OFF_LXW: return (wx && !b->xfused) ? (ny + kLineWaves - 1) / kLineWaves : 0;
+    case OFF_XU: return b->xfused ? (ny + kXUWaves - 1) / kXUWaves : 0;
     case OFF_LYW: return wy ? (b->slab_y ? (nx + kSlabWaves - 1) / kSlabWaves : (nx + kLineWaves - 1) / kLineWaves) : 0;
     case OFF_T16: return ((nx + 15) / 16) * ((ny + 15) / 16);
     case OFF_PROJ: return pl->d.n_force_modes > 0 ? pl->nblk_proj : 0;
@@ -1410,6 +1414,14 @@ void batch_offsets(hitx_batch* b) {
   b->wave_y = b->line_form >= 2 || (b->line_form == 0 && ly >= kWaveFormMinLines);
   if (b->line_form == 1) b->wave_x = b->wave_y = false;
   b->slab_y = b->wave_y && b->line_form == 3;
+  // the fused x-sweep + update (line form 4 only: measured slower than the
+  // separate launches on the 64-instance workload, so never chosen by auto):
+  // FP32, every grid periodic with both line lengths in the wave form, the
+  // y-sweep on the transposed state
+  bool fusable = b->esize == 4 && b->xu_lds > 0 && b->wave_x && b->wave_y && !b->slab_y && b->line_form == 4;
+  for (const hitx_plan* pl : b->plans)
+    fusable = fusable && pl->d.periodic && line_wave_ok(pl->d.nx) && line_wave_ok(pl->d.ny);
+  b->xfused = fusable;
   b->offs.assign((size_t)OFF_KINDS * (n + 1), 0);
   for (int k = 0; k < OFF_KINDS; ++k) {
     int acc = 0;
@@ -1452,6 +1464,7 @@ size_t batch_offs_capacity(const hitx_batch* b) {
              2 * ((rng_chunks(pl->N) + kRngScatterThreads - 1) / kRngScatterThreads) +
              (rng_chunks(pl->N) + kRngAttemptThreads - 1) / kRngAttemptThreads + walk_blocks(nx, ny, kFsRows, 0);
     cap += (size_t)(2 * ny + kBlock - 1) / kBlock;
+    cap += (size_t)(ny + kXUWaves - 1) / kXUWaves;
   }
   return cap;
 }
@@ -1550,9 +1563,10 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before
   const bool drag_now = DRAG && st < 3;
   b_stage_finalize<T, DRAG><<<dim3(n, drag_now ? 1 + kFinDragGroups : 1), kBlock, 0, s>>>(tab, st,
                                                                                      DRAG && b->stage_walk ? 1 : 0);
-  // stages 0-2: fork the second stream (drag row table, then the x-sweep)
+  // stages 0-2: fork the second stream (drag row table, then the x-sweep
+  // unless it is fused into the update)
   hipStream_t s2 = b->side2;
-  const bool fork = st < 3 && s2 != nullptr;
+  const bool fork = st < 3 && s2 != nullptr && (drag_now || !b->xfused);
   if (fork) {
     HIPCHK(hipEventRecord(b->sfork_ev, s));
     HIPCHK(hipStreamWaitEvent(s2, b->sfork_ev, 0));
@@ -1581,6 +1595,22 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before
     return HITX_OK;
   }
   const int sb = (int)b->wave_slice;
+  if constexpr (sizeof(T) == 4) {
+    if (b->xfused) {  // the y-sweep, then the x-sweep and the update per row panel
+      b_line_wave<T, true><<<b->total[OFF_LYW], 64 * kLineWaves, kLineWaves * sb, s>>>(tab, offs, n, 1, st, sb);
+      if (fork) {  // the drag row table is ready
+        HIPCHK(hipEventRecord(b->sjoin_ev, s2));
+        HIPCHK(hipStreamWaitEvent(s, b->sjoin_ev, 0));
+      }
+      if (before_update) HIPCHK(hipStreamWaitEvent(s, before_update, 0));  // the forcing field is ready
+      int rc = batch_mark(b, s);
+      if (rc) return rc;
+      b_xupdate<T><<<b->total[OFF_XU], 64 * kXUWaves, b->xu_lds, s>>>(tab, offs, n, st, sb);
+      if ((rc = batch_mark(b, s))) return rc;
+      LAUNCHCHK();
+      return HITX_OK;
+    }
+  }
   // the x-sweep on the second stream once the mask is ready, the y-sweep here
   if (fork) {
     HIPCHK(hipEventRecord(b->smask_ev, s));
@@ -1723,6 +1753,12 @@ int batch_set_lds(hitx_batch* b) {
   if (b->line_lds > 65536) HIPCHK(allow_line_lds<T>(b->line_lds));
   if (b->slab_lds > (size_t)maxsh) return fail(HITX_EINVAL, "grid column too long for the slab y-sweep");
   if (b->slab_lds > 65536) HIPCHK(allow_slab_lds<T>(b->slab_lds));
+  if constexpr (sizeof(T) == 4) {
+    if (b->xu_lds > (size_t)maxsh) b->xu_lds = 0;  // no fused form for this batch
+    if (b->xu_lds > 65536)
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(&b_xupdate<float>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->xu_lds));
+  }
   return HITX_OK;
 }
 
@@ -1773,6 +1809,8 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
   for (const hitx_plan* pl : b->plans)
     if (line_wave_ok(pl->d.ny))
       b->slab_lds = std::max(b->slab_lds, (size_t)4 * kSlabWaves * slab_cstride(pl->d.ny) * b->esize);
+  if (b->esize == 4)
+    b->xu_lds = std::max((size_t)kXUWaves * b->wave_slice, sizeof(UpdLds<float, kXTX, kXUWaves>));
   int rc = b->esize == 8 ? batch_set_lds<double>(b) : batch_set_lds<float>(b);
   if (rc) {
     std::string keep = g_err;
@@ -1883,8 +1921,10 @@ int hitx_batch_tune(hitx_batch* b, int32_t rb, int32_t rbp, int32_t line_form, i
   if (!b) return fail(HITX_EINVAL, "null batch");
   if ((rb != 0 && (rb < kTuneRbMin || rb > kTuneRbMax)) || (rbp != 0 && (rbp < kTuneRbpMin || rbp > kTuneRbpMax)))
     return fail(HITX_EINVAL, "tile heights out of range (rb 0 or 2..64, rbp 0 or 8..256)");
-  if (line_form < 0 || line_form > 3)
-    return fail(HITX_EINVAL, "line form must be 0 (auto), 1 (block), 2 (wave) or 3 (wave, slab y-sweep)");
+  if (line_form < 0 || line_form > 4)
+    return fail(HITX_EINVAL,
+                "line form must be 0 (auto), 1 (block), 2 (wave), 3 (wave, slab y-sweep) or 4 (wave, x-sweep fused "
+                "into the update)");
   if (sensor_form < 0 || sensor_form > 3)
     return fail(HITX_EINVAL, "sensor form must be 0 (auto), 1 (fused), 2 (split, column walk) or 3 (split, tiles)");
   b->line_form = line_form;
@@ -1920,6 +1960,17 @@ int hitx_batch_timing_read(hitx_batch* b, double* total_ms, int32_t* launches) {
   *total_ms = t;
   *launches = (int32_t)(b->ev_used / 2);
   b->ev_used = 0;
+  return HITX_OK;
+}
+
+int hitx_batch_forms(const hitx_batch* b, int32_t* forms) {
+  if (!b || !forms) return fail(HITX_EINVAL, "null argument");
+  forms[0] = b->xfused ? 1 : 0;
+  forms[1] = b->wave_x ? 1 : 0;
+  forms[2] = b->wave_y ? 1 : 0;
+  forms[3] = b->slab_y ? 1 : 0;
+  forms[4] = b->split_sensor ? 1 : 0;
+  forms[5] = b->stage_walk ? 1 : 0;
   return HITX_OK;
 }
 

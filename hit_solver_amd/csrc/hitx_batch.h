@@ -51,7 +51,7 @@ struct Inst {
 // offsets kinds (rows of the offsets table, each ninst+1 ints)
 enum {
   OFF_RED = 0, OFF_SENS, OFF_LX, OFF_LY, OFF_T16, OFF_PROJ, OFF_RNG, OFF_SR, OFF_DROWS, OFF_FROWS, OFF_LXW, OFF_LYW,
-  OFF_RNGS, OFF_DB, OFF_FS, OFF_KINDS
+  OFF_RNGS, OFF_DB, OFF_FS, OFF_XU, OFF_KINDS
 };
 
 // rows per wavefront of the batch forcing synthesis (column walk)
@@ -292,6 +292,40 @@ __global__ void __launch_bounds__(kUBlock) b_update(const Inst<T>* __restrict__ 
   const GPtr<T> out = (st == 2 && I.ctl[C_HVNOW] == 0.0 && I.P.guard == 0) ? I.Q : stage_out(I, st);
   update_body<T, PER>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy, I.scal, I.drows, (T)I.ctl[C_DT],
                  out, bx, by, (T)I.ctl[C_SCALE], (T)I.ctl[C_ALPHA]);
+}
+
+// The x-sweep and the stage update of a panel of kXUWaves rows in one block
+// (periodic grids, lines in the wave form): each wave sweeps one row into
+// dfx, then the block updates the panel in kXTX x kXUWaves tiles, reading
+// the x-derivatives its own waves just wrote (L2 hits, not HBM) and the
+// panel's state again (its rows were just read by the sweep) instead of in a
+// separate update launch.  The same arithmetic as b_line_wave (axis 0)
+// followed by b_update, so the results are bit-identical; the line slices
+// and the update tiles share the dynamic LDS.
+constexpr int kXUWaves = 8, kXTX = kUBlock / kXUWaves;
+template <class T>
+__global__ void __launch_bounds__(64 * kXUWaves) HITX_LINE_WAVE_ATTR(T, true)
+    b_xupdate(const Inst<T>* __restrict__ tab, const int* __restrict__ offs, int ninst, int st, int slice_bytes) {
+  extern __shared__ __align__(16) unsigned char wsm[];
+  HITX_BATCH_TILE(OFF_XU)
+  const int nx = I.P.nx, ny = I.P.ny;
+  const int y0 = lb * kXUWaves;
+  const int w = threadIdx.x >> 6;
+  const int line = y0 + w;
+  if (line < ny)
+    line_wave_any<T, true>(I.P, I.Kx, 0, 0, stage_in(I, st), I.mask, I.scal, I.dfx, line, ny,
+                           wsm + (size_t)w * slice_bytes);
+  __syncthreads();  // the panel's x-derivatives are written; the slices are free
+  // the last stage writes Q in place when no hyperviscosity pass follows
+  // (each node then reads Q only at itself; these grids have no guard)
+  const GPtr<T> out = (st == 2 && I.ctl[C_HVNOW] == 0.0) ? I.Q : stage_out(I, st);
+  UpdLds<T, kXTX, kXUWaves>& L = *reinterpret_cast<UpdLds<T, kXTX, kXUWaves>*>(wsm);
+  const T dt = (T)I.ctl[C_DT], fsc = (T)I.ctl[C_SCALE], fal = (T)I.ctl[C_ALPHA];
+  for (int x0 = 0; x0 < nx; x0 += kXTX) {
+    if (x0 > 0) __syncthreads();  // the previous tile's LDS reads are done
+    update_tile<T, true, kXTX, kXUWaves>(I.P, MODE_STAGE, st, I.Q, stage_in(I, st), I.dfx, I.dfyT, I.fx, I.fy,
+                                         I.scal, I.drows, dt, out, x0, y0, L, fsc, fal);
+  }
 }
 
 // After the stages: hyperviscosity S -> Q with the fresh mask, or the S -> Q

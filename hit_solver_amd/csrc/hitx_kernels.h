@@ -1779,33 +1779,36 @@ __global__ void __launch_bounds__(kLineBlock, 4)
 // Update tiles are kUTX x kUTY nodes, one thread each: a row of the tile is
 // one 128-byte line of an FP32 plane (two of an FP64 plane).
 constexpr int kUTX = 32, kUTY = 16, kUBlock = kUTX * kUTY;
-constexpr int kVW = kUTX + 4, kVH = kUTY + 4;  // primitives tile with its 2-node halo
-constexpr int kVS = kVW + 1;
-constexpr int kFW = kUTX + 2, kFH = kUTY + 2;  // flux tile with its 1-node halo
-constexpr int kFS = kFW + 1;
-template <class T>
-struct ViscLds {
-  T U[kVH * kVS], V[kVH * kVS], Tt[kVH * kVS];                    // primitives (2-node halo)
-  T xx[kFH * kFS], yy[kFH * kFS], xy[kFH * kFS], ex[kFH * kFS], ey[kFH * kFS];  // fluxes (1-node halo)
+// LDS of an update tile of TX x TY nodes (TX * TY = kUBlock threads): the
+// primitives with their 2-node halo, the viscous fluxes with their 1-node
+// halo, the tile's y-derivative staged transposed.
+template <class T, int TX, int TY>
+struct UpdLds {
+  static constexpr int VW = TX + 4, VH = TY + 4, VS = VW + 1;  // primitives tile
+  static constexpr int FW = TX + 2, FH = TY + 2, FS = FW + 1;  // flux tile
+  T U[VH * VS], V[VH * VS], Tt[VH * VS];
+  T xx[FH * FS], yy[FH * FS], xy[FH * FS], ex[FH * FS], ey[FH * FS];
+  T D[4][TX][TY + 1];
 };
-// The fluxes at the kFW x kFH halo-ring nodes (the primitives in LDS); the
+// The fluxes at the FW x FH halo-ring nodes (the primitives in LDS); the
 // caller synchronises before and after.
-template <class T>
-__device__ __forceinline__ void viscous_fluxes(const Params<T>& P, ViscLds<T>& L) {
-  for (int i = threadIdx.x; i < kFW * kFH; i += kUBlock) {
-    const int fr = i / kFW, fc = i - fr * kFW;
-    const int b = (fr + 1) * kVS + (fc + 1);  // the node in the primitives tile
+template <class T, int TX, int TY>
+__device__ __forceinline__ void viscous_fluxes(const Params<T>& P, UpdLds<T, TX, TY>& L) {
+  using Ls = UpdLds<T, TX, TY>;
+  for (int i = threadIdx.x; i < Ls::FW * Ls::FH; i += kUBlock) {
+    const int fr = i / Ls::FW, fc = i - fr * Ls::FW;
+    const int b = (fr + 1) * Ls::VS + (fc + 1);  // the node in the primitives tile
     const T ux = dv(L.U[b + 1] - L.U[b - 1], P.two_dx);
-    const T uy = dv(L.U[b + kVS] - L.U[b - kVS], P.two_dy);
+    const T uy = dv(L.U[b + Ls::VS] - L.U[b - Ls::VS], P.two_dy);
     const T vx = dv(L.V[b + 1] - L.V[b - 1], P.two_dx);
-    const T vy = dv(L.V[b + kVS] - L.V[b - kVS], P.two_dy);
+    const T vy = dv(L.V[b + Ls::VS] - L.V[b - Ls::VS], P.two_dy);
     const T txx = madd(-P.tt_mu, ux + vy, P.two_mu * ux);
     const T tyy = madd(-P.tt_mu, ux + vy, P.two_mu * vy);
     const T txy = P.mu * (uy + vx);
     const T tx = dv(L.Tt[b + 1] - L.Tt[b - 1], P.two_dx);
-    const T ty = dv(L.Tt[b + kVS] - L.Tt[b - kVS], P.two_dy);
+    const T ty = dv(L.Tt[b + Ls::VS] - L.Tt[b - Ls::VS], P.two_dy);
     const T u = L.U[b], v = L.V[b];
-    const int f = fr * kFS + fc;
+    const int f = fr * Ls::FS + fc;
     L.xx[f] = txx;
     L.yy[f] = tyy;
     L.xy[f] = txy;
@@ -1814,16 +1817,18 @@ __device__ __forceinline__ void viscous_fluxes(const Params<T>& P, ViscLds<T>& L
   }
 }
 // Viscous terms at the tile node (ty, tx) from the fluxes in LDS.
-template <class T>
-__device__ __forceinline__ void viscous_div(const Params<T>& P, const ViscLds<T>& L, int ty, int tx, T out[4]) {
-  const int c = (ty + 1) * kFS + (tx + 1);
+template <class T, int TX, int TY>
+__device__ __forceinline__ void viscous_div(const Params<T>& P, const UpdLds<T, TX, TY>& L, int ty, int tx,
+                                            T out[4]) {
+  constexpr int FS = UpdLds<T, TX, TY>::FS;
+  const int c = (ty + 1) * FS + (tx + 1);
   out[0] = T(0);
-  out[1] = dv(L.xx[c + 1] - L.xx[c - 1], P.two_dx) + dv(L.xy[c + kFS] - L.xy[c - kFS], P.two_dy);
-  out[2] = dv(L.xy[c + 1] - L.xy[c - 1], P.two_dx) + dv(L.yy[c + kFS] - L.yy[c - kFS], P.two_dy);
-  out[3] = dv(L.ex[c + 1] - L.ex[c - 1], P.two_dx) + dv(L.ey[c + kFS] - L.ey[c - kFS], P.two_dy);
+  out[1] = dv(L.xx[c + 1] - L.xx[c - 1], P.two_dx) + dv(L.xy[c + FS] - L.xy[c - FS], P.two_dy);
+  out[2] = dv(L.xy[c + 1] - L.xy[c - 1], P.two_dx) + dv(L.yy[c + FS] - L.yy[c - FS], P.two_dy);
+  out[3] = dv(L.ex[c + 1] - L.ex[c - 1], P.two_dx) + dv(L.ey[c + FS] - L.ey[c - FS], P.two_dy);
 }
 
-// Primitives u, v, p/(rho R) of the kVW x kVH tile around a kUTX x kUTY
+// Primitives u, v, p/(rho R) of the (TX+4) x (TY+4) tile around a TX x TY
 // block (periodic wrap) into LDS.
 // Split in two so that a caller can issue the loads, do other work, then
 // form the primitives: each thread holds at most two of the tile nodes.
@@ -1831,33 +1836,37 @@ template <class T>
 struct VTileLoad {
   T a[2][4];
 };
-template <class T>
+template <class T, int TX, int TY>
 __device__ __forceinline__ VTileLoad<T> viscous_tile_load(const Params<T>& P, const T* __restrict__ q, int x0,
                                                            int y0) {
+  constexpr int VW = TX + 4, VH = TY + 4;
+  static_assert(VW * VH <= 2 * kUBlock, "two tile nodes per thread");
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   VTileLoad<T> v;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int i = threadIdx.x + s * kUBlock;
-    if (i < kVW * kVH) {
-      const int ly = i / kVW, lx = i - ly * kVW;
+    if (i < VW * VH) {
+      const int ly = i / VW, lx = i - ly * VW;
       const int gy = wrap(y0 - 2 + ly, ny), gx = wrap(x0 - 2 + lx, nx);
       load4(q, N, gy * nx + gx, v.a[s][0], v.a[s][1], v.a[s][2], v.a[s][3]);
     }
   }
   return v;
 }
-template <class T>
-__device__ __forceinline__ void viscous_tile_store(const Params<T>& P, const VTileLoad<T>& v, T* U, T* V, T* Tt) {
+template <class T, int TX, int TY>
+__device__ __forceinline__ void viscous_tile_store(const Params<T>& P, const VTileLoad<T>& v,
+                                                   UpdLds<T, TX, TY>& L) {
+  using Ls = UpdLds<T, TX, TY>;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int i = threadIdx.x + s * kUBlock;
-    if (i < kVW * kVH) {
-      const int ly = i / kVW, lx = i - ly * kVW;
+    if (i < Ls::VW * Ls::VH) {
+      const int ly = i / Ls::VW, lx = i - ly * Ls::VW;
       const Prim<T> pr = prim(P, v.a[s][0], v.a[s][1], v.a[s][2], v.a[s][3]);
-      U[ly * kVS + lx] = pr.u;
-      V[ly * kVS + lx] = pr.v;
-      Tt[ly * kVS + lx] = dv(pr.p, pr.rho * P.rgas);
+      L.U[ly * Ls::VS + lx] = pr.u;
+      L.V[ly * Ls::VS + lx] = pr.v;
+      L.Tt[ly * Ls::VS + lx] = dv(pr.p, pr.rho * P.rgas);
     }
   }
 }
@@ -1869,20 +1878,20 @@ enum { MODE_RHS = 0, MODE_STAGE = 1, MODE_VISC = 2 };
 // the Shu-Osher combination (OOP/time_operator.py:24-26) and clean =
 // enforce (+ outflow guard: every node takes the value of its clamped
 // interior source, OOP/problems/riemann_config3.py:419-429).
-// One kUTX x kUTY tile per block: the y-derivative (stored transposed,
-// [c][x][y]) is staged through LDS so both derivative reads are coalesced.
-template <class T, bool PER_ONLY = false>  // PER_ONLY: a periodic grid (no outflow guard), known at compile time
-__device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
+// One TX x TY tile per block (kUBlock threads, thread t at (t % TX, t / TX)):
+// the y-derivative (stored transposed, [c][x][y]) is staged through LDS so
+// both derivative reads are coalesced.  The node arithmetic does not depend
+// on the tile shape.  Every thread passes the tile's barriers.
+template <class T, bool PER_ONLY, int TX, int TY>  // PER_ONLY: a periodic grid (no outflow guard), known at compile time
+__device__ __forceinline__ void update_tile(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
                                             const T* __restrict__ qs, const T* __restrict__ dfx,
                                             const T* __restrict__ dfyT, const T* __restrict__ fx,
                                             const T* __restrict__ fy, const double* __restrict__ scal,
-                                            const double* __restrict__ drows, T dt, T* __restrict__ out, int bx,
-                                            int by, T fsc = T(1), T fal = T(1)) {
-  __shared__ ViscLds<T> sVis;
-  __shared__ T sD[4][kUTX][kUTY + 1];
+                                            const double* __restrict__ drows, T dt, T* __restrict__ out, int x0,
+                                            int y0, UpdLds<T, TX, TY>& L, T fsc, T fal) {
+  static_assert(TX * TY == kUBlock, "one thread per tile node");
   const int nx = P.nx, ny = P.ny, N = nx * ny;
-  const int x0 = bx * kUTX, y0 = by * kUTY;
-  const int tx = threadIdx.x % kUTX, ty = threadIdx.x / kUTX;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
   const bool in = x < nx && y < ny;
   int cx = x, cy = y;
@@ -1926,11 +1935,11 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
     }
   }
   VTileLoad<T> vt;
-  if (visc) vt = viscous_tile_load(P, qs, x0, y0);
+  if (visc) vt = viscous_tile_load<T, TX, TY>(P, qs, x0, y0);
   T dsh[4];
   if (mode != MODE_VISC) {
-    // dfyT tile: thread t reads x-row x0 + t / kUTY, y = y0 + t % kUTY (unit stride in y)
-    const int rx = x0 + (int)threadIdx.x / kUTY, ry = y0 + (int)threadIdx.x % kUTY;
+    // dfyT tile: thread t reads x-row x0 + t / TY, y = y0 + t % TY (unit stride in y)
+    const int rx = x0 + (int)threadIdx.x / TY, ry = y0 + (int)threadIdx.x % TY;
 #pragma unroll
     for (int c = 0; c < 4; ++c) dsh[c] = (rx < nx && ry < ny) ? at32(dfyT + (size_t)c * N, rx * ny + ry) : T(0);
   }
@@ -1959,20 +1968,20 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
       dsrc[3] = madd(st.v, d2, st.u * d1);
     }
   }
-  if (visc) viscous_tile_store(P, vt, sVis.U, sVis.V, sVis.Tt);
+  if (visc) viscous_tile_store<T, TX, TY>(P, vt, L);
   if (mode != MODE_VISC) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sD[c][threadIdx.x / kUTY][threadIdx.x % kUTY] = dsh[c];
+    for (int c = 0; c < 4; ++c) L.D[c][threadIdx.x / TY][threadIdx.x % TY] = dsh[c];
   }
   __syncthreads();
   if (visc) {
-    viscous_fluxes(P, sVis);
+    viscous_fluxes<T, TX, TY>(P, L);
     __syncthreads();
   }
   if (!in) return;
   if (mode == MODE_VISC) {  // standalone viscous_rhs of qs
     T vv[4];
-    viscous_div(P, sVis, ty, tx, vv);
+    viscous_div<T, TX, TY>(P, L, ty, tx, vv);
 #pragma unroll
     for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = vv[c];
     return;
@@ -1981,7 +1990,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   T Lv[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const T gy = own ? sD[c][tx][ty] : ld_global(dfyT + c * N + (size_t)cx * ny + cy);
+    const T gy = own ? L.D[c][tx][ty] : ld_global(dfyT + c * N + (size_t)cx * ny + cy);
     Lv[c] = (-gx[c]) - gy;
   }
   if (mode == MODE_RHS) {
@@ -1991,7 +2000,7 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   }
   if (P.visc_on) {
     T vv[4];
-    viscous_div(P, sVis, ty, tx, vv);
+    viscous_div<T, TX, TY>(P, L, ty, tx, vv);
 #pragma unroll
     for (int c = 0; c < 4; ++c) Lv[c] = Lv[c] + vv[c];
   }
@@ -2019,6 +2028,19 @@ __device__ __forceinline__ void update_body(const Params<T>& P, int mode, int st
   enforce(P, o[0], o[1], o[2], o[3]);
 #pragma unroll
   for (int c = 0; c < 4; ++c) at32(out + (size_t)c * N, oidx) = o[c];
+}
+
+// The kUTX x kUTY tile (bx, by) with its LDS in static shared memory.
+template <class T, bool PER_ONLY = false>
+__device__ __forceinline__ void update_body(const Params<T>& P, int mode, int stage, const T* __restrict__ q0,
+                                            const T* __restrict__ qs, const T* __restrict__ dfx,
+                                            const T* __restrict__ dfyT, const T* __restrict__ fx,
+                                            const T* __restrict__ fy, const double* __restrict__ scal,
+                                            const double* __restrict__ drows, T dt, T* __restrict__ out, int bx,
+                                            int by, T fsc = T(1), T fal = T(1)) {
+  __shared__ UpdLds<T, kUTX, kUTY> L;
+  update_tile<T, PER_ONLY, kUTX, kUTY>(P, mode, stage, q0, qs, dfx, dfyT, fx, fy, scal, drows, dt, out, bx * kUTX,
+                                       by * kUTY, L, fsc, fal);
 }
 
 template <class T>

@@ -390,12 +390,20 @@ int hitx_batch_bind(hitx_batch* batch, void* ws, size_t ws_bytes);
  * split sensor (2..64), rbp rows per wave of the forcing projection (8..256)
  * (0 for either: each instance's own default, set by its grid alone),
  * the line-sweep form (0 auto, 1 block, 2 one wavefront per line where the
- * line allows it) and the sensor form (0 auto, 1 fused tile, 2 split: the
+ * line allows it, 3 as 2 with the y-lines read through an LDS slab, 4 as 2
+ * with each row panel's x-sweep and stage update in one launch where the
+ * batch allows it: FP32, periodic grids, both line lengths in the wave form)
+ * and the sensor form (0 auto, 1 fused tile, 2 split: the
  * undilated mask by column walk, then the dilation; 3 split with the
  * undilated mask on tiles with a one-node halo).  Changes the summation
  * order of the reductions and the rounding of the line solve only.  Blocking
  * when bound (re-uploads the instance table); no work may be in flight. */
 int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_form, int32_t sensor_form);
+/* The launch forms in effect (no reference counterpart; for tests and
+ * measurements): forms[0] x-sweep fused into the stage update, [1] x-lines in
+ * the wave form, [2] y-lines in the wave form, [3] y-lines through the slab,
+ * [4] split sensor, [5] drag stage sums by column walk (each 0 or 1). */
+int hitx_batch_forms(const hitx_batch* batch, int32_t* forms);
 /* Form of the drag stage reductions (no reference counterpart): 0 auto (on
  * the sensor tiles while the batch has at most 4096 of them, by column walk
  * above), 1 always on tiles, 2 always by column walk.  The two forms sum the

@@ -308,3 +308,37 @@ def test_mixed_hit_riemann_batch_bitexact(dtype, stage_form, line_form):
         np.testing.assert_array_equal(host(mixed.q[slot]), host(alone[g].q[i]), err_msg=f"slot {slot} = {g},{i}")
         np.testing.assert_array_equal(ctl[slot], alone[g].control()[i])
         assert ctl[slot, 4] == steps
+
+
+def test_fused_x_sweep_update_bitexact():
+    """Line form 4 runs each row panel's x-sweep and stage update in one
+    launch (FP32, periodic grids, wave-form lines): the same arithmetic as the
+    separate x-sweep and update of line form 2, so the batch is
+    bit-identical, with hyperviscosity steps (the last stage writing S) and
+    the other steps (writing Q in place) both covered."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+
+    kw = dict(cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0, viscosity=7.5e-4,
+              mach_control=True, large_scale_drag=0.10, cooling_time=5.0, tfinal=1.0, max_steps=4,
+              hyperviscosity_interval=2, dtype="fp32")
+    cfgs = [HIT2DConfig(nx=64, ny=128, target_mach=0.25, forcing_seed=1, **kw),
+            HIT2DConfig(nx=192, ny=192, target_mach=0.1, forcing_seed=2, **kw),
+            HIT2DConfig(nx=448, ny=64, target_mach=0.4, forcing_seed=3, **kw),
+            HIT2DConfig(nx=512, ny=256, target_mach=0.25, forcing_seed=4, **kw)]
+    print(f"[batch] fused x-sweep + update vs wave form, {len(cfgs)} instances fp32", flush=True)
+    a, b = BatchRunner(cfgs, max_steps=4), BatchRunner(cfgs, max_steps=4)
+    a.tune(4, 16, 2, 3)
+    b.tune(4, 16, 4, 3)
+    assert not a.forms()["xfused"] and b.forms()["xfused"]
+    a.run()
+    b.run()
+    for i in range(len(cfgs)):
+        np.testing.assert_array_equal(host(b.q[i]), host(a.q[i]), err_msg=f"instance {i}")
+    np.testing.assert_array_equal(b.control(), a.control())
+    # a batch with a grid that is not periodic keeps the separate launches
+    from hit_solver_amd.riemann import RiemannConfig3
+
+    c = BatchRunner([cfgs[0], RiemannConfig3(nx=64, ny=64, shear_threshold=None, progress_every=10**9, dtype="fp32")])
+    c.tune(4, 16, 4, 3)
+    assert not c.forms()["xfused"]
