@@ -338,6 +338,12 @@ class BatchRunner:
         keys = ("xfused", "wave_x", "wave_y", "slab_y", "split_sensor", "stage_walk")
         return {k: bool(v) for k, v in zip(keys, out)}
 
+    def serial_draws(self, on: bool = True):
+        """Make every noise draw take the serial path (a check of that path:
+        the same values, projected from the noise buffer)."""
+        check(lib().hitx_batch_serial_draws(self.handle, 1 if on else 0))
+        self._graph = None
+
     def set_stage_form(self, form: int):
         """Form of the drag stage reductions: 0 auto (tiles while the batch has
         at most 4096 sensor tiles, column walk above), 1 tiles, 2 column walk.

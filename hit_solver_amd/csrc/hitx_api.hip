@@ -1339,6 +1339,7 @@ struct hitx_batch {
   int line_form = 0;   // 0 auto (by the batch's total line count), 1 block, 2 wave, 3 wave with the slab y-sweep,
                        // 4 wave with the x-sweep fused into the stage update (where the batch allows it)
   bool xfused = false;  // stages run the y-sweep, then b_xupdate (x-sweep + update per row panel)
+  int serial_draws = 0;  // every noise draw takes the serial path (a check of that path)
   size_t xu_lds = 0;    // dynamic LDS of b_xupdate
   bool wave_x = false, wave_y = false, slab_y = false;
   size_t slab_lds = 0;  // dynamic LDS of the slab y-sweep (slab or slices, whichever is larger)
@@ -1490,6 +1491,7 @@ void fill_inst(const hitx_batch* b, int i, uint8_t* base, Inst<T>& I) {
   I.nblk_stage = batch_red_blocks(pl);
   I.nblk_sr = pl->nblk_sr;
   I.nblk_proj = pl->nblk_proj;
+  I.nsw = (rng_chunks(pl->N) + 63) / 64;
   I.ntile = ((pl->d.nx + kSTX - 1) / kSTX) * ((pl->d.ny + kSTY - 1) / kSTY);
   I.use_qT = !(b->wave_y && b->slab_y && line_wave_ok(pl->d.ny));
 }
@@ -1673,7 +1675,7 @@ void batch_draw(hitx_batch* b, hipStream_t f, int pre) {
   const int* offs = boffs(b);
   const int n = b->n;
   b_rng_attempts<T><<<b->total[OFF_RNG], kRngAttemptThreads, 0, f>>>(tab, offs, n, pre);
-  b_rng_scan<T><<<n, kRngScan, 0, f>>>(tab, pre);
+  b_rng_scan<T><<<n, kRngScan, 0, f>>>(tab, pre, b->serial_draws);
   b_rng_scatter<T><<<b->total[OFF_RNGS], kRngScatterThreads, 0, f>>>(tab, offs, n, pre);
   b_rng_finish<T><<<n, 64, 0, f>>>(tab, pre);
   b_fproject<T><<<b->total[OFF_PROJ], kBlock, b->proj_lds, f>>>(tab, offs, n, pre);
@@ -1843,8 +1845,10 @@ int hitx_batch_create(const hitx_desc* descs, const double* kpar, int32_t n, hit
     u.phi = take(2 * nf * sizeof(double));
     u.fhat = take(4 * nf * sizeof(double));
     u.noise = take(pl->d.n_force_modes > 0 ? N * sizeof(double) : 8);
-    // forcing partials: the projection's (2 per mode and block), then the synthesis's (5 per block)
-    u.fpart = take(std::max((size_t)std::max(pl->nblk_proj, pj_max) * 2 * nf, (size_t)walk_blocks(pl->d.nx, pl->d.ny, kFsRows, 0) * 5) *
+    // forcing partials: the projection's (2 per mode and projection block or
+    // scatter wavefront), then the synthesis's (5 per block)
+    const size_t prow = (size_t)std::max({pl->nblk_proj, pj_max, (rng_chunks(pl->N) + 63) / 64});
+    u.fpart = take(std::max(prow * 2 * nf, (size_t)walk_blocks(pl->d.nx, pl->d.ny, kFsRows, 0) * 5) *
                    sizeof(double));
     u.kxy = take(2 * nf * sizeof(double));
     u.rng = take(4 * sizeof(uint64_t));
@@ -1960,6 +1964,12 @@ int hitx_batch_timing_read(hitx_batch* b, double* total_ms, int32_t* launches) {
   *total_ms = t;
   *launches = (int32_t)(b->ev_used / 2);
   b->ev_used = 0;
+  return HITX_OK;
+}
+
+int hitx_batch_serial_draws(hitx_batch* b, int32_t on) {
+  if (!b) return fail(HITX_EINVAL, "null batch");
+  b->serial_draws = on != 0 ? 1 : 0;
   return HITX_OK;
 }
 

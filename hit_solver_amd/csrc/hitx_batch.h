@@ -23,6 +23,7 @@ enum {
   C_SFF, C_SP, C_FXX, C_FYY, C_FXY, C_PIN, C_PBEFORE, C_ATGT, C_MACH, C_PDES, C_NSTEPS,
   C_NPRE,   // 1: the noise and projection partials of the next step are drawn (prefetched)
   C_NPREW,  // PCG64 words that prefetched draw consumed (a copy of the instance rewinds by them)
+  C_NFALL,  // 1: the last draw took the serial path (noise in the noise buffer, projected by b_fproject)
   C_COUNT = 32
 };
 // per-instance run parameters (double, constant); NaN encodes "None"
@@ -44,6 +45,7 @@ struct Inst {
   RngJob R;            // forcing noise request: nx*ny normals of the rng stream into noise
   GPtr<double> drows, frows;  // per-row synthesis tables: drag low-pass, forcing
   int nblk_stage, nblk_sr, nblk_proj;
+  int nsw;     // wavefronts of the noise scatter (one projection partial row each)
   int ntile;   // 32x16 tiles of the grid
   int use_qT;  // the y-sweep reads the transposed state (block form or wave form without the slab)
 };
@@ -448,13 +450,15 @@ __global__ void b_ctl_end(const Inst<T>* __restrict__ tab, int ninst, int* __res
 }
 
 // ---------------------------------------------------------------- forcing
-// (after b_rng_finish: with pre = 1 the mark is set and this projection is
-// part of the prefetch, so only pre = 0 reads it)
+// The projection of a draw that took the serial path (its noise is in the
+// noise buffer); the parallel draw projects in its scatter.  (After
+// b_rng_finish: with pre = 1 the mark is set and this projection is part of
+// the prefetch, so only pre = 0 reads it.)
 template <class T>
 __global__ void __launch_bounds__(kBlock) b_fproject(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                      int ninst, int pre) {
   HITX_BATCH_INST(OFF_PROJ)
-  if (rng_skip(I.ctl, I.kpar, pre)) return;
+  if (rng_skip(I.ctl, I.kpar, pre) || I.ctl[C_NFALL] == 0.0) return;
   fproject_body<T>(I.P, I.noise, I.fpart, lb);
 }
 
@@ -480,10 +484,108 @@ __global__ void __launch_bounds__(kRngAttemptThreads) b_rng_attempts(const Inst<
 }
 
 template <class T>
-__global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict__ tab, int pre) {
+__global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict__ tab, int pre, int force_serial) {
   const Inst<T>& I = tab[blockIdx.x];
   if (I.ctl[C_ACTIVE] == 0.0 || rng_skip(I.ctl, I.kpar, pre)) return;
-  rng_scan_body(I.R, 0);
+  rng_scan_body(I.R, force_serial);
+}
+
+// Sum of v over the wavefront, wave-uniform: inclusive sums within the rows
+// of 16 lanes by DPP row shifts (lane 15 of a row then holds the row's sum),
+// then the four row sums in row order (a fixed order: the same bits run to
+// run).
+__device__ __forceinline__ double wave_sum_rows(double v) {
+  v += dpp<0x111>(v);
+  v += dpp<0x112>(v);
+  v += dpp<0x114>(v);
+  v += dpp<0x118>(v);
+  return ((rdlane(v, 15) + rdlane(v, 31)) + rdlane(v, 47)) + rdlane(v, 63);
+}
+
+// The noise draw's scatter with the projection formed in place of the
+// noise field (OOP/forcing.py:77-79: fft2 of the normals at the shell
+// modes).  Each wavefront's values are a contiguous index range [lo, hi)
+// staged in its LDS slice (rng_scatter_body); row by row the wave sums
+// v exp(-i k x) over its part of the row for k = 0..fkx (twiddles of k from
+// the k = 1 entry by the angle-addition recurrence; lane partial sums added
+// over the wave in a fixed order), then lane m adds exp(-i ky y) times the
+// row sum at its mode's kx (a negative kx takes the conjugate), and writes
+// the wave's partial row of fft2 at the modes.  The noise never goes to
+// memory; a draw that falls back to the serial form writes the noise and
+// b_fproject projects it.
+template <class T>
+__device__ __forceinline__ void rng_scatter_project_body(const Params<T>& P, const RngJob& J, int c, double* lds,
+                                                          double* __restrict__ part) {
+  long long lo, hi;
+  if (!rng_scatter_stage(J, c, lds, lo, hi)) return;
+  const int lane = threadIdx.x & 63;
+  const int nx = P.nx, nf = P.nforce, fk = P.fkx;
+  int mkx = 0, mky = 0;
+  if (lane < nf) {
+    mkx = P.force_modes[2 * lane];
+    mky = P.force_modes[2 * lane + 1];
+  }
+  const int mka = mkx < 0 ? -mkx : mkx, mkya = mky < 0 ? -mky : mky;
+  double mre = 0.0, mim = 0.0;
+  for (long long i0 = lo; i0 < hi;) {
+    const int y = (int)(i0 / nx);
+    const long long row0 = (long long)y * nx, rend = min(row0 + nx, hi);
+    double rc[kFK + 1], rs[kFK + 1];
+#pragma unroll
+    for (int k = 0; k <= kFK; ++k) {
+      rc[k] = 0.0;
+      rs[k] = 0.0;
+    }
+    for (long long i = i0 + lane; i < rend; i += 64) {
+      const double v = lds[i - lo];
+      const int x = (int)(i - row0);
+      double c1 = 1.0, s1 = 0.0;
+      if (fk >= 1) {  // (the table holds |k| <= kmx, and kmx >= fkx)
+        const double* t1 = P.twx + ((size_t)(1 + P.kmx) * nx + x) * 2;
+        c1 = t1[0];
+        s1 = t1[1];
+      }
+      rc[0] += v;
+      double ck = c1, sk = s1;
+#pragma unroll
+      for (int k = 1; k <= kFK; ++k) {
+        if (k <= fk) {
+          rc[k] = fma(v, ck, rc[k]);
+          rs[k] = fma(v, sk, rs[k]);
+          const double cn = ck * c1 - sk * s1, sn = sk * c1 + ck * s1;
+          ck = cn;
+          sk = sn;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k <= kFK; ++k) {
+      if (k <= fk) {
+        rc[k] = wave_sum_rows(rc[k]);
+        rs[k] = wave_sum_rows(rs[k]);
+      }
+    }
+    if (lane < nf) {
+      double Rc = 0.0, Rs = 0.0;
+#pragma unroll
+      for (int k = 0; k <= kFK; ++k)
+        if (k == mka) {
+          Rc = rc[k];
+          Rs = rs[k];
+        }
+      if (mkx < 0) Rs = -Rs;  // sin(kx x) is odd in kx
+      const double* ty = P.twy + ((size_t)(mkya + P.kmy) * P.ny + y) * 2;
+      const double cy = ty[0], sy = mky < 0 ? -ty[1] : ty[1];
+      // (cy - i sy)(Rc - i Rs)
+      mre += cy * Rc - sy * Rs;
+      mim -= cy * Rs + sy * Rc;
+    }
+    i0 = rend;
+  }
+  if (lane < nf) {
+    part[2 * lane] = mre;
+    part[2 * lane + 1] = mim;
+  }
 }
 
 template <class T>
@@ -492,14 +594,18 @@ __global__ void __launch_bounds__(kRngScatterThreads) b_rng_scatter(const Inst<T
   __shared__ double lds[kRngScatterThreads / 64][64 * kRngChunk];
   HITX_BATCH_INST(OFF_RNGS)
   if (rng_skip(I.ctl, I.kpar, pre)) return;
-  rng_scatter_body(I.R, lb * kRngScatterThreads + threadIdx.x, lds[threadIdx.x >> 6]);
+  const int c = lb * kRngScatterThreads + threadIdx.x;
+  rng_scatter_project_body<T>(I.P, I.R, c, lds[threadIdx.x >> 6],
+                              I.fpart + (size_t)((c >> 6)) * 2 * I.P.nforce);
 }
 
 template <class T>
 __global__ void b_rng_finish(const Inst<T>* __restrict__ tab, int pre) {
   const Inst<T>& I = tab[blockIdx.x];
   if (threadIdx.x != 0 || I.ctl[C_ACTIVE] == 0.0 || rng_skip(I.ctl, I.kpar, pre)) return;
+  const bool serial = I.R.stat[0] != 0;
   const unsigned long long words = rng_finish_body(I.R);
+  I.ctl[C_NFALL] = serial ? 1.0 : 0.0;
   if (pre) {
     I.ctl[C_NPREW] = (double)words;
     I.ctl[C_NPRE] = 1.0;
@@ -514,9 +620,10 @@ __global__ void __launch_bounds__(kBlock) b_fou(const Inst<T>* __restrict__ tab)
   if (I.ctl[C_ACTIVE] == 0.0 || I.kpar[K_FORCE] == 0.0) return;
   const int nm = I.P.nforce;
   const double decay = I.ctl[C_DECAY], incr = I.ctl[C_INCR];
-  const int nb = I.nblk_proj;
+  // partial rows: the scatter's wavefronts, or the serial path's projection blocks
+  const int nb = I.ctl[C_NFALL] != 0.0 ? I.nblk_proj : I.nsw;
   for (int m = threadIdx.x; m < nm; m += blockDim.x) {
-    // the partials in block order, eight blocks' loads in flight at a time
+    // the partials in row order, eight rows' loads in flight at a time
     double re = 0.0, im = 0.0;
     const double* fp = I.fpart + (size_t)m * 2;
     const size_t stride = (size_t)nm * 2;

@@ -341,11 +341,14 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
 // slice (lds: kRngChunk*64 doubles) and written out coalesced.  The lane
 // that writes value n-1 records how many words the n values consumed.
 constexpr int kRngScatterThreads = 128;
-__device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double* lds) {
-  if (J.stat[0] != 0) return;
+// Stages the wave's values [lo, hi) in lds (value i at i - lo); false when
+// the wave has nothing to do (serial path, chunks past the request).
+__device__ __forceinline__ bool rng_scatter_stage(const RngJob& J, int c, double* lds, long long& lo_out,
+                                                  long long& hi_out) {
+  if (J.stat[0] != 0) return false;
   const int lane = threadIdx.x & 63;
   const int c0 = c - lane;  // chunk of lane 0
-  if (c0 >= J.nch) return;
+  if (c0 >= J.nch) return false;
   const long long lo = J.base[c0];
   long long k = lo;
   if (c < J.nch) {
@@ -385,6 +388,14 @@ __device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double*
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  lo_out = lo;
+  hi_out = hi;
+  return true;
+}
+__device__ __forceinline__ void rng_scatter_body(const RngJob& J, int c, double* lds) {
+  long long lo, hi;
+  if (!rng_scatter_stage(J, c, lds, lo, hi)) return;
+  const int lane = threadIdx.x & 63;
   for (long long i = lo + lane; i < hi; i += 64) J.out[i] = lds[i - lo];
 }
 

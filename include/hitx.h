@@ -323,14 +323,17 @@ int hitx_timing_read(hitx_plan* plan, double* total_ms, int32_t* launches);
  * Control scalars (HITX_BUF_CTL, doubles): 0 active, 1 time, 2 tfinal, 3 dt, 4 step,
  *   7 hyperviscosity applications, 11 target power, 12 alpha, 18-21 Fxx Fyy Fxy P_in,
  *   22 power before rescale, 23 alpha target, 24 Mach, 25 desired power, 26 n steps (fixed dt),
- *   27 the next step's noise is drawn, 28 PCG64 words that draw consumed.
+ *   27 the next step's noise is drawn, 28 PCG64 words that draw consumed,
+ *   29 that draw took the serial path.
  * The caller writes each instance's initial state (HITX_BUF_STATE), control
  * scalars and, for forced HIT, the PCG64 state of its noise stream
  * (HITX_BUF_RNG, 4 uint64 words as for hitx_normals); a step draws the
- * instance's normals of the next step into HITX_BUF_NOISE on the device while
- * it runs (slot 27 set), and a step that finds slot 27 clear draws its own
- * first.  To continue an instance elsewhere with slot 27 set, move its PCG64
- * state back by slot 28's words and clear slot 27. */
+ * instance's normals of the next step on the device while it runs (slot 27
+ * set), and a step that finds slot 27 clear draws its own first.  The draw
+ * projects the normals onto the forcing modes as it scatters them; only a
+ * draw that takes the serial path writes them to HITX_BUF_NOISE.  To continue
+ * an instance elsewhere with slot 27 set, move its PCG64 state back by slot
+ * 28's words and clear slot 27. */
 #define HITX_KPAR 24
 #define HITX_CTL 32
 enum { HITX_BUF_STATE = 0, HITX_BUF_NOISE, HITX_BUF_CTL, HITX_BUF_PHI, HITX_BUF_RNG, HITX_BUF_FX, HITX_BUF_FY };
@@ -404,6 +407,10 @@ int hitx_batch_tune(hitx_batch* batch, int32_t rb, int32_t rbp, int32_t line_for
  * the wave form, [2] y-lines in the wave form, [3] y-lines through the slab,
  * [4] split sensor, [5] drag stage sums by column walk (each 0 or 1). */
 int hitx_batch_forms(const hitx_batch* batch, int32_t* forms);
+/* Every noise draw of the batch's steps takes the serial path (no reference
+ * counterpart; a check of that path: the values are the same, the projection
+ * is formed from the noise buffer in another summation order). */
+int hitx_batch_serial_draws(hitx_batch* batch, int32_t on);
 /* Form of the drag stage reductions (no reference counterpart): 0 auto (on
  * the sensor tiles while the batch has at most 4096 of them, by column walk
  * above), 1 always on tiles, 2 always by column walk.  The two forms sum the
