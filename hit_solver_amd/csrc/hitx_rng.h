@@ -36,7 +36,7 @@ typedef unsigned __int128 u128;
 
 constexpr int kRngChunk = 32;    // words per chunk (one thread)
 constexpr int kRngEntries = 8;   // entry offsets tracked per chunk
-constexpr int kRngScan = 1024;   // threads of the scan block
+constexpr int kRngScan = 256;    // threads of the scan block (four wavefronts: a request holds few CU slots)
 constexpr uint64_t kMask52 = (1ull << 52) - 1;
 
 __device__ __forceinline__ u128 pcg_mult() {
