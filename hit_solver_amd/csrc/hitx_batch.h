@@ -490,25 +490,13 @@ __global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict
   rng_scan_body(I.R, force_serial);
 }
 
-// Sum of v over the wavefront, wave-uniform: inclusive sums within the rows
-// of 16 lanes by DPP row shifts (lane 15 of a row then holds the row's sum),
-// then the four row sums in row order (a fixed order: the same bits run to
-// run).
-__device__ __forceinline__ double wave_sum_rows(double v) {
-  v += dpp<0x111>(v);
-  v += dpp<0x112>(v);
-  v += dpp<0x114>(v);
-  v += dpp<0x118>(v);
-  return ((rdlane(v, 15) + rdlane(v, 31)) + rdlane(v, 47)) + rdlane(v, 63);
-}
-
 // The noise draw's scatter with the projection formed in place of the
 // noise field (OOP/forcing.py:77-79: fft2 of the normals at the shell
 // modes).  Each wavefront's values are a contiguous index range [lo, hi)
 // staged in its LDS slice (rng_scatter_body); row by row the wave sums
 // v exp(-i k x) over its part of the row for k = 0..fkx (twiddles of k from
 // the k = 1 entry by the angle-addition recurrence; lane partial sums added
-// over the wave in a fixed order), then lane m adds exp(-i ky y) times the
+// over the wave in a fixed order, wave_reduce), then lane m adds exp(-i ky y) times the
 // row sum at its mode's kx (a negative kx takes the conjugate), and writes
 // the wave's partial row of fft2 at the modes.  The noise never goes to
 // memory; a draw that falls back to the serial form writes the noise and
@@ -561,8 +549,8 @@ __device__ __forceinline__ void rng_scatter_project_body(const Params<T>& P, con
 #pragma unroll
     for (int k = 0; k <= kFK; ++k) {
       if (k <= fk) {
-        rc[k] = wave_sum_rows(rc[k]);
-        rs[k] = wave_sum_rows(rs[k]);
+        rc[k] = wave_reduce<false>(rc[k]);
+        rs[k] = wave_reduce<false>(rs[k]);
       }
     }
     if (lane < nf) {

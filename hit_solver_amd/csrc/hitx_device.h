@@ -286,20 +286,46 @@ __device__ __forceinline__ T weno7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
 }
 
 // ---------------------------------------------------------------- reductions
-// Block-wide reduction of NR doubles; slot k is a max when bit k of MAXMASK
-// is set, else a sum.  Thread 0..NR-1 write the block's partials.
+// DPP move of a double within rows of 16 lanes (GFX9 control CTRL, e.g.
+// row_shr:d = 0x110 + d); lanes without a source receive 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Sum (MAX false) or maximum (MAX true, of values >= 0: the lanes a row shift
+// leaves without a source read 0) over the 64 lanes of a wavefront, every
+// lane active; wave-uniform result in a fixed order (the same bits run to
+// run): inclusive scans within the rows of 16 lanes by DPP row shifts (one
+// VALU move each, no LDS round trip), then the four row totals in row order.
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double v) {
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  v = op(v, dpp_row_d<0x111>(v));
+  v = op(v, dpp_row_d<0x112>(v));
+  v = op(v, dpp_row_d<0x114>(v));
+  v = op(v, dpp_row_d<0x118>(v));
+  return op(op(op(readlane_d(v, 15), readlane_d(v, 31)), readlane_d(v, 47)), readlane_d(v, 63));
+}
+
+// Block-wide reduction of NR doubles (every thread of a kBlock block calls
+// it); slot k is a max (of values >= 0) when bit k of MAXMASK is set, else a
+// sum.  Thread 0..NR-1 write the block's partials.
 template <int NR, uint64_t MAXMASK>
 __device__ __forceinline__ void block_reduce_write(double (&v)[NR], double* out) {
   __shared__ double sh[kWaves][NR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
-    double a = v[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      double o = __shfl_xor(a, off, 64);
-      a = ((MAXMASK >> k) & 1) ? fmax(a, o) : a + o;
-    }
+    const double a = ((MAXMASK >> k) & 1) ? wave_reduce<true>(v[k]) : wave_reduce<false>(v[k]);
     if (lane == 0) sh[w][k] = a;
   }
   __syncthreads();

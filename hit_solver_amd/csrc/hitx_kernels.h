@@ -84,17 +84,6 @@ __device__ __forceinline__ T lane_from(T v, int src) {
   return __shfl(v, src & 63, 64);
 }
 
-// Sum (or max when MAX) over the 64 lanes of a wave; every lane gets it.
-template <bool MAX>
-__device__ __forceinline__ double wave_reduce(double a) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(a, off, 64);
-    a = MAX ? fmax(a, o) : a + o;
-  }
-  return a;
-}
-
 // Per-stage reductions that gate the sensor and the flux split
 // (OOP/spatial_operator.py:148 eps, :271 div_rms, :279 vort_rms, :332 alpha),
 // plus the HIT closure's cooling mean pressure (OOP/hit2d.py:423) and drag
