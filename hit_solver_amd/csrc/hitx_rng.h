@@ -24,7 +24,11 @@
 //      value n-1 records how many words the n values consumed.
 // A spill of kRngEntries words or more, or too few values in the words
 // evaluated, sets a flag, and one thread then regenerates the whole request
-// sequentially.  Both forms give the same numbers.
+// sequentially (tens of milliseconds for 2^18 values).  Both forms give the
+// same numbers.  A spill of s words needs a tail attempt (1 in 256) of at
+// least s + 1 words ending past the chunk, i.e. (s - 1)/2 rejected tail tries
+// (about 6% each): with 8 entry offsets that happened about once per 7e8
+// normals, with 16 it is about 1e-5 times as rare.
 #pragma once
 
 #include "hitx_device.h"
@@ -35,7 +39,7 @@ namespace hitx {
 typedef unsigned __int128 u128;
 
 constexpr int kRngChunk = 32;    // words per chunk (one thread)
-constexpr int kRngEntries = 8;   // entry offsets tracked per chunk
+constexpr int kRngEntries = 16;  // entry offsets tracked per chunk
 constexpr int kRngScan = 256;    // threads of the scan block (four wavefronts: a request holds few CU slots)
 constexpr uint64_t kMask52 = (1ull << 52) - 1;
 
@@ -223,14 +227,22 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
   }
 }
 
-// Entry e of a chunk's map held as one 16-byte load (8 uint16).
-static_assert(kRngEntries == 8, "a chunk's map is one 16-byte load");
-__device__ __forceinline__ uint32_t map_entry(const uint4& m, int e) {
-  const int h = e >> 1;
-  const uint32_t w = h == 0 ? m.x : (h == 1 ? m.y : (h == 2 ? m.z : m.w));
+// A chunk's map (16 uint16) held as two 16-byte loads; entry e of it.
+static_assert(kRngEntries == 16, "a chunk's map is two 16-byte loads");
+struct ChunkMap {
+  uint4 a, b;
+};
+__device__ __forceinline__ ChunkMap load_map(const RngJob& J, int c) {
+  const uint4* m = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(J.map)) + 2 * (size_t)c;
+  return {m[0], m[1]};
+}
+__device__ __forceinline__ uint32_t map_entry(const ChunkMap& m, int e) {
+  const uint4& q = e < 8 ? m.a : m.b;
+  const int h = (e >> 1) & 3;
+  const uint32_t w = h == 0 ? q.x : (h == 1 ? q.y : (h == 2 ? q.z : q.w));
   return (e & 1) ? (w >> 16) : (w & 0xffffu);
 }
-constexpr int kRngScanBatch = 16;  // chunk maps loaded together (one latency per batch)
+constexpr int kRngScanBatch = 8;  // chunk maps loaded together (one latency per batch)
 
 // Step 3 (one block of kRngScan threads per request).  A chunk's map sends
 // the entry offset e to (spill, count); spill 0xff marks an overflow.  Each
@@ -247,7 +259,6 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
   const int t = threadIdx.x;
   const int per = (J.nch + kRngScan - 1) / kRngScan;
   const int c0 = min(t * per, J.nch), c1 = min(c0 + per, J.nch);
-  const uint4* maps = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(J.map));
   uint8_t mx[kRngEntries];
   int mn[kRngEntries];
 #pragma unroll
@@ -256,10 +267,10 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
     mn[e] = 0;
   }
   for (int cb = c0; cb < c1; cb += kRngScanBatch) {
-    uint4 mv[kRngScanBatch];
+    ChunkMap mv[kRngScanBatch];
 #pragma unroll
     for (int j = 0; j < kRngScanBatch; ++j)
-      if (cb + j < c1) mv[j] = maps[cb + j];
+      if (cb + j < c1) mv[j] = load_map(J, cb + j);
 #pragma unroll
     for (int j = 0; j < kRngScanBatch; ++j) {
       if (cb + j < c1) {
@@ -313,10 +324,10 @@ __device__ __forceinline__ void rng_scan_body(const RngJob& J, int force_serial)
     cnt = sn[t - 1];
   }
   for (int cb = c0; cb < c1 && e < kRngEntries; cb += kRngScanBatch) {
-    uint4 mv[kRngScanBatch];
+    ChunkMap mv[kRngScanBatch];
 #pragma unroll
     for (int j = 0; j < kRngScanBatch; ++j)
-      if (cb + j < c1) mv[j] = maps[cb + j];
+      if (cb + j < c1) mv[j] = load_map(J, cb + j);
 #pragma unroll
     for (int j = 0; j < kRngScanBatch; ++j) {
       const int c = cb + j;

@@ -366,3 +366,24 @@ def test_batch_serial_noise_draws_agree():
         for k in range(4):
             e = rel_l2(host(b.q[i])[k], host(a.q[i])[k])
             assert e <= 1e-12, (i, k, e)
+
+
+def test_batch_noise_draws_stay_parallel():
+    """The noise stream of forcing seed 61 at 512^2 (default_rng(62)) holds a
+    ziggurat tail attempt that spills 8 words past a 32-word chunk in its
+    292nd request, which chunk maps of 8 entry offsets sent to the serial
+    path (about 60 ms); with 16 entry offsets every draw of the first 300
+    steps stays on the parallel path."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+
+    cfg = HIT2DConfig.production_512(nx=512, ny=512, forcing_seed=61, target_mach=0.25, mach_control_target=0.25,
+                                     dtype="fp32", tfinal=1e9)
+    b = BatchRunner([cfg])
+    print("[batch] 300 steps of seed 61 at 512^2, serial-path draws counted", flush=True)
+    serial = []
+    for s in range(300):
+        b.step()
+        if b.control()[0, 29] != 0.0:
+            serial.append(s)
+    assert serial == [], serial
