@@ -1565,22 +1565,27 @@ int batch_stage_sequence(hitx_batch* b, int st, hipStream_t s, hipEvent_t before
   const bool drag_now = DRAG && st < 3;
   b_stage_finalize<T, DRAG><<<dim3(n, drag_now ? 1 + kFinDragGroups : 1), kBlock, 0, s>>>(tab, st,
                                                                                      DRAG && b->stage_walk ? 1 : 0);
-  // stages 0-2: fork the second stream (drag row table, then the x-sweep
-  // unless it is fused into the update)
+  // the drag row table rides in the tile raw mask's launch (extra blocks);
+  // otherwise (other sensor forms) it runs on the second stream, forked here
+  const bool drows_in_raw = drag_now && b->split_sensor && b->raw_tile && b->any_per;
+  const int ndrows = drows_in_raw ? b->total[OFF_DROWS] : 0;
+  // stages 0-2: the second stream takes the drag row table (when not in the
+  // raw-mask launch) and the x-sweep (unless it is fused into the update)
   hipStream_t s2 = b->side2;
-  const bool fork = st < 3 && s2 != nullptr && (drag_now || !b->xfused);
-  if (fork) {
+  const bool fork = st < 3 && s2 != nullptr && ((drag_now && !drows_in_raw) || !b->xfused);
+  const bool early_fork = fork && drag_now && !drows_in_raw;
+  if (early_fork) {
     HIPCHK(hipEventRecord(b->sfork_ev, s));
     HIPCHK(hipStreamWaitEvent(s2, b->sfork_ev, 0));
   }
-  if (drag_now) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, fork ? s2 : s>>>(tab, offs, n, st);
+  if (drag_now && !drows_in_raw) b_drows<T><<<b->total[OFF_DROWS], kBlock, 0, fork ? s2 : s>>>(tab, offs, n, st);
   if (b->split_sensor) {
     // the tile raw mask writes the transposed state itself, so the dilation
     // is bit work only; after the column-walk raw mask the byte dilation
     // writes it (stages 0-2)
     if (b->raw_tile) {
-      if (b->any_per) b_rawtile<T, true><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
-      if (b->any_nonper) b_rawtile<T, false><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st);
+      if (b->any_per) b_rawtile<T, true><<<b->total[OFF_SENS] + ndrows, kBlock, 0, s>>>(tab, offs, n, st, ndrows);
+      if (b->any_nonper) b_rawtile<T, false><<<b->total[OFF_SENS], kBlock, 0, s>>>(tab, offs, n, st, 0);
       b_dilate_bits<T><<<b->total[OFF_DB], 64 * kDBWaves, 0, s>>>(tab, offs, n, st);
     } else {
       b_rawmask<T><<<b->total[OFF_SR], kBlock, 0, s>>>(tab, offs, n, st);

@@ -185,9 +185,17 @@ __global__ void __launch_bounds__(kBlock) b_rawmask(const Inst<T>* __restrict__ 
 
 template <class T, bool PER>
 __global__ void __launch_bounds__(kBlock) b_rawtile(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
-                                                    int ninst, int st) {
+                                                    int ninst, int st, int ndrows) {
   HITX_POST_STEP_EXIT
-  HITX_BATCH_TILE(OFF_SENS)
+  // the first ndrows blocks build the drag row table (stages 0-2 of a batch
+  // with drag: it reads the stage scalars the finalize wrote, as the raw mask
+  // does, so it rides in this launch instead of a launch on another stream)
+  if ((int)blockIdx.x < ndrows) {
+    HITX_BATCH_INST_AT(OFF_DROWS, (int)blockIdx.x)
+    drows_body<T>(I.P, I.scal + S_DRAG, I.drows, lb * kBlock + threadIdx.x);
+    return;
+  }
+  HITX_BATCH_INST_AT(OFF_SENS, xcd_remap((int)blockIdx.x - ndrows, (int)gridDim.x - ndrows))
   if (st == 3 && I.ctl[C_HVNOW] == 0.0) return;
   if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch
   const int ntx = (I.P.nx + kSTX - 1) / kSTX;
