@@ -290,8 +290,19 @@ __global__ void __launch_bounds__(64 * kSlabWaves) HITX_LINE_WAVE_ATTR(T, PER)
 
 // update tiles are the sensor's 32x16 tiles (the same block offsets)
 static_assert(kUTX == kSTX && kUTY == kSTY, "update and sensor tiles differ");
+// FP32 update tiles fit 8 wavefronts per SIMD (<= 64 VGPRs, four 512-thread
+// blocks and their 30 KB of LDS per CU) without spilling; FP64 keeps its
+// registers.
+template <class T>
+struct UpdateOcc {
+  static constexpr int waves = 1;
+};
+template <>
+struct UpdateOcc<float> {
+  static constexpr int waves = 8;
+};
 template <class T, bool PER>
-__global__ void __launch_bounds__(kUBlock) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
+__global__ void __launch_bounds__(kUBlock) __attribute__((amdgpu_waves_per_eu(UpdateOcc<T>::waves))) b_update(const Inst<T>* __restrict__ tab, const int* __restrict__ offs,
                                                     int ninst, int st) {
   HITX_BATCH_TILE(OFF_SENS)
   if (I.P.periodic != (PER ? 1 : 0)) return;  // the other periodicity's launch

@@ -1770,7 +1770,8 @@ __global__ void __launch_bounds__(kLineBlock, 4)
 constexpr int kUTX = 32, kUTY = 16, kUBlock = kUTX * kUTY;
 // LDS of an update tile of TX x TY nodes (TX * TY = kUBlock threads): the
 // primitives with their 2-node halo, the viscous fluxes with their 1-node
-// halo, the tile's y-derivative staged transposed.
+// halo, the tile's y-derivative staged transposed, the drag tables of the
+// tile's rows and columns.
 template <class T, int TX, int TY>
 struct UpdLds {
   static constexpr int VW = TX + 4, VH = TY + 4, VS = VW + 1;  // primitives tile
@@ -1778,6 +1779,9 @@ struct UpdLds {
   T U[VH * VS], V[VH * VS], Tt[VH * VS];
   T xx[FH * FS], yy[FH * FS], xy[FH * FS], ex[FH * FS], ey[FH * FS];
   T D[4][TX][TY + 1];
+  // the tile rows' drag table rows ([TY][10]), then its columns' twiddles of
+  // |kx| = 1, 2 ([TX][4])
+  double drag[TY * 10 + TX * 4];
 };
 // The fluxes at the FW x FH halo-ring nodes (the primitives in LDS); the
 // caller synchronises before and after.
@@ -1894,10 +1898,6 @@ __device__ __forceinline__ void update_tile(const Params<T>& P, int mode, int st
   // latency overlaps it
   T gx[4] = {T(0), T(0), T(0), T(0)}, a0[4] = {T(0), T(0), T(0), T(0)};
   T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0), f1r = T(0), f2r = T(0);
-  // drag low-pass tables: the node's row of the per-row table and the
-  // column twiddles of |kx| = 1, 2 (|kx| <= 2 for drag modes)
-  double dH[10], dtw[2][2];
-  const bool drag_node = in && mode == MODE_STAGE && P.drag_on;
   if (in && mode != MODE_VISC) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) gx[c] = at32(dfx + (size_t)c * N, idx);
@@ -1911,18 +1911,6 @@ __device__ __forceinline__ void update_tile(const Params<T>& P, int mode, int st
       }
     }
   }
-  if (drag_node) {
-    const double* H = drows + (size_t)cy * 10;
-#pragma unroll
-    for (int j = 0; j < 10; ++j) dH[j] = H[j];
-#pragma unroll
-    for (int k = 1; k <= 2; ++k) {
-      const int kk = k <= P.dkx ? k : 0;  // an in-range row when |kx| = k is absent
-      const double* tw = P.twx + ((size_t)(kk + P.kmx) * nx + cx) * 2;
-      dtw[k - 1][0] = tw[0];
-      dtw[k - 1][1] = tw[1];
-    }
-  }
   VTileLoad<T> vt;
   if (visc) vt = viscous_tile_load<T, TX, TY>(P, qs, x0, y0);
   T dsh[4];
@@ -1932,36 +1920,44 @@ __device__ __forceinline__ void update_tile(const Params<T>& P, int mode, int st
 #pragma unroll
     for (int c = 0; c < 4; ++c) dsh[c] = (rx < nx && ry < ny) ? at32(dfyT + (size_t)c * N, rx * ny + ry) : T(0);
   }
+  // drag low-pass tables (|kx| <= 2 for drag modes), one entry per thread
+  // for LDS: the per-row table's rows of the tile, then the column twiddles
+  // of |kx| = 1, 2 (cos, sin of |kx| = 1, then of 2)
+  constexpr int kDragTab = TY * 10 + TX * 4;
+  static_assert(kDragTab <= kUBlock, "one drag table entry per thread");
+  const bool drag = mode == MODE_STAGE && P.drag_on;
+  double dtab = 0.0;
+  if (drag && threadIdx.x < kDragTab) {
+    const bool clamp = !PER_ONLY && P.guard > 0;
+    const int i = threadIdx.x;
+    if (i < TY * 10) {
+      const int r = i / 10, j = i - r * 10;
+      int yr = min(y0 + r, ny - 1);  // rows past the grid: any in-range row (unused)
+      if (clamp) yr = clampi(yr, P.guard, ny - 1 - P.guard);
+      dtab = drows[(size_t)yr * 10 + j];
+    } else {
+      const int e = i - TY * 10, c = e >> 2, k = ((e >> 1) & 1) + 1;
+      int xc = min(x0 + c, nx - 1);
+      if (clamp) xc = clampi(xc, P.guard, nx - 1 - P.guard);
+      const int kk = k <= P.dkx ? k : 0;  // an in-range row when |kx| = k is absent
+      dtab = P.twx[((size_t)(kk + P.kmx) * nx + xc) * 2 + (e & 1)];
+    }
+  }
   // node-local source terms while the tile loads are in flight
-  T fsrc[4] = {T(0), T(0), T(0), T(0)}, dsrc[4] = {T(0), T(0), T(0), T(0)};
-  if (in && mode == MODE_STAGE && (P.force_on || P.drag_on)) {
+  T fsrc[4] = {T(0), T(0), T(0), T(0)};
+  if (in && mode == MODE_STAGE && P.force_on) {  // OOP/hit2d.py:264-276; f = (f_raw*scale)*alpha (OOP/forcing.py:91-112)
     const Prim<T> st = prim(P, s0, s1, s2, s3);
-    if (P.force_on) {  // OOP/hit2d.py:264-276; f = (f_raw*scale)*alpha (OOP/forcing.py:91-112)
-      const T f1 = (f1r * fsc) * fal, f2 = (f2r * fsc) * fal;
-      fsrc[1] = st.rho * f1;
-      fsrc[2] = st.rho * f2;
-      fsrc[3] = st.rho * madd(st.v, f2, st.u * f1);
-    }
-    if (P.drag_on) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
-      double lx = dH[0], ly = dH[5];
-#pragma unroll
-      for (int k = 1; k <= 2; ++k) {
-        if (k > P.dkx) break;
-        const double c = dtw[k - 1][0], sn = dtw[k - 1][1];
-        lx += dH[2 * k - 1] * c - dH[2 * k] * sn;
-        ly += dH[5 + 2 * k - 1] * c - dH[5 + 2 * k] * sn;
-      }
-      const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
-      dsrc[1] = d1;
-      dsrc[2] = d2;
-      dsrc[3] = madd(st.v, d2, st.u * d1);
-    }
+    const T f1 = (f1r * fsc) * fal, f2 = (f2r * fsc) * fal;
+    fsrc[1] = st.rho * f1;
+    fsrc[2] = st.rho * f2;
+    fsrc[3] = st.rho * madd(st.v, f2, st.u * f1);
   }
   if (visc) viscous_tile_store<T, TX, TY>(P, vt, L);
   if (mode != MODE_VISC) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) L.D[c][threadIdx.x / TY][threadIdx.x % TY] = dsh[c];
   }
+  if (drag && threadIdx.x < kDragTab) L.drag[threadIdx.x] = dtab;
   __syncthreads();
   if (visc) {
     viscous_fluxes<T, TX, TY>(P, L);
@@ -1998,9 +1994,22 @@ __device__ __forceinline__ void update_tile(const Params<T>& P, int mode, int st
 #pragma unroll
     for (int c = 1; c < 4; ++c) Lv[c] = Lv[c] + fsrc[c];
   }
-  if (P.drag_on) {
+  if (drag) {  // OOP/hit2d.py:370-399, low-pass field from the per-row table
+    const Prim<T> st = prim(P, s0, s1, s2, s3);
+    const double* H = L.drag + ty * 10;
+    const double* tw = L.drag + TY * 10 + tx * 4;
+    double lx = H[0], ly = H[5];
 #pragma unroll
-    for (int c = 1; c < 4; ++c) Lv[c] = Lv[c] + dsrc[c];
+    for (int k = 1; k <= 2; ++k) {
+      if (k > P.dkx) break;
+      const double c = tw[2 * k - 2], sn = tw[2 * k - 1];
+      lx += H[2 * k - 1] * c - H[2 * k] * sn;
+      ly += H[5 + 2 * k - 1] * c - H[5 + 2 * k] * sn;
+    }
+    const T d1 = P.neg_drag * (T)lx, d2 = P.neg_drag * (T)ly;
+    Lv[1] = Lv[1] + d1;
+    Lv[2] = Lv[2] + d2;
+    Lv[3] = Lv[3] + madd(st.v, d2, st.u * d1);
   }
   if (P.cool_on) Lv[3] = Lv[3] + (T)scal[S_COOL];  // OOP/hit2d.py:402-426
   T o[4];
