@@ -734,7 +734,13 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
   constexpr int EX = kSTX + 2, EY = kSTY + 2, NE = EX * EY, NIT = (NE + kBlock - 1) / kBlock;
   constexpr int PF = 2;  // partial rows per thread loaded with the tile
   __shared__ T Lr[EX * EY], Lu[EX * EY], Lv[EX * EY], Lp[EX * EY], Le[EX * EY];
-  __shared__ T st[4 * kSTX * (kSTY + 1)];  // the tile's (enforced) state, transposed, when qT is wanted
+  // the tile's (enforced) state, transposed, when qT is wanted: column cc
+  // holds its 16 rows at cc*16 + (r ^ ((cc >> 1) & 15)) (a row of 32 columns
+  // and a column of 16 rows each touch 32 distinct banks; no padding, so
+  // eight 4-wavefront blocks fit a CU in FP32)
+  static_assert(kSTY == 16, "the swizzle spans 16 rows");
+  __shared__ T st[4 * kSTX * kSTY];
+  auto sw = [](int cc, int r) { return cc * kSTY + (r ^ ((cc >> 1) & 15)); };
   const int nx = P.nx, ny = P.ny, N = nx * ny;
   const int x0 = bx * kSTX, y0 = by * kSTY;
   constexpr bool per = PER;  // periodic grids take no clamps and no end tests
@@ -780,11 +786,11 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
       Le[t] = dv(pr.p, pr.rho * P.gm1);
       const int r = t / EX - 1, cc = t - (r + 1) * EX - 1;
       if (qT != nullptr && r >= 0 && r < kSTY && cc >= 0 && cc < kSTX) {
-        const int o = cc * (kSTY + 1) + r;
+        const int o = sw(cc, r);
         st[o] = a;
-        st[kSTX * (kSTY + 1) + o] = b;
-        st[2 * kSTX * (kSTY + 1) + o] = c;
-        st[3 * kSTX * (kSTY + 1) + o] = d;
+        st[kSTX * kSTY + o] = b;
+        st[2 * kSTX * kSTY + o] = c;
+        st[3 * kSTX * kSTY + o] = d;
       }
     }
   }
@@ -821,10 +827,10 @@ __device__ __forceinline__ void rawtile_body(const Params<T>& P, const T* __rest
       const int c = t / kSTY, r = t - c * kSTY;
       const int gy = y0 + r, gx = x0 + c;
       if (gy < ny && gx < nx) {
-        const int o = c * (kSTY + 1) + r;
+        const int o = sw(c, r);
         const size_t g = (size_t)gx * ny + gy;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) qT[k * (size_t)N + g] = st[k * kSTX * (kSTY + 1) + o];
+        for (int k = 0; k < 4; ++k) qT[k * (size_t)N + g] = st[k * kSTX * kSTY + o];
       }
     }
   }
