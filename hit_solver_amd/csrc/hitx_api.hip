@@ -50,13 +50,13 @@ int default_rbp(int nx, int ny) { return std::max(8, std::min(64, ((nx + 63) / 6
 
 // Workspace of one normal-variate request (see hitx_rng.h).
 struct RngLayout {
-  size_t vals, att, map, entry, base, stat, total;
+  size_t cst, att, map, entry, base, stat, total;
 };
 RngLayout rng_layout(long long n) {
   const size_t nch = (size_t)rng_chunks(n), words = nch * kRngChunk;
   RngLayout L;
-  L.vals = 0;
-  L.att = align256(L.vals + words * sizeof(double));
+  L.cst = 0;
+  L.att = align256(L.cst + nch * 2 * sizeof(uint64_t));
   L.map = align256(L.att + words);
   L.entry = align256(L.map + nch * kRngEntries * sizeof(uint16_t));
   L.base = align256(L.entry + nch);
@@ -68,7 +68,7 @@ RngJob make_rngjob(uint8_t* ws, const RngLayout& L, uint64_t* state, double* out
   RngJob J;
   J.state = state;
   J.out = out;
-  J.vals = reinterpret_cast<double*>(ws + L.vals);
+  J.cst = reinterpret_cast<uint64_t*>(ws + L.cst);
   J.att = ws + L.att;
   J.map = reinterpret_cast<uint16_t*>(ws + L.map);
   J.entry = ws + L.entry;

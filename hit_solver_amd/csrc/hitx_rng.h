@@ -20,8 +20,10 @@
 //      the spill into the next chunk and the number of values;
 //   3. these small maps are composed by a block-wide scan, which fixes the
 //      true entry offset and the index of the first value of every chunk;
-//   4. each chunk writes its values at their indices; the thread that writes
-//      value n-1 records how many words the n values consumed.
+//   4. each chunk regenerates its words from its first word's LCG state
+//      (stored by step 1) and writes the values of the attempts on its chain
+//      at their indices; the thread that writes value n-1 records how many
+//      words the n values consumed.
 // A spill of kRngEntries words or more, or too few values in the words
 // evaluated, sets a flag, and one thread then regenerates the whole request
 // sequentially (tens of milliseconds for 2^18 values).  Both forms give the
@@ -155,11 +157,22 @@ __device__ __forceinline__ ZigAttempt zig_attempt(uint64_t w, Pcg g) {
   return a;
 }
 
+// The value of an attempt that yields one (word w, length len; g positioned
+// after w): the layer's x for the first test and the wedge, the tail's value
+// (replayed) for a base-layer attempt that went to the tail.
+__device__ __forceinline__ double zig_value(uint64_t w, int len, const Pcg& g) {
+  const int idx = (int)(w & 0xff);
+  if (idx == 0 && len > 1) return zig_attempt<true>(w, g).v;
+  const uint64_t r = w >> 8;
+  const double x = (double)((r >> 1) & kMask52) * kZigWi[idx];
+  return (r & 1ull) ? -x : x;
+}
+
 // Buffers of one request of n normals (all in device memory).
 struct RngJob {
   GPtr<uint64_t> state;  // s hi, s lo, inc hi, inc lo; advanced past the n values
   GPtr<double> out;      // n values
-  GPtr<double> vals;     // [kRngChunk][nch] attempt values (transposed: coalesced)
+  GPtr<uint64_t> cst;    // [nch][2] LCG state (hi, lo) before each chunk's first word
   GPtr<uint8_t> att;     // [nch][kRngChunk] attempt length | yields << 7
   GPtr<uint16_t> map;    // [nch][kRngEntries] spill | count << 8 (spill 0xff: overflow)
   GPtr<uint8_t> entry;   // [nch] resolved entry offset
@@ -190,12 +203,12 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
   __syncthreads();
   if (c >= J.nch) return;
   g.s = kPcgJumps.a[threadIdx.x] * (((u128)sb[0] << 64) | (u128)sb[1]) + g.inc * kPcgJumps.m[threadIdx.x];
+  J.cst[2 * (size_t)c] = (uint64_t)(g.s >> 64);
+  J.cst[2 * (size_t)c + 1] = (uint64_t)g.s;
   for (int j = 0; j < kRngChunk; ++j) {
     const uint64_t w = g.next();
     const ZigAttempt a = zig_attempt<true>(w, g);
-    const uint8_t code = (uint8_t)(a.len | (a.out ? 0x80 : 0));
-    J.vals[(size_t)j * J.nch + c] = a.v;
-    lds[j] = code;
+    lds[j] = (uint8_t)(a.len | (a.out ? 0x80 : 0));
   }
   {  // the chunk's codes, contiguous: two 16-byte stores
     uint4* dst = reinterpret_cast<uint4*>(static_cast<uint8_t*>(J.att) + (size_t)c * kRngChunk);
@@ -371,17 +384,19 @@ __device__ __forceinline__ bool rng_scatter_stage(const RngJob& J, int c, double
     const uint32_t cw[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
     int nxt = J.entry[c];
     k = J.base[c];
-    // the chunk's attempt values, every load in flight together
-    double vv[kRngChunk];
-#pragma unroll
-    for (int q = 0; q < kRngChunk; ++q) vv[q] = J.vals[(size_t)q * J.nch + c];
+    // the chunk's words regenerated from its first word's state
+    Pcg g;
+    g.s = ((u128)J.cst[2 * (size_t)c] << 64) | (u128)J.cst[2 * (size_t)c + 1];
+    g.inc = ((u128)J.state[2] << 64) | (u128)J.state[3];
 #pragma unroll
     for (int q = 0; q < kRngChunk; ++q) {
+      const uint64_t w = g.next();
       if (q == nxt) {
         const uint32_t code = (cw[q >> 2] >> ((q & 3) * 8)) & 0xffu;
         const int len = (int)(code & 0x7f);
         if (code & 0x80) {
-          if (k < J.n) lds[k - lo] = vv[q];
+          const double v = zig_value(w, len, g);
+          if (k < J.n) lds[k - lo] = v;
           if (k == J.n - 1) J.stat[1] = c * kRngChunk + q + len;  // words consumed
           ++k;
         }
