@@ -249,11 +249,14 @@ __device__ __forceinline__ ChunkMap load_map(const RngJob& J, int c) {
   const uint4* m = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(J.map)) + 2 * (size_t)c;
   return {m[0], m[1]};
 }
+// (selects between registers: a data-dependent index into the map would
+// put the maps in scratch memory)
 __device__ __forceinline__ uint32_t map_entry(const ChunkMap& m, int e) {
-  const uint4& q = e < 8 ? m.a : m.b;
+  const bool hi = e >= 8;
+  const uint32_t x = hi ? m.b.x : m.a.x, y = hi ? m.b.y : m.a.y, z = hi ? m.b.z : m.a.z, w = hi ? m.b.w : m.a.w;
   const int h = (e >> 1) & 3;
-  const uint32_t w = h == 0 ? q.x : (h == 1 ? q.y : (h == 2 ? q.z : q.w));
-  return (e & 1) ? (w >> 16) : (w & 0xffffu);
+  const uint32_t v = h == 0 ? x : (h == 1 ? y : (h == 2 ? z : w));
+  return (e & 1) ? (v >> 16) : (v & 0xffffu);
 }
 constexpr int kRngScanBatch = 8;  // chunk maps loaded together (one latency per batch)
 
