@@ -101,6 +101,7 @@ SIGNATURES = {
     "hitx_batch_serial_draws": (c_i32, [c_vp, c_i32]),
     "hitx_batch_timing_enable": (c_i32, [c_vp, c_i32]),
     "hitx_batch_timing_read": (c_i32, [c_vp, ctypes.POINTER(c_dbl), ctypes.POINTER(c_i32)]),
+    "hitx_batch_time_updates": (c_i32, [c_vp, c_i32, c_vp, ctypes.POINTER(c_dbl)]),
     "hitx_batch_step": (c_i32, [c_vp, c_vp]),
     "hitx_timing_read": (c_i32, [c_vp, ctypes.POINTER(c_dbl), ctypes.POINTER(c_i32)]),
 }

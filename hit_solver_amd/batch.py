@@ -363,6 +363,17 @@ class BatchRunner:
         check(lib().hitx_batch_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def time_updates(self, reps: int = 5):
+        """Average stage-update kernel milliseconds: the three stage updates
+        of every instance launched reps times back to back between two events
+        (per periodicity present).  The state is not a solution afterwards."""
+        ms = ctypes.c_double()
+        check(lib().hitx_batch_time_updates(self.handle, int(reps), self._stream(), ctypes.byref(ms)))
+        # forced HIT instances are periodic, the Riemann problems not: one
+        # update launch per periodicity present
+        launches = 3 * int(reps) * (int(any(self.is_hit)) + int(not all(self.is_hit)))
+        return ms.value / launches, launches
+
     def autotune(self, candidates=None, steps: int = 3):
         """Time a few steps per candidate (rb, rbp, line_form, sensor_form)
         on a copy of the batch's device state, restore the state and keep the

@@ -1744,6 +1744,31 @@ int batch_step_t(hitx_batch* b, hipStream_t s) {
 }
 
 template <class T>
+int batch_time_updates_t(hitx_batch* b, int reps, hipStream_t s, double* total_ms) {
+  const Inst<T>* tab = btab<T>(b);
+  const int* offs = boffs(b);
+  const int n = b->n;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    for (int st = 0; st < 3; ++st) {
+      if (b->any_per) b_update<T, true><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
+      if (b->any_nonper) b_update<T, false><<<b->total[OFF_SENS], kUBlock, 0, s>>>(tab, offs, n, st);
+    }
+  LAUNCHCHK();
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *total_ms = ms;
+  return HITX_OK;
+}
+
+template <class T>
 int batch_fixed_dt_t(hitx_batch* b, hipStream_t s) {
   const Inst<T>* tab = btab<T>(b);
   b_step_reduce<T><<<b->total[OFF_RED], kBlock, 0, s>>>(tab, boffs(b), b->n);
@@ -1970,6 +1995,15 @@ int hitx_batch_timing_read(hitx_batch* b, double* total_ms, int32_t* launches) {
   *launches = (int32_t)(b->ev_used / 2);
   b->ev_used = 0;
   return HITX_OK;
+}
+
+int hitx_batch_time_updates(hitx_batch* b, int32_t reps, void* stream, double* total_ms) {
+  if (!b || !total_ms) return fail(HITX_EINVAL, "null argument");
+  if (!b->ws) return fail(HITX_EINVAL, "batch not bound to a workspace");
+  if (reps < 1) return fail(HITX_EINVAL, "reps must be positive");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->esize == 8 ? batch_time_updates_t<double>(b, reps, s, total_ms)
+                       : batch_time_updates_t<float>(b, reps, s, total_ms);
 }
 
 int hitx_batch_serial_draws(hitx_batch* b, int32_t on) {

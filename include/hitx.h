@@ -423,6 +423,13 @@ int hitx_batch_stage_form(hitx_batch* batch, int32_t form);
  * the summed milliseconds and the launch count since, and clears them. */
 int hitx_batch_timing_enable(hitx_batch* batch, int32_t on);
 int hitx_batch_timing_read(hitx_batch* batch, double* total_ms, int32_t* launches);
+/* Stage-update kernel time (no reference counterpart; measurement support):
+ * launches the three stage updates of every instance reps times back to back
+ * on the stream between two events and returns the milliseconds (3*reps
+ * launches per periodicity present).  The updates run on whatever the stage
+ * buffers hold and the last stage may write the state in place, so the
+ * batch's state is not a solution afterwards. */
+int hitx_batch_time_updates(hitx_batch* batch, int32_t reps, void* stream, double* total_ms);
 /* Device pointer of one of instance i's buffers inside the bound workspace. */
 int hitx_batch_buffer(const hitx_batch* batch, int32_t i, int32_t which, void** ptr);
 /* Fixed dt of every instance from its current state (Riemann loop). */
