@@ -522,9 +522,9 @@ __global__ void __launch_bounds__(kRngScan) b_rng_scan(const Inst<T>* __restrict
 // b_fproject projects it.
 template <class T>
 __device__ __forceinline__ void rng_scatter_project_body(const Params<T>& P, const RngJob& J, int c, double* lds,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ part, const double* Wi) {
   long long lo, hi;
-  if (!rng_scatter_stage(J, c, lds, lo, hi)) return;
+  if (!rng_scatter_stage(J, c, lds, lo, hi, Wi)) return;
   const int lane = threadIdx.x & 63;
   const int nx = P.nx, nf = P.nforce, fk = P.fkx;
   int mkx = 0, mky = 0;
@@ -599,11 +599,14 @@ template <class T>
 __global__ void __launch_bounds__(kRngScatterThreads) b_rng_scatter(const Inst<T>* __restrict__ tab,
                                                                     const int* __restrict__ offs, int ninst, int pre) {
   __shared__ double lds[kRngScatterThreads / 64][64 * kRngChunk];
+  __shared__ double sWi[256];  // the layer table of the values, in LDS
   HITX_BATCH_INST(OFF_RNGS)
-  if (rng_skip(I.ctl, I.kpar, pre)) return;
+  if (rng_skip(I.ctl, I.kpar, pre)) return;  // (uniform over the block)
+  for (int k = threadIdx.x; k < 256; k += kRngScatterThreads) sWi[k] = kZigWi[k];
+  __syncthreads();
   const int c = lb * kRngScatterThreads + threadIdx.x;
   rng_scatter_project_body<T>(I.P, I.R, c, lds[threadIdx.x >> 6],
-                              I.fpart + (size_t)((c >> 6)) * 2 * I.P.nforce);
+                              I.fpart + (size_t)((c >> 6)) * 2 * I.P.nforce, sWi);
 }
 
 template <class T>

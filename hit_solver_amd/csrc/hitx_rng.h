@@ -43,6 +43,7 @@ typedef unsigned __int128 u128;
 constexpr int kRngChunk = 32;    // words per chunk (one thread)
 constexpr int kRngEntries = 16;  // entry offsets tracked per chunk
 constexpr int kRngScan = 256;    // threads of the scan block (four wavefronts: a request holds few CU slots)
+constexpr int kRngAttemptThreads = 256;  // threads of an attempt block (one chunk each)
 constexpr uint64_t kMask52 = (1ull << 52) - 1;
 
 __device__ __forceinline__ u128 pcg_mult() {
@@ -118,16 +119,18 @@ struct ZigAttempt {
 // on the local copy.  CAP bounds the tail loop so that the length fits the
 // 7-bit code of the parallel form (a tail run that long is reported as a
 // spill and the request is regenerated serially without the bound).
+// Ki, Wi: the layer tables (the device tables, or copies in LDS).
 template <bool CAP>
-__device__ __forceinline__ ZigAttempt zig_attempt(uint64_t w, Pcg g) {
+__device__ __forceinline__ ZigAttempt zig_attempt(uint64_t w, Pcg g, const uint64_t* Ki = kZigKi,
+                                                  const double* Wi = kZigWi) {
   const int idx = (int)(w & 0xff);
   const uint64_t r = w >> 8;
   const uint64_t rabs = (r >> 1) & kMask52;
-  double x = (double)rabs * kZigWi[idx];
+  double x = (double)rabs * Wi[idx];
   if (r & 1ull) x = -x;
   ZigAttempt a;
   a.v = x;
-  if (rabs < kZigKi[idx]) {
+  if (rabs < Ki[idx]) {
     a.len = 1;
     a.out = true;
     return a;
@@ -160,11 +163,11 @@ __device__ __forceinline__ ZigAttempt zig_attempt(uint64_t w, Pcg g) {
 // The value of an attempt that yields one (word w, length len; g positioned
 // after w): the layer's x for the first test and the wedge, the tail's value
 // (replayed) for a base-layer attempt that went to the tail.
-__device__ __forceinline__ double zig_value(uint64_t w, int len, const Pcg& g) {
+__device__ __forceinline__ double zig_value(uint64_t w, int len, const Pcg& g, const double* Wi = kZigWi) {
   const int idx = (int)(w & 0xff);
   if (idx == 0 && len > 1) return zig_attempt<true>(w, g).v;
   const uint64_t r = w >> 8;
-  const double x = (double)((r >> 1) & kMask52) * kZigWi[idx];
+  const double x = (double)((r >> 1) & kMask52) * Wi[idx];
   return (r & 1ull) ? -x : x;
 }
 
@@ -193,6 +196,12 @@ __host__ __device__ inline int rng_chunks(long long n) { return (rng_words(n) + 
 // lds: the thread's 3 * kRngChunk bytes (codes, chain spills, chain counts).
 __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_t* lds) {
   __shared__ uint64_t sb[2];
+  // the layer tables in LDS: every word is tested against them
+  __shared__ uint64_t sKi[256];
+  __shared__ double sWi[256];
+  static_assert(kRngAttemptThreads == 256, "one table entry per thread");
+  sKi[threadIdx.x] = kZigKi[threadIdx.x];
+  sWi[threadIdx.x] = kZigWi[threadIdx.x];
   const int c = c0 + threadIdx.x;
   Pcg g = pcg_load(J.state);
   if (threadIdx.x == 0) {
@@ -207,7 +216,7 @@ __device__ __forceinline__ void rng_attempt_body(const RngJob& J, int c0, uint8_
   J.cst[2 * (size_t)c + 1] = (uint64_t)g.s;
   for (int j = 0; j < kRngChunk; ++j) {
     const uint64_t w = g.next();
-    const ZigAttempt a = zig_attempt<true>(w, g);
+    const ZigAttempt a = zig_attempt<true>(w, g, sKi, sWi);
     lds[j] = (uint8_t)(a.len | (a.out ? 0x80 : 0));
   }
   {  // the chunk's codes, contiguous: two 16-byte stores
@@ -371,7 +380,7 @@ constexpr int kRngScatterThreads = 128;
 // Stages the wave's values [lo, hi) in lds (value i at i - lo); false when
 // the wave has nothing to do (serial path, chunks past the request).
 __device__ __forceinline__ bool rng_scatter_stage(const RngJob& J, int c, double* lds, long long& lo_out,
-                                                  long long& hi_out) {
+                                                  long long& hi_out, const double* Wi = kZigWi) {
   if (J.stat[0] != 0) return false;
   const int lane = threadIdx.x & 63;
   const int c0 = c - lane;  // chunk of lane 0
@@ -398,7 +407,7 @@ __device__ __forceinline__ bool rng_scatter_stage(const RngJob& J, int c, double
         const uint32_t code = (cw[q >> 2] >> ((q & 3) * 8)) & 0xffu;
         const int len = (int)(code & 0x7f);
         if (code & 0x80) {
-          const double v = zig_value(w, len, g);
+          const double v = zig_value(w, len, g, Wi);
           if (k < J.n) lds[k - lo] = v;
           if (k == J.n - 1) J.stat[1] = c * kRngChunk + q + len;  // words consumed
           ++k;
@@ -450,8 +459,6 @@ __device__ __forceinline__ unsigned long long rng_finish_body(const RngJob& J) {
   J.stat[0] = 0;
   return words;
 }
-
-constexpr int kRngAttemptThreads = 256;
 
 static_assert(kRngAttemptThreads <= 256, "jump table covers 256 threads");
 __global__ void __launch_bounds__(kRngAttemptThreads) k_rng_attempts(RngJob J) {

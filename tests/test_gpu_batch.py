@@ -387,3 +387,25 @@ def test_batch_noise_draws_stay_parallel():
         if b.control()[0, 29] != 0.0:
             serial.append(s)
     assert serial == [], serial
+
+
+def test_batch_time_updates_counts_launches():
+    """The update timing entry point launches the three stage updates per
+    repetition and periodicity present and returns a positive time per launch;
+    a mixed batch (periodic HIT and non-periodic Riemann) launches both
+    update kernels."""
+    from hit_solver_amd.batch import BatchRunner
+    from hit_solver_amd.hit2d import HIT2DConfig
+    from hit_solver_amd.riemann import RiemannConfig3
+
+    hit = HIT2DConfig(nx=64, ny=64, cfl=0.05, initial_kmin=3, initial_kmax=5, forcing_kmin=3.0, forcing_kmax=5.0,
+                      viscosity=7.5e-4, tfinal=1.0)
+    rie = RiemannConfig3(nx=64, ny=64, shear_threshold=None, progress_every=10**9)
+    for cfgs, kinds in (([hit], 1), ([hit, rie], 2)):
+        b = BatchRunner(cfgs)
+        b.step()
+        per_launch_ms, launches = b.time_updates(reps=2)
+        assert launches == 3 * 2 * kinds
+        assert np.isfinite(per_launch_ms) and 0.0 < per_launch_ms < 50.0, per_launch_ms
+        with pytest.raises(Exception):
+            b.time_updates(reps=0)
