@@ -162,10 +162,27 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
     // row y with pv, cu, nn the primitives of rows y-1, y, y+1, cr the state
     // as given of row y; the loads of row y+2 go to nr (in flight during the
     // row) and its primitives to pn (the set row y-1 held)
+    // the drag twiddles cos/sin(ky y) of |ky| = 1, 2 for the current row
+    // (twr) and the next (twn): wave-uniform loads issued a row ahead, so
+    // their latency overlaps the row instead of stalling it
+    double twr[4] = {0.0, 0.0, 0.0, 0.0}, twn[4] = {0.0, 0.0, 0.0, 0.0};
+    auto tw_load = [&](int y, double (&t)[4]) {
+      if (!DRAG) return;
+#pragma unroll
+      for (int k = 1; k <= 2; ++k) {
+        if (k <= P.dky) {
+          const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + y) * 2;
+          t[2 * k - 2] = ty[0];
+          t[2 * k - 1] = ty[1];
+        }
+      }
+    };
+    tw_load(W.ya, twr);
     auto row = [&](int y, Prim<T>& pv, const Prim<T>& cu, const Prim<T>& nn, const Raw& cr, Raw& nr,
                    Prim<T>& pn) {
       const bool more = y + 1 < W.yb;
       if (more) load4(q, N, row_of(y + 2) * nx + W.xl, nr.a, nr.b, nr.c, nr.d);
+      if (more) tw_load(y + 1, twn);
       const T ul = lane_step(cu.u, -1), ur = lane_step(cu.u, 1);
       const T vl = lane_step(cu.v, -1), vr = lane_step(cu.v, 1);
       const T c0 = cr.a, c1 = cr.b, c2 = cr.c, c3 = cr.d;
@@ -209,8 +226,7 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
 #pragma unroll
             for (int k = 1; k <= 2; ++k) {
               if (k <= P.dky) {
-                const double* ty = P.twy + ((size_t)(k + P.kmy) * ny + y) * 2;
-                const double cy = ty[0], sy = ty[1];
+                const double cy = twr[2 * k - 2], sy = twr[2 * k - 1];
                 bs[0][2 * k - 1] = fma(fb, cy, bs[0][2 * k - 1]);
                 bs[0][2 * k] = fma(fb, sy, bs[0][2 * k]);
                 bs[1][2 * k - 1] = fma(fc, cy, bs[1][2 * k - 1]);
@@ -221,6 +237,8 @@ __device__ __forceinline__ void stage_reduce_body(const Params<T>& P, const T* _
         }
       }
       if (more) pn = prim_e(nr.a, nr.b, nr.c, nr.d);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) twr[k] = twn[k];
     };
     for (int y = W.ya; y < W.yb; y += 3) {
       row(y, P0, P1, P2, R1, R0, P0);
